@@ -1,0 +1,120 @@
+/* dmtz.h — C ABI of the MI355X topology-correction path.
+ *
+ * The reference (YuxiaoLi1234/DMTz, main_3d.cu) has no library interface: its hot path is the
+ * device-resident span of main() from the host-to-device copy of f and g (main_3d.cu:6384-6393)
+ * to the device-to-host copy of the corrected field (main_3d.cu:7156), the span it times as
+ * additional_time (main_3d.cu:6384, 7152-7155). Its inputs arrive through argv and raw files
+ * (main_3d.cu:6260-6271, 153-181, 6300, 6364) and its state lives in __device__ globals
+ * (main_3d.cu:43-129). Each entry point below replaces one piece of that span; the comment on
+ * each names the reference lines it stands in for.
+ *
+ * Conventions
+ *   - Fields are contiguous, x fastest: index = x + y*nx + z*nx*ny (main_3d.cu:1945-1947).
+ *   - Grids need nx >= 3, ny >= 2, nz >= 2 and nx*ny*nz < 2^31; values must be finite.
+ *   - Return codes: 0 = converged / OK, 1 = not converged within max_sweeps,
+ *     negative = error (see DMTZ_E_*); dmtz_last_error() gives a message. Nothing calls exit().
+ *   - A context owns one HIP stream and its scratch memory; calls on one context are
+ *     host-synchronous and a context must not be used from two threads at once.
+ */
+#ifndef DMTZ_H
+#define DMTZ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DMTZ_OK 0
+#define DMTZ_NOT_CONVERGED 1
+#define DMTZ_E_ARG (-1)
+#define DMTZ_E_HIP (-2)
+#define DMTZ_E_CAPACITY (-3)
+#define DMTZ_E_UNSUPPORTED (-4)
+#define DMTZ_E_WALK (-5)
+
+/* The reference's positional flags (main_3d.cu:6264-6271) plus a sweep cap.
+ * max_sweeps <= 0 selects the default cap (100000). mode: 0 = plain loop. */
+typedef struct dmtz_flags {
+    int32_t preserve_saddles;    /* argv[4]: fix 1- and 2-saddle mismatches            */
+    int32_t preserve_vpath;      /* argv[5]: also match separatrices (V-paths)          */
+    int32_t preserve_geometry;   /* argv[6]: 1 = path geometry, 0 = endpoint labels      */
+    int32_t filtration;          /* argv[7]: pin f-critical cells to f - eps first       */
+    int32_t preserve_connectors; /* argv[8]: 2-saddle walls (not supported on device)   */
+    int32_t preserve_type;       /* argv[9]: nonzero = every edit snaps to f - eps       */
+    int32_t edit_type;           /* argv[10]: 0 = halve the gap, 1 = -2^-k eps steps     */
+    int32_t max_sweeps;          /* cap on correction sweeps (the reference has none)    */
+    int32_t mode;                /* 0 = plain                                           */
+    int32_t reserved;
+} dmtz_flags;
+
+/* Mismatch counters count_f_* (main_3d.cu:68-75) in the order min, 1-saddle, 2-saddle, max,
+ * descending path, ascending path, descending wall, ascending wall. */
+typedef struct dmtz_counts {
+    int64_t min, saddle1, saddle2, max, ds_vpath, as_vpath, ds_wall, as_wall;
+} dmtz_counts;
+
+typedef struct dmtz_ctx dmtz_ctx;
+
+/* Context with its own stream on HIP device `device`. Replaces the global device state of
+ * main_3d.cu:43-129 and the allocations of main_3d.cu:6465-6714. */
+int dmtz_ctx_create(dmtz_ctx** out, int device);
+void dmtz_ctx_destroy(dmtz_ctx* ctx);
+const char* dmtz_last_error(const dmtz_ctx* ctx);
+
+/* eps = rel * (max f - min f) and range = max f - min f, main_3d.cu:6301-6309. */
+int dmtz_eps_from_rel_f32(const float* f, int64_t n, double rel, double* eps_out, double* range_out);
+int dmtz_eps_from_rel_f64(const double* f, int64_t n, double rel, double* eps_out, double* range_out);
+
+/* Full correction of g (host pointers): main_3d.cu:6384-7156 (H2D copy, gradient of f and g,
+ * mismatch detection, the sweep loop of main_3d.cu:6784-7149, D2H copy of the corrected field).
+ * g_inout is corrected in place. counter_out (nullable, nx*ny*nz bytes) receives delta_counter
+ * (main_3d.cu:84, values 1..6). sweeps_out (nullable) receives the number of sweeps. */
+int dmtz_correct_f32(dmtz_ctx* ctx, const float* f, float* g_inout, int64_t nx, int64_t ny, int64_t nz,
+                     double eps, double range, const dmtz_flags* flags, dmtz_counts* out,
+                     uint8_t* counter_out, int32_t* sweeps_out);
+int dmtz_correct_f64(dmtz_ctx* ctx, const double* f, double* g_inout, int64_t nx, int64_t ny, int64_t nz,
+                     double eps, double range, const dmtz_flags* flags, dmtz_counts* out,
+                     uint8_t* counter_out, int32_t* sweeps_out);
+
+/* Same loop on device-resident fields (the span main_3d.cu:6394-7155, after the H2D copy):
+ * d_f, d_g are device pointers on the context's device; d_counter_out is a nullable device
+ * pointer. No host copies of field data are made. */
+int dmtz_correct_dev_f32(dmtz_ctx* ctx, const float* d_f, float* d_g, int64_t nx, int64_t ny, int64_t nz,
+                         double eps, double range, const dmtz_flags* flags, dmtz_counts* out,
+                         uint8_t* d_counter_out, int32_t* sweeps_out);
+int dmtz_correct_dev_f64(dmtz_ctx* ctx, const double* d_f, double* d_g, int64_t nx, int64_t ny, int64_t nz,
+                         double eps, double range, const dmtz_flags* flags, dmtz_counts* out,
+                         uint8_t* d_counter_out, int32_t* sweeps_out);
+
+/* Per-sweep counts of the last correction on ctx: 8 int64 per sweep (dmtz_counts order), up to
+ * cap sweeps; returns the number of sweeps recorded. Mirrors the per-sweep prints of
+ * main_3d.cu:6787-6791, 7146-7147. */
+int64_t dmtz_trajectory(const dmtz_ctx* ctx, int64_t* out, int64_t cap);
+
+/* Phase times of the last correction in seconds, the reference's phase names
+ * (main_3d.cu:6419-6435): out[0] gradient_pairing (classification), [1] get_f_cp,
+ * [2] get_f_path, [3] fix_f_cp, [4] fix_f_vpath, [5] total. */
+int dmtz_phase_times(const dmtz_ctx* ctx, double* out6);
+
+/* The reference's pairing arrays of one field (main_3d.cu:1930-2316, the five passes of
+ * main_3d.cu:6550-6564): host arrays of N, E, F, T int64 entries (reference simplex IDs,
+ * -1 = critical). Evaluated on the device from per-vertex lower stars. */
+int dmtz_gradient_f32(dmtz_ctx* ctx, const float* field, int64_t nx, int64_t ny, int64_t nz,
+                      int64_t* pv, int64_t* pe, int64_t* pf, int64_t* pt);
+int dmtz_gradient_f64(dmtz_ctx* ctx, const double* field, int64_t nx, int64_t ny, int64_t nz,
+                      int64_t* pv, int64_t* pe, int64_t* pf, int64_t* pt);
+
+/* One detection + repair phase on (f, g) without applying it: the mismatch counts of
+ * main_3d.cu:2492-2550 (counts4 = min, 1-saddle, 2-saddle, max) and the edit set the repair
+ * kernels of main_3d.cu:2554-4303 write into d_deltaBuffer (targets: N bytes, 0/1). */
+int dmtz_targets_f32(dmtz_ctx* ctx, const float* f, const float* g, int64_t nx, int64_t ny, int64_t nz,
+                     double eps, double range, const dmtz_flags* flags, uint8_t* targets, int64_t* counts4);
+int dmtz_targets_f64(dmtz_ctx* ctx, const double* f, const double* g, int64_t nx, int64_t ny, int64_t nz,
+                     double eps, double range, const dmtz_flags* flags, uint8_t* targets, int64_t* counts4);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DMTZ_H */

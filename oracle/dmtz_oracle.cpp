@@ -1,0 +1,1904 @@
+// DMTz topology-correction oracle: a CPU restatement of the reference path in main_3d.cu.
+//
+// TEST INFRASTRUCTURE ONLY. Only the test suite, the smoke check and the CPU-baseline leg of the
+// benchmark may load this library, and only as the checker. The shipped HIP path never links it.
+//
+// Structure follows the reference literally: dense per-simplex pairing arrays keyed by the
+// reference's global simplex IDs, five gradient passes, XOR mismatch lists, the repair rules,
+// the edit buffer and the correction loop. Every function cites the main_3d.cu lines it restates.
+// Two gradient implementations exist: LITERAL (the reference's 14^k brute-force neighbour
+// searches) and DIRECT (the same candidate sets enumerated through an adjacency table); tests
+// check that both produce identical arrays.
+//
+// Field type T is float or double. With T = double all value arithmetic is the reference's.
+// With T = float the values are float32, comparisons are exact float32 ordering with index
+// tie-break, and edit arithmetic is done in double then rounded to float (see Arith<float>).
+//
+// Resolutions of reference defects (all documented, none change defined behaviour):
+//   * out-of-range pairing-array reads (index -1 or beyond the array) read as -1 ("unpaired");
+//   * vertex indices outside [0, N) produced by decoding a wrong-type ID are never edited;
+//   * descending/ascending path walks are capped at N+1 steps (the reference has no cap);
+//   * ascending connectivity labels are indexed by face id (main_3d.cu:4750/4823 shadowing bug);
+//   * walls use a private per-start DFS with a separate index array (main_3d.cu:6706, 5088);
+//   * the correction loop stops after max_sweeps sweeps and reports non-convergence.
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <cfloat>
+#include <climits>
+#include <vector>
+#include <algorithm>
+#include <atomic>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef int64_t i64;
+
+namespace orc {
+
+// main_3d.cu:44-51 — the 14-neighbour stencil of the Freudenthal lattice.
+static const int DIRS[14][3] = {{1, 0, 0},  {-1, 0, 0}, {0, 1, 0},  {0, -1, 0}, {0, 0, 1},
+                                {0, 0, -1}, {-1, 1, 0}, {1, -1, 0}, {0, 1, 1},  {0, -1, -1},
+                                {-1, 0, 1}, {1, 0, -1}, {-1, 1, 1}, {1, -1, -1}};
+
+// dir_of[(dx+1)*9+(dy+1)*3+(dz+1)] = direction index or -1.
+static int dir_of[27];
+static bool dir_init = false;
+static void init_dirs() {
+    if (dir_init) return;
+    for (int i = 0; i < 27; i++) dir_of[i] = -1;
+    for (int i = 0; i < 14; i++)
+        dir_of[(DIRS[i][0] + 1) * 9 + (DIRS[i][1] + 1) * 3 + (DIRS[i][2] + 1)] = i;
+    dir_init = true;
+}
+
+struct Grid {
+    i64 W, H, D, N, E, F, T;
+    Grid(i64 w, i64 h, i64 d) : W(w), H(h), D(d) {
+        // main_3d.cu:6401-6415
+        N = W * H * D;
+        E = (W - 1) * H * D + W * (H - 1) * D + W * H * (D - 1) + (W - 1) * (H - 1) * D +
+            (W - 1) * H * (D - 1) + W * (H - 1) * (D - 1) + (W - 1) * (H - 1) * (D - 1);
+        F = 2 * ((W - 1) * (H - 1) * D + W * (H - 1) * (D - 1) + (W - 1) * H * (D - 1)) +
+            6 * (W - 1) * (H - 1) * (D - 1);
+        T = 6 * (W - 1) * (H - 1) * (D - 1);
+    }
+    inline void xyz(i64 v, i64& x, i64& y, i64& z) const {
+        x = v % W;
+        y = (v / W) % H;
+        z = (v / (W * H)) % D;
+    }
+    inline bool inb(i64 x, i64 y, i64 z) const {
+        return x >= 0 && x < W && y >= 0 && y < H && z >= 0 && z < D;
+    }
+    inline i64 idx(i64 x, i64 y, i64 z) const { return x + y * W + z * W * H; }
+    // Are two in-grid vertices 14-neighbours?
+    inline bool adjacent(i64 a, i64 b) const {
+        i64 ax, ay, az, bx, by, bz;
+        xyz(a, ax, ay, az);
+        xyz(b, bx, by, bz);
+        i64 dx = bx - ax, dy = by - ay, dz = bz - az;
+        if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) return false;
+        return dir_of[(dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)] >= 0;
+    }
+};
+
+// ----------------------------------------------------------------------------------------------
+// Implicit complex indexing (main_3d.cu:246-740, 917-1012, 1760-1900, 4363-4430)
+// ----------------------------------------------------------------------------------------------
+
+// getEdgeID main_3d.cu:917-1012
+static i64 edge_id(const Grid& g, i64 x, i64 y, i64 z, i64 x1, i64 y1, i64 z1) {
+    const i64 W = g.W, H = g.H, D = g.D;
+    i64 dx = x1 - x, dy = y1 - y, dz = z1 - z;
+    const i64 o3 = (W - 1) * H * D + W * (H - 1) * D + W * H * (D - 1);
+    const i64 o4 = o3 + (W - 1) * (H - 1) * D;
+    const i64 o5 = o4 + W * (H - 1) * (D - 1);
+    const i64 o6 = o5 + (W - 1) * H * (D - 1);
+    if (dx == 1 && dy == 0 && dz == 0) return z * (W - 1) * H + y * (W - 1) + x;
+    if (dx == -1 && dy == 0 && dz == 0) return z * (W - 1) * H + y * (W - 1) + (x - 1);
+    if (dx == 0 && dy == 1 && dz == 0) return (W - 1) * H * D + z * W * (H - 1) + x * (H - 1) + y;
+    if (dx == 0 && dy == -1 && dz == 0)
+        return (W - 1) * H * D + z * W * (H - 1) + x * (H - 1) + y - 1;
+    if (dx == 0 && dy == 0 && dz == 1)
+        return (W - 1) * H * D + W * (H - 1) * D + y * W * (D - 1) + x * (D - 1) + z;
+    if (dx == 0 && dy == 0 && dz == -1)
+        return (W - 1) * H * D + W * (H - 1) * D + y * W * (D - 1) + x * (D - 1) + z - 1;
+    if (dx == 1 && dy == -1 && dz == 0) return o3 + z * (H - 1) * (W - 1) + (y - 1) * (W - 1) + x;
+    if (dx == -1 && dy == 1 && dz == 0) return o3 + z * (H - 1) * (W - 1) + y * (W - 1) + (x - 1);
+    if (dx == 0 && dy == 1 && dz == 1) return o4 + x * (H - 1) * (D - 1) + y * (D - 1) + z;
+    if (dx == 0 && dy == -1 && dz == -1)
+        return o4 + x * (H - 1) * (D - 1) + (y - 1) * (D - 1) + z - 1;
+    if (dx == -1 && dy == 0 && dz == 1) return o5 + y * (W - 1) * (D - 1) + z * (W - 1) + x - 1;
+    if (dx == 1 && dy == 0 && dz == -1) return o5 + y * (W - 1) * (D - 1) + (z - 1) * (W - 1) + x;
+    if (dx != 0 && dy != 0 && dz != 0) {
+        i64 cx = std::min(x, x1), cy = std::min(y, y1), cz = std::min(z, z1);
+        return o6 + cz * (W - 1) * (H - 1) + cy * (W - 1) + cx;
+    }
+    return -1;
+}
+
+// getVertexIDsFromEdgeID main_3d.cu:1760-1900; returns id1 > id2 (index order).
+static void edge_verts(const Grid& g, i64 e, i64& id1, i64& id2) {
+    const i64 W = g.W, H = g.H, D = g.D;
+    i64 counts[7] = {(W - 1) * H * D,       W * (H - 1) * D,       W * H * (D - 1),
+                     (W - 1) * (H - 1) * D, W * (H - 1) * (D - 1), (W - 1) * H * (D - 1),
+                     (W - 1) * (H - 1) * (D - 1)};
+    i64 acc = 0;
+    int dir = -1;
+    for (int i = 0; i < 7; ++i) {
+        if (e < acc + counts[i]) {
+            dir = i;
+            break;
+        }
+        acc += counts[i];
+    }
+    if (dir == -1) {
+        id1 = id2 = -1;
+        return;
+    }
+    i64 r = e - acc;
+    i64 x = 0, y = 0, z = 0, x1 = 0, y1 = 0, z1 = 0;
+    // C++ integer division truncates toward zero, as in the reference, for the r < 0 case (e<0).
+    if (dir == 0) {
+        z = r / ((W - 1) * H);
+        y = (r % ((W - 1) * H)) / (W - 1);
+        x = r % (W - 1);
+        x1 = x + 1; y1 = y; z1 = z;
+    } else if (dir == 1) {
+        z = r / (W * (H - 1));
+        x = (r % (W * (H - 1))) / (H - 1);
+        y = r % (H - 1);
+        y1 = y + 1; x1 = x; z1 = z;
+    } else if (dir == 2) {
+        y = r / (W * (D - 1));
+        x = (r % (W * (D - 1))) / (D - 1);
+        z = r % (D - 1);
+        y1 = y; x1 = x; z1 = z + 1;
+    } else if (dir == 3) {
+        i64 z2 = r / ((W - 1) * (H - 1));
+        i64 y2 = (r % ((W - 1) * (H - 1))) / (W - 1);
+        i64 x2 = r % (W - 1);
+        x = x2 + 1; y = y2; z = z2; y1 = y2 + 1; x1 = x2; z1 = z2;
+    } else if (dir == 4) {
+        i64 x2 = r / ((H - 1) * (D - 1));
+        i64 y2 = (r % ((H - 1) * (D - 1))) / (D - 1);
+        i64 z2 = r % (D - 1);
+        x = x2; y = y2; z = z2; y1 = y2 + 1; x1 = x2; z1 = z2 + 1;
+    } else if (dir == 5) {
+        i64 y2 = r / ((W - 1) * (D - 1));
+        i64 z2 = (r % ((W - 1) * (D - 1))) / (W - 1);
+        i64 x2 = r % (W - 1);
+        x = x2 + 1; y = y2; z = z2; y1 = y2; x1 = x2; z1 = z2 + 1;
+    } else {
+        i64 z2 = r / ((W - 1) * (H - 1));
+        i64 y2 = (r % ((W - 1) * (H - 1))) / (W - 1);
+        i64 x2 = r % (W - 1);
+        x = x2 + 1; y = y2; z = z2; y1 = y2 + 1; x1 = x2; z1 = z2 + 1;
+    }
+    id1 = x + y * W + z * (W * H);
+    id2 = x1 + y1 * W + z1 * (W * H);
+    if (id1 < id2) std::swap(id1, id2);
+}
+
+static inline void sort3_desc(i64& a, i64& b, i64& c) {
+    if (a < b) std::swap(a, b);
+    if (a < c) std::swap(a, c);
+    if (b < c) std::swap(b, c);
+}
+
+// getTriangleID1 main_3d.cu:246-363 (inputs already index-sorted descending).
+static i64 tri_id1(const Grid& g, i64 v1, i64 v2, i64 v3, int direction) {
+    const i64 W = g.W, H = g.H, D = g.D;
+    i64 s1 = v1, s2 = v2, s3 = v3;
+    sort3_desc(s1, s2, s3);
+    i64 x1, y1, z1, x2, y2, z2, x3, y3, z3;
+    g.xyz(s1, x1, y1, z1);
+    g.xyz(s2, x2, y2, z2);
+    g.xyz(s3, x3, y3, z3);
+    i64 mx = std::min(x1, std::min(x2, x3));
+    i64 my = std::min(y1, std::min(y2, y3));
+    i64 mz = std::min(z1, std::min(z2, z3));
+    if (direction == 0) {
+        i64 cell = my * (W - 1) + mx;
+        if (x3 == x1 && x1 == mx && y3 == y2 && y2 == my) return cell * 2 + 2 * mz * (W - 1) * (H - 1);
+        return cell * 2 + 1 + 2 * mz * (W - 1) * (H - 1);
+    } else if (direction == 1) {
+        i64 cell = my * (D - 1) + mz;
+        i64 off = 2 * D * (W - 1) * (H - 1) + 2 * mx * (H - 1) * (D - 1);
+        if (y2 == y1 && y1 == my + 1 && z3 == z2 && z2 == mz) return cell * 2 + off;
+        return cell * 2 + 1 + off;
+    } else if (direction == 2) {
+        i64 cell = mz * (W - 1) + mx;
+        i64 off = 2 * (D * (W - 1) * (H - 1) + W * (H - 1) * (D - 1) + my * (W - 1) * (D - 1));
+        if (z2 == z3 && z2 == mz && x1 == x3 && x1 == mx) return cell * 2 + off;
+        return cell * 2 + 1 + off;
+    } else if (direction == 3) {
+        i64 cell = mz * (W - 1) * (H - 1) + my * (W - 1) + mx;
+        int cur = -1;
+        if (x3 == mx && y3 == my && z3 == mz && x2 == mx + 1 && y2 == my && z2 == mz && x1 == mx &&
+            y1 == my + 1 && z1 == mz + 1)
+            cur = 0;
+        if (x3 == mx + 1 && y3 == my && z3 == mz && x2 == mx && y2 == my + 1 && z2 == mz &&
+            x1 == mx && y1 == my + 1 && z1 == mz + 1)
+            cur = 1;
+        if (x3 == mx + 1 && y3 == my && z3 == mz && x2 == mx + 1 && y2 == my + 1 && z2 == mz &&
+            x1 == mx && y1 == my + 1 && z1 == mz + 1)
+            cur = 2;
+        if (x3 == mx + 1 && y3 == my && z3 == mz && x2 == mx && y2 == my && z2 == mz + 1 &&
+            x1 == mx && y1 == my + 1 && z1 == mz + 1)
+            cur = 3;
+        if (x3 == mx + 1 && y3 == my && z3 == mz && x2 == mx + 1 && y2 == my && z2 == mz + 1 &&
+            x1 == mx && y1 == my + 1 && z1 == mz + 1)
+            cur = 4;
+        if (x3 == mx + 1 && y3 == my && z3 == mz && x2 == mx && y2 == my + 1 && z2 == mz + 1 &&
+            x1 == mx + 1 && y1 == my + 1 && z1 == mz + 1)
+            cur = 5;
+        return cell * 6 + cur + 2 * D * (W - 1) * (H - 1) + 2 * W * (H - 1) * (D - 1) +
+               2 * H * (W - 1) * (D - 1);
+    }
+    return -1;
+}
+
+// getTriangleID main_3d.cu:364-423
+static i64 tri_id(const Grid& g, i64 v1, i64 v2, i64 v3) {
+    i64 s1 = v1, s2 = v2, s3 = v3;
+    sort3_desc(s1, s2, s3);
+    i64 x1, y1, z1, x2, y2, z2, x3, y3, z3;
+    g.xyz(s1, x1, y1, z1);
+    g.xyz(s2, x2, y2, z2);
+    g.xyz(s3, x3, y3, z3);
+    if (z1 == z2 && z2 == z3) return tri_id1(g, s1, s2, s3, 0);
+    if (x1 == x2 && x2 == x3) return tri_id1(g, s1, s2, s3, 1);
+    if (y1 == y2 && y2 == y3) return tri_id1(g, s1, s2, s3, 2);
+    return tri_id1(g, s1, s2, s3, 3);
+}
+
+// getVerticesFromTriangleID main_3d.cu:425-565; returns v1 > v2 > v3 (index order).
+static void tri_verts(const Grid& g, i64 t, i64& v1, i64& v2, i64& v3) {
+    const i64 W = g.W, H = g.H, D = g.D;
+    i64 b, mx, my, mz;
+    if (t < 2 * D * (W - 1) * (H - 1)) {
+        b = t / 2;
+        mz = b / ((W - 1) * (H - 1));
+        my = (b % ((W - 1) * (H - 1))) / (W - 1);
+        mx = b % (W - 1);
+        if (t % 2 == 0) {
+            v1 = g.idx(mx, my, mz); v2 = g.idx(mx + 1, my, mz); v3 = g.idx(mx, my + 1, mz);
+        } else {
+            v1 = g.idx(mx + 1, my + 1, mz); v2 = g.idx(mx + 1, my, mz); v3 = g.idx(mx, my + 1, mz);
+        }
+        sort3_desc(v1, v2, v3);
+        return;
+    }
+    t -= 2 * D * (W - 1) * (H - 1);
+    if (t < 2 * W * (H - 1) * (D - 1)) {
+        b = t / 2;
+        mx = b / ((H - 1) * (D - 1));
+        my = (b % ((H - 1) * (D - 1))) / (D - 1);
+        mz = b % (D - 1);
+        if (t % 2 == 0) {
+            v1 = g.idx(mx, my, mz); v2 = g.idx(mx, my + 1, mz); v3 = g.idx(mx, my + 1, mz + 1);
+        } else {
+            v1 = g.idx(mx, my, mz); v2 = g.idx(mx, my, mz + 1); v3 = g.idx(mx, my + 1, mz + 1);
+        }
+        sort3_desc(v1, v2, v3);
+        return;
+    }
+    t -= 2 * W * (H - 1) * (D - 1);
+    if (t < 2 * H * (W - 1) * (D - 1)) {
+        b = t / 2;
+        my = b / ((W - 1) * (D - 1));
+        mz = (b % ((W - 1) * (D - 1))) / (W - 1);
+        mx = b % (W - 1);
+        if (t % 2 == 0) {
+            v1 = g.idx(mx, my, mz); v2 = g.idx(mx + 1, my, mz); v3 = g.idx(mx, my, mz + 1);
+        } else {
+            v1 = g.idx(mx + 1, my, mz); v2 = g.idx(mx, my, mz + 1); v3 = g.idx(mx + 1, my, mz + 1);
+        }
+        sort3_desc(v1, v2, v3);
+        return;
+    }
+    t -= 2 * H * (W - 1) * (D - 1);
+    b = t / 6;
+    mz = b / ((W - 1) * (H - 1));
+    my = (b % ((W - 1) * (H - 1))) / (W - 1);
+    mx = b % (W - 1);
+    switch (t % 6) {
+        case 0: v1 = g.idx(mx, my, mz); v2 = g.idx(mx + 1, my, mz); v3 = g.idx(mx, my + 1, mz + 1); break;
+        case 1: v1 = g.idx(mx + 1, my, mz); v2 = g.idx(mx, my + 1, mz); v3 = g.idx(mx, my + 1, mz + 1); break;
+        case 2: v1 = g.idx(mx + 1, my, mz); v2 = g.idx(mx + 1, my + 1, mz); v3 = g.idx(mx, my + 1, mz + 1); break;
+        case 3: v1 = g.idx(mx + 1, my, mz); v2 = g.idx(mx, my, mz + 1); v3 = g.idx(mx, my + 1, mz + 1); break;
+        case 4: v1 = g.idx(mx + 1, my, mz); v2 = g.idx(mx + 1, my, mz + 1); v3 = g.idx(mx, my + 1, mz + 1); break;
+        default: v1 = g.idx(mx + 1, my, mz); v2 = g.idx(mx, my + 1, mz + 1); v3 = g.idx(mx + 1, my + 1, mz + 1); break;
+    }
+    sort3_desc(v1, v2, v3);
+}
+
+static inline void sort4_desc(i64& a, i64& b, i64& c, i64& d) {
+    // the reference's exchange sequence, main_3d.cu:598-605 / 732-739
+    if (a < b) std::swap(a, b);
+    if (a < c) std::swap(a, c);
+    if (a < d) std::swap(a, d);
+    if (b < c) std::swap(b, c);
+    if (b < d) std::swap(b, d);
+    if (c < d) std::swap(c, d);
+}
+
+// getTetrahedraID main_3d.cu:572-670
+static i64 tet_id(const Grid& g, i64 v1, i64 v2, i64 v3, i64 v4) {
+    const i64 W = g.W, H = g.H;
+    i64 x[4], y[4], z[4];
+    i64 vs[4] = {v1, v2, v3, v4};
+    for (int k = 0; k < 4; k++) g.xyz(vs[k], x[k], y[k], z[k]);
+    i64 mx = std::min(std::min(x[0], x[1]), std::min(x[2], x[3]));
+    i64 my = std::min(std::min(y[0], y[1]), std::min(y[2], y[3]));
+    i64 mz = std::min(std::min(z[0], z[1]), std::min(z[2], z[3]));
+    i64 cell = mz * (W - 1) * (H - 1) + my * (W - 1) + mx;
+    i64 a = v1, b = v2, c = v3, d = v4;
+    sort4_desc(a, b, c, d);
+    i64 ax, ay, az, bx, by, bz, cx, cy, cz, dx, dy, dz;
+    g.xyz(a, ax, ay, az);
+    g.xyz(b, bx, by, bz);
+    g.xyz(c, cx, cy, cz);
+    g.xyz(d, dx, dy, dz);
+    // (d,c,b,a) = sorted ascending: sortedV4, sortedV3, sortedV2, sortedV1
+    auto is = [](i64 px, i64 py, i64 pz, i64 qx, i64 qy, i64 qz) {
+        return px == qx && py == qy && pz == qz;
+    };
+    if (is(dx, dy, dz, mx, my, mz) && is(cx, cy, cz, mx + 1, my, mz) && is(bx, by, bz, mx, my + 1, mz) &&
+        is(ax, ay, az, mx, my + 1, mz + 1))
+        return cell * 6;
+    if (is(dx, dy, dz, mx, my, mz) && is(cx, cy, cz, mx + 1, my, mz) && is(bx, by, bz, mx, my, mz + 1) &&
+        is(ax, ay, az, mx, my + 1, mz + 1))
+        return cell * 6 + 1;
+    if (is(dx, dy, dz, mx + 1, my, mz) && is(cx, cy, cz, mx, my + 1, mz) &&
+        is(bx, by, bz, mx + 1, my + 1, mz) && is(ax, ay, az, mx, my + 1, mz + 1))
+        return cell * 6 + 2;
+    if (is(dx, dy, dz, mx + 1, my, mz) && is(cx, cy, cz, mx + 1, my + 1, mz) &&
+        is(bx, by, bz, mx, my + 1, mz + 1) && is(ax, ay, az, mx + 1, my + 1, mz + 1))
+        return cell * 6 + 3;
+    if (is(dx, dy, dz, mx + 1, my, mz) && is(cx, cy, cz, mx, my, mz + 1) &&
+        is(bx, by, bz, mx + 1, my, mz + 1) && is(ax, ay, az, mx, my + 1, mz + 1))
+        return cell * 6 + 4;
+    if (is(dx, dy, dz, mx + 1, my, mz) && is(cx, cy, cz, mx + 1, my, mz + 1) &&
+        is(bx, by, bz, mx, my + 1, mz + 1) && is(ax, ay, az, mx + 1, my + 1, mz + 1))
+        return cell * 6 + 5;
+    return -1;
+}
+
+// getVerticesFromTetrahedraID main_3d.cu:672-740; returns v1 > v2 > v3 > v4 (index order).
+static void tet_verts(const Grid& g, i64 q, i64& v1, i64& v2, i64& v3, i64& v4) {
+    const i64 W = g.W, H = g.H;
+    i64 b = q / 6;
+    int type = (int)(q % 6);
+    i64 cz = b / ((W - 1) * (H - 1));
+    i64 cy = (b % ((W - 1) * (H - 1))) / (W - 1);
+    i64 cx = b % (W - 1);
+    switch (type) {
+        case 0: v1 = g.idx(cx, cy, cz); v2 = g.idx(cx + 1, cy, cz); v3 = g.idx(cx, cy + 1, cz); v4 = g.idx(cx, cy + 1, cz + 1); break;
+        case 1: v1 = g.idx(cx, cy, cz); v2 = g.idx(cx + 1, cy, cz); v3 = g.idx(cx, cy, cz + 1); v4 = g.idx(cx, cy + 1, cz + 1); break;
+        case 2: v1 = g.idx(cx + 1, cy, cz); v2 = g.idx(cx, cy + 1, cz); v3 = g.idx(cx + 1, cy + 1, cz); v4 = g.idx(cx, cy + 1, cz + 1); break;
+        case 3: v1 = g.idx(cx + 1, cy, cz); v2 = g.idx(cx + 1, cy + 1, cz); v3 = g.idx(cx, cy + 1, cz + 1); v4 = g.idx(cx + 1, cy + 1, cz + 1); break;
+        case 4: v1 = g.idx(cx + 1, cy, cz); v2 = g.idx(cx, cy, cz + 1); v3 = g.idx(cx + 1, cy, cz + 1); v4 = g.idx(cx, cy + 1, cz + 1); break;
+        default: v1 = g.idx(cx + 1, cy, cz); v2 = g.idx(cx + 1, cy, cz + 1); v3 = g.idx(cx, cy + 1, cz + 1); v4 = g.idx(cx + 1, cy + 1, cz + 1); break;
+    }
+    sort4_desc(v1, v2, v3, v4);
+}
+
+// Common neighbours of (v1,v2,v3) that complete a tetrahedron, in the reference's search order
+// (direction index from v1). LITERAL = the 14^3 search of main_3d.cu:4380-4426 / 2234-2304.
+static int tri_cofacet_v4(const Grid& g, i64 v1, i64 v2, i64 v3, bool literal, i64 out[14]) {
+    int n = 0;
+    i64 x1, y1, z1, x2, y2, z2, x3, y3, z3;
+    g.xyz(v1, x1, y1, z1);
+    g.xyz(v2, x2, y2, z2);
+    g.xyz(v3, x3, y3, z3);
+    if (literal) {
+        for (int i = 0; i < 14; i++) {
+            i64 nx = x1 + DIRS[i][0], ny = y1 + DIRS[i][1], nz = z1 + DIRS[i][2];
+            i64 n1 = nx + ny * g.W + nz * g.W * g.H;
+            if (nx >= g.W || ny >= g.H || nz >= g.D || nx < 0 || ny < 0 || nz < 0 || n1 >= g.N || n1 < 0) continue;
+            for (int j = 0; j < 14; j++) {
+                i64 mx = x2 + DIRS[j][0], my = y2 + DIRS[j][1], mz = z2 + DIRS[j][2];
+                i64 n2 = mx + my * g.W + mz * g.W * g.H;
+                if (mx >= g.W || my >= g.H || mz >= g.D || mx < 0 || my < 0 || mz < 0 || n2 >= g.N || n2 < 0) continue;
+                for (int k = 0; k < 14; k++) {
+                    i64 px = x3 + DIRS[k][0], py = y3 + DIRS[k][1], pz = z3 + DIRS[k][2];
+                    i64 n3 = px + py * g.W + pz * g.W * g.H;
+                    if (px >= g.W || py >= g.H || pz >= g.D || px < 0 || py < 0 || pz < 0 || n3 >= g.N || n3 < 0) continue;
+                    if (!(n1 == n2 && n2 == n3 && n1 == n3)) continue;
+                    if (n1 == v1 || n1 == v2 || n1 == v3) continue;
+                    out[n++] = n1;
+                }
+            }
+        }
+    } else {
+        for (int i = 0; i < 14; i++) {
+            i64 nx = x1 + DIRS[i][0], ny = y1 + DIRS[i][1], nz = z1 + DIRS[i][2];
+            if (!g.inb(nx, ny, nz)) continue;
+            i64 n1 = g.idx(nx, ny, nz);
+            if (n1 == v2 || n1 == v3) continue;
+            if (g.adjacent(n1, v2) && g.adjacent(n1, v3)) out[n++] = n1;
+        }
+    }
+    return n;
+}
+
+// Common neighbours of an edge (v1,v2), main_3d.cu:2027-2040 (14x14 search) or direct.
+static int edge_common(const Grid& g, i64 v1, i64 v2, bool literal, i64 out[14]) {
+    int n = 0;
+    i64 x1, y1, z1, x2, y2, z2;
+    g.xyz(v1, x1, y1, z1);
+    g.xyz(v2, x2, y2, z2);
+    if (literal) {
+        for (int i = 0; i < 14; i++) {
+            i64 nx = x1 + DIRS[i][0], ny = y1 + DIRS[i][1], nz = z1 + DIRS[i][2];
+            for (int j = 0; j < 14; j++) {
+                i64 mx = x2 + DIRS[j][0], my = y2 + DIRS[j][1], mz = z2 + DIRS[j][2];
+                i64 nb = nx + ny * g.W + nz * (g.H * g.W);
+                if (nx == mx && ny == my && nz == mz && nx >= 0 && nx < g.W && ny >= 0 && ny < g.H &&
+                    nz >= 0 && nz < g.D && nb < g.N && nb >= 0)
+                    out[n++] = nb;
+            }
+        }
+    } else {
+        for (int i = 0; i < 14; i++) {
+            i64 nx = x1 + DIRS[i][0], ny = y1 + DIRS[i][1], nz = z1 + DIRS[i][2];
+            if (!g.inb(nx, ny, nz)) continue;
+            i64 nb = g.idx(nx, ny, nz);
+            if (nb == v2) continue;
+            if (g.adjacent(nb, v2)) out[n++] = nb;
+        }
+    }
+    return n;
+}
+
+// get_cell_tetra main_3d.cu:4363-4430: x = first cofacet found, y = second (-1 if none).
+static void cell_tetra(const Grid& g, i64 face, i64& x, i64& y, bool literal) {
+    i64 v1, v2, v3;
+    tri_verts(g, face, v1, v2, v3);
+    i64 c[14];
+    int n = tri_cofacet_v4(g, v1, v2, v3, literal, c);
+    x = -1;
+    y = -1;
+    for (int k = 0; k < n; k++) {
+        i64 t = tet_id(g, v1, v2, v3, c[k]);
+        if (k == 0) x = t;
+        else y = t;
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// Arithmetic of the edit buffer (main_3d.cu:239-244, 1122-1164)
+// ----------------------------------------------------------------------------------------------
+template <class T> struct Arith;
+template <> struct Arith<double> {
+    static double delta0(double f, double g, double eps) { return (f - eps - g) / 2.0; }
+    // main_3d.cu:1133-1135: accept if |f - (g + d)| <= eps, then g += d.
+    static bool step(double f, double& g, double d, double eps) {
+        if (std::fabs(f - (g + d)) <= eps) {
+            g += d;
+            return true;
+        }
+        return false;
+    }
+    static double snap(double f, double eps) { return f - eps; }
+};
+static inline float next_up_f(float x) {
+    if (x != x) return x;
+    if (x == 0.0f) return std::ldexp(1.0f, -149);  // smallest positive subnormal
+    uint32_t b;
+    std::memcpy(&b, &x, 4);
+    if (x > 0) b += 1;
+    else b -= 1;
+    float r;
+    std::memcpy(&r, &b, 4);
+    return r;
+}
+template <> struct Arith<float> {
+    static double delta0(float f, float g, double eps) { return ((double)f - eps - (double)g) / 2.0; }
+    // Float32 storage: the candidate is rounded to float first and the eps check is done on the
+    // rounded value, so |g' - f| <= eps holds for the stored value.
+    static bool step(float f, float& g, double d, double eps) {
+        float c = (float)((double)g + d);
+        if (std::fabs((double)f - (double)c) <= eps) {
+            g = c;
+            return true;
+        }
+        return false;
+    }
+    // Smallest float x with f - x <= eps (evaluated in double): the float32 form of f - eps.
+    static float snap(float f, double eps) {
+        float x = (float)((double)f - eps);
+        if ((double)f - (double)x > eps) x = next_up_f(x);
+        return x;
+    }
+};
+
+// ----------------------------------------------------------------------------------------------
+// Correction state
+// ----------------------------------------------------------------------------------------------
+struct Flags {
+    int32_t preserve_saddles, preserve_vpath, preserve_geometry, filtration, preserve_connectors,
+        preserve_type, edit_type, max_sweeps;
+};
+
+enum { F_FIELD = 0, G_FIELD = 1 };
+
+template <class T> struct State {
+    Grid g;
+    const T* f;
+    T* d;  // decompressed field, corrected in place
+    double eps, range;
+    Flags fl;
+    bool literal;
+    std::vector<i64> pv[2], pe[2], pf[2], pt[2];
+    std::vector<int32_t> counter;
+    std::vector<uint8_t> target;
+    std::vector<i64> l_min, l_s1, l_s2, l_max, l_as, l_ds, l_dsw;
+    std::vector<i64> as_index, ds_index, dsw_index;  // index arrays (2T, 2E, F)
+    std::vector<i64> min_label, max_label;           // connectivity-mode f labels (2E, 2F)
+    i64 walk_overflow = 0;
+
+    State(i64 W, i64 H, i64 D) : g(W, H, D) {}
+
+    inline T val(int which, i64 v) const { return which == F_FIELD ? f[v] : d[v]; }
+    // compare_vertices main_3d.cu:1043-1049: a > b in (value, index) order.
+    inline bool cmp(i64 a, i64 b, int which) const {
+        T va = val(which, a), vb = val(which, b);
+        return va > vb || (va == vb && a > b);
+    }
+    // Overloaded pairing-array reads with the out-of-range resolution.
+    inline i64 PV(int w, i64 i) const { return (i >= 0 && i < g.N) ? pv[w][i] : -1; }
+    inline i64 PE(int w, i64 i) const { return (i >= 0 && i < g.E) ? pe[w][i] : -1; }
+    inline i64 PF(int w, i64 i) const { return (i >= 0 && i < g.F) ? pf[w][i] : -1; }
+    inline i64 PT(int w, i64 i) const { return (i >= 0 && i < g.T) ? pt[w][i] : -1; }
+
+    double delta(i64 v) const {
+        // get_delta main_3d.cu:239-244
+        if (fl.edit_type == 1) return -std::ldexp(eps, -counter[v]);
+        return Arith<T>::delta0(f[v], d[v], eps);
+    }
+    inline void tgt(i64 v) {
+        if (v >= 0 && v < g.N) __atomic_store_n(&target[v], (uint8_t)1, __ATOMIC_RELAXED);
+    }
+
+    void alloc() {
+        for (int w = 0; w < 2; w++) {
+            pv[w].assign(g.N, -1);
+            pe[w].assign(g.E, -1);
+            pf[w].assign(g.F, -1);
+            pt[w].assign(g.T, -1);
+        }
+        counter.assign(g.N, 1);
+        target.assign(g.N, 0);
+        as_index.assign(2 * g.T, -1);
+        ds_index.assign(2 * g.E, -1);
+        dsw_index.assign(g.F, -1);
+    }
+
+    // ------------------------------------------------------------------------------------------
+    // Discrete gradient: five passes (main_3d.cu:1930-2113, 2197-2316)
+    // ------------------------------------------------------------------------------------------
+    void reset(int w) {
+        // init_vertices/edges/faces/tetras main_3d.cu:4305-4361
+        std::fill(pv[w].begin(), pv[w].end(), -1);
+        std::fill(pe[w].begin(), pe[w].end(), -1);
+        std::fill(pf[w].begin(), pf[w].end(), -1);
+        std::fill(pt[w].begin(), pt[w].end(), -1);
+    }
+
+    // processVertices main_3d.cu:1930-1988
+    void pass_vertices(int w) {
+        std::vector<i64>& PVw = pv[w];
+        std::vector<i64>& PEw = pe[w];
+#pragma omp parallel for schedule(static)
+        for (i64 v = 0; v < g.N; v++) {
+            i64 x, y, z;
+            g.xyz(v, x, y, z);
+            bool have = false;
+            T best = 0;
+            i64 best_id = -1, paired = -1;
+            T dv = val(w, v);
+            for (int i = 0; i < 14; i++) {
+                i64 nx = x + DIRS[i][0], ny = y + DIRS[i][1], nz = z + DIRS[i][2];
+                if (!g.inb(nx, ny, nz)) continue;
+                i64 n = g.idx(nx, ny, nz);
+                T dn = val(w, n);
+                bool lower = dn < dv || (dn == dv && n < v);
+                // "largest_v" starts at DBL_MAX: any finite value is smaller.
+                bool better = !have || dn < best || (dn == best && n < best_id);
+                if (lower && better) {
+                    have = true;
+                    best = dn;
+                    best_id = n;
+                    paired = edge_id(g, x, y, z, nx, ny, nz);
+                }
+            }
+            if (paired != -1) {
+                PVw[v] = paired;
+                PEw[paired] = v;
+            }
+        }
+    }
+
+    // edgePairingKernel main_3d.cu:1990-2113
+    void pass_edges(int w, int a2) {
+        std::vector<i64>& PEw = pe[w];
+        std::vector<i64>& PFw = pf[w];
+#pragma omp parallel for schedule(static)
+        for (i64 e = 0; e < g.E; e++) {
+            if (PEw[e] != -1) continue;
+            i64 v1, v2;
+            edge_verts(g, e, v1, v2);
+            i64 cn[14];
+            int n = edge_common(g, v1, v2, literal, cn);
+            bool have = false;
+            T best = 0;
+            i64 best_id = INT64_MAX, paired = -1;
+            for (int k = 0; k < n; k++) {
+                i64 nb = cn[k];
+                T dn = val(w, nb);
+                i64 tid = tri_id(g, v1, v2, nb);
+                if (a2 == 0) {
+                    if (cmp(nb, v1, w)) continue;  // main_3d.cu:2057
+                    if (cmp(nb, v2, w)) continue;  // main_3d.cu:2058
+                    if (!have || dn < best || (dn == best && nb < best_id)) {
+                        have = true;
+                        best = dn;
+                        best_id = nb;
+                        paired = tid;
+                    }
+                } else {
+                    bool c1 = cmp(v1, nb, w), c2 = cmp(v2, nb, w);  // main_3d.cu:2075-2078
+                    if (c1 != c2) {
+                        if (!have || dn < best || (dn == best && nb < best_id)) {
+                            have = true;
+                            best = dn;
+                            best_id = nb;
+                            paired = tid;
+                        }
+                    }
+                }
+            }
+            if (paired != -1 && PFw[paired] == -1) {  // main_3d.cu:2103-2109
+                PEw[e] = paired;
+                PFw[paired] = e;
+            }
+        }
+    }
+
+    // trianglePairingKernel main_3d.cu:2197-2316
+    void pass_triangles(int w, int a2) {
+        std::vector<i64>& PFw = pf[w];
+        std::vector<i64>& PTw = pt[w];
+#pragma omp parallel for schedule(dynamic, 4096)
+        for (i64 t = 0; t < g.F; t++) {
+            if (PFw[t] != -1) continue;
+            i64 v1, v2, v3;
+            tri_verts(g, t, v1, v2, v3);
+            i64 c[14];
+            int n = tri_cofacet_v4(g, v1, v2, v3, literal, c);
+            bool have = false;
+            T best = 0;
+            i64 best_id = INT64_MAX, paired = -1;
+            for (int k = 0; k < n; k++) {
+                i64 v4 = c[k];
+                i64 qid = tet_id(g, v1, v2, v3, v4);
+                T d4 = val(w, v4);
+                if (a2 == 0) {
+                    if (cmp(v4, v1, w) || cmp(v4, v2, w) || cmp(v4, v3, w)) continue;  // :2269-2271
+                    if (!have || d4 < best || (d4 == best && v4 < best_id)) {
+                        have = true;
+                        best = d4;
+                        best_id = v4;
+                        paired = qid;
+                    }
+                } else {
+                    bool c1 = cmp(v1, v4, w), c2 = cmp(v2, v4, w), c3 = cmp(v3, v4, w);
+                    if ((c1 && c2 && !c3) || (c1 && !c2 && c3) || (!c1 && c2 && c3)) {  // :2289
+                        if (!have || d4 < best || (d4 == best && v4 < best_id)) {
+                            have = true;
+                            best = d4;
+                            best_id = v4;
+                            paired = qid;
+                        }
+                    }
+                }
+            }
+            if (paired != -1 && PTw[paired] == -1) {  // main_3d.cu:2307-2313
+                PFw[t] = paired;
+                PTw[paired] = t;
+            }
+        }
+    }
+
+    // The pass sequence of main_3d.cu:6550-6564 (f) and 6864-6874 (g).
+    void gradient(int w) {
+        reset(w);
+        pass_vertices(w);
+        pass_edges(w, 0);
+        pass_triangles(w, 0);
+        pass_edges(w, 1);
+        pass_triangles(w, 1);
+    }
+
+    // ------------------------------------------------------------------------------------------
+    // Mismatch detection (main_3d.cu:2492-2550). Lists are sorted by id (the reference's order is
+    // that of atomicAdd, nondeterministic; only the set matters downstream).
+    // ------------------------------------------------------------------------------------------
+    static void xor_list(const std::vector<i64>& a, const std::vector<i64>& b, std::vector<i64>& out) {
+        out.clear();
+        for (size_t i = 0; i < a.size(); i++)
+            if ((a[i] == -1) != (b[i] == -1)) out.push_back((i64)i);
+    }
+    void detect_min() { xor_list(pv[0], pv[1], l_min); }
+    void detect_s1() { xor_list(pe[0], pe[1], l_s1); }
+    void detect_s2() { xor_list(pf[0], pf[1], l_s2); }
+    void detect_max() { xor_list(pt[0], pt[1], l_max); }
+
+    // ------------------------------------------------------------------------------------------
+    // Repair rules
+    // ------------------------------------------------------------------------------------------
+    // fix_minimum main_3d.cu:2554-2586
+    void fix_minimum(i64 id) {
+        if (PV(F_FIELD, id) != -1) {
+            i64 vertex, v2;
+            edge_verts(g, PV(F_FIELD, id), vertex, v2);
+            if (vertex != id) v2 = vertex;
+            tgt(v2);
+        } else {
+            tgt(id);
+        }
+    }
+
+    // fix_saddle main_3d.cu:2588-3037
+    void fix_saddle(i64 id) {
+        i64 edge = id;
+        i64 p = PE(F_FIELD, id);
+        if (p != -1) {
+            if (p < g.N && p >= 0 && PV(F_FIELD, p) == id) {  // :2602-2641
+                i64 v1, v2;
+                edge_verts(g, edge, v1, v2);
+                if (v2 != p) v1 = p;
+                else {
+                    v2 = v1;
+                    v1 = p;
+                }
+                i64 dv = PV(G_FIELD, v1);
+                if (dv == -1 || dv < 0 || dv >= g.E) return;
+                i64 fv1, fv2;
+                edge_verts(g, dv, fv1, fv2);
+                i64 false_v2 = (v1 == fv1) ? fv2 : fv1;
+                if (cmp(v2, false_v2, G_FIELD)) tgt(v2);
+            } else if (p < g.F && p >= 0 && PF(F_FIELD, p) == id) {  // :2643-2827
+                i64 face = p;
+                i64 v1, v2;
+                edge_verts(g, edge, v1, v2);
+                i64 a, b, v3;
+                tri_verts(g, face, a, b, v3);
+                if (a != v1 && a != v2) v3 = a;
+                else if (b != v1 && b != v2) v3 = b;
+                i64 other1 = !cmp(v1, v3, G_FIELD) ? v3 : v1;    // :2664
+                i64 other1_1 = !cmp(v2, v3, G_FIELD) ? v3 : v2;  // :2670
+                bool v1_edge = cmp(v1, v3, G_FIELD), v2_edge = cmp(v2, v3, G_FIELD);
+                bool o_v1 = cmp(v1, v3, F_FIELD), o_v2 = cmp(v2, v3, F_FIELD);
+                if (v1_edge && v2_edge) {
+                    tgt(v3);  // :2684-2699
+                } else if (o_v1 && o_v2) {
+                    tgt(v3);  // :2700-2708
+                } else {
+                    i64 q = PF(G_FIELD, p);
+                    if (q != -1 && PE(G_FIELD, q) == p) {  // :2719
+                        i64 false_edge = q;
+                        i64 a1, a2;
+                        edge_verts(g, false_edge, a1, a2);
+                        i64 false_v = (a1 != v1 && a1 != v2) ? a1 : a2;  // :2730
+                        double diff = delta(false_v);
+                        // :2737-2744: the false edge's endpoints and the face's third vertex
+                        i64 w1 = a1, w2 = a2;
+                        i64 b1, b2, b3;
+                        tri_verts(g, face, b1, b2, b3);
+                        if (b1 != w1 && b1 != w2) b3 = b1;
+                        else if (b2 != w1 && b2 != w2) b3 = b2;
+                        bool ov1 = cmp(w2, b3, F_FIELD), ov2 = cmp(w1, b3, F_FIELD);  // :2765-2766
+                        i64 pfe = PE(F_FIELD, false_edge);
+                        if (ov1 && ov2 && pfe != -1 && PF(F_FIELD, pfe) == false_edge) {  // :2768
+                            i64 c1, c2, c3;
+                            tri_verts(g, pfe, c1, c2, c3);
+                            if (c1 != w1 && c1 != w2) c3 = c1;
+                            else if (c2 != w1 && c2 != w2) c3 = c2;
+                            tgt(c3);
+                        } else if (std::fabs(diff) > 0) {  // :2787
+                            tgt(false_v);
+                        }
+                    } else {  // :2801-2823
+                        if (o_v1 && !v1_edge) tgt(other1);
+                        else tgt(other1_1);
+                    }
+                }
+            }
+        } else {  // false negative, :2830-3036
+            i64 q = PE(G_FIELD, id);
+            if (q != -1 && q < g.N && q >= 0 && PV(G_FIELD, q) == id) {  // :2839-2850
+                tgt(q);
+            } else if (q != -1 && q < g.F && q >= 0 && PF(G_FIELD, q) == id) {  // :2852
+                i64 face = q;
+                i64 v1, v2;
+                edge_verts(g, edge, v1, v2);
+                i64 a, b, v3;
+                tri_verts(g, face, a, b, v3);
+                if (a != v1 && a != v2) v3 = a;
+                else if (b != v1 && b != v2) v3 = b;
+                bool v1_edge = cmp(v2, v3, G_FIELD), v2_edge = cmp(v1, v3, G_FIELD);  // :2886-2887
+                bool or1 = cmp(v2, v3, F_FIELD), or2 = cmp(v1, v3, F_FIELD);
+                if (v1_edge == v2_edge) {
+                    if (v1_edge != or1) tgt(v2);
+                    if (v2_edge != or2) tgt(v1);
+                } else {
+                    i64 t = PE(F_FIELD, id);
+                    if (t != -1) {
+                        if (t >= 0 && t < g.N && PV(F_FIELD, t) == id) tgt(v1 == t ? v2 : v1);
+                    } else {
+                        // :2942-3003 — lowest f-vertex strictly between the endpoints
+                        i64 cn[14];
+                        int n = edge_common(g, v1, v2, literal, cn);
+                        bool have = false;
+                        T best = 0;
+                        i64 best_id = INT64_MAX;
+                        for (int k = 0; k < n; k++) {
+                            i64 nb = cn[k];
+                            bool c1 = cmp(v1, nb, F_FIELD), c2 = cmp(v2, nb, F_FIELD);
+                            if (c1 != c2) {
+                                T dn = f[nb];
+                                if (!have || dn < best || (dn == best && nb < best_id)) {
+                                    have = true;
+                                    best = dn;
+                                    best_id = nb;
+                                }
+                            }
+                        }
+                        if (have) tgt(best_id);
+                        else {
+                            tgt(v1);
+                            tgt(v2);
+                        }
+                    }
+                }
+            }
+        }
+    }
+
+    // get_smallest_vertex main_3d.cu:2115-2194
+    i64 smallest_vertex(i64 v1, i64 v2, i64 v3) {
+        i64 c[14];
+        int n = tri_cofacet_v4(g, v1, v2, v3, literal, c);
+        bool have = false;
+        T best = 0;
+        i64 best_id = INT64_MAX;
+        for (int k = 0; k < n; k++) {
+            i64 v4 = c[k];
+            bool c1 = cmp(v1, v4, F_FIELD), c2 = cmp(v2, v4, F_FIELD), c3 = cmp(v3, v4, F_FIELD);
+            if ((c1 && c2 && !c3) || (c1 && !c2 && c3) || (!c1 && c2 && c3)) {
+                T d4 = f[v4];
+                if (!have || d4 < best || (d4 == best && v4 < best_id)) {
+                    have = true;
+                    best = d4;
+                    best_id = v4;
+                }
+            }
+        }
+        return have ? best_id : -1;
+    }
+
+    static inline i64 other_of_tet(i64 q1, i64 q2, i64 q3, i64 q4, i64 a, i64 b, i64 c) {
+        // main_3d.cu:3237-3239 style: first tet vertex not in the face, else the 4th
+        i64 v4 = q4;
+        if (q1 != a && q1 != b && q1 != c) v4 = q1;
+        else if (q2 != a && q2 != b && q2 != c) v4 = q2;
+        else if (q3 != a && q3 != b && q3 != c) v4 = q3;
+        return v4;
+    }
+
+    // fix_2saddle main_3d.cu:3039-3983
+    void fix_2saddle(i64 id) {
+        i64 face = id;
+        i64 p = PF(F_FIELD, id);
+        if (p != -1) {
+            if (p < g.E && p >= 0 && PE(F_FIELD, p) == id) {  // :3057-3223
+                i64 edge = p;
+                i64 v1, v2;
+                edge_verts(g, edge, v1, v2);
+                i64 a, b, v3;
+                tri_verts(g, face, a, b, v3);
+                if (a != v1 && a != v2) v3 = a;
+                else if (b != v1 && b != v2) v3 = b;
+                bool v1_edge = cmp(v2, v3, G_FIELD), v2_edge = cmp(v1, v3, G_FIELD);
+                bool o_v1 = cmp(v2, v3, F_FIELD), o_v2 = cmp(v1, v3, F_FIELD);
+                if (v1_edge && v2_edge) {  // :3097
+                    i64 f_id = PE(F_FIELD, edge);
+                    if (f_id == -1 || PF(F_FIELD, f_id) != edge) return;
+                    i64 c1, c2, c3;
+                    tri_verts(g, f_id, c1, c2, c3);
+                    if (c1 != v1 && c1 != v2) c3 = c1;
+                    else if (c2 != v1 && c2 != v2) c3 = c2;
+                    tgt(c3);
+                } else if (o_v1 != o_v2 && v1_edge != v2_edge) {  // :3120
+                    i64 f_id = PE(G_FIELD, edge);
+                    if (f_id == -1 || PF(G_FIELD, f_id) != edge) {
+                        tgt(v3);
+                    } else if (f_id < g.F && f_id >= 0) {  // :3135
+                        i64 d1, d2, v3_1;
+                        tri_verts(g, f_id, d1, d2, v3_1);
+                        if (d1 != v1 && d1 != v2) v3_1 = d1;
+                        else if (d2 != v1 && d2 != v2) v3_1 = d2;
+                        bool e1 = cmp(v2, v3_1, G_FIELD), e2 = cmp(v1, v3_1, G_FIELD);
+                        bool oe1 = cmp(v2, v3_1, F_FIELD), oe2 = cmp(v1, v3_1, F_FIELD);
+                        if (e1 && e2 && ((!oe1 && oe2) || (oe1 && !oe2))) {
+                            if (!oe1) tgt(v2);
+                            else tgt(v1);
+                        } else {
+                            tgt(v3);
+                        }
+                    }
+                } else if (!cmp(v1, v3, G_FIELD) || !cmp(v2, v3, G_FIELD)) {  // :3205
+                    tgt(v3);
+                }
+            } else if (p < g.T && p >= 0 && PT(F_FIELD, p) == id) {  // :3226-3477
+                i64 tet = p;
+                i64 v1, v2, v3;
+                tri_verts(g, id, v1, v2, v3);
+                i64 q1, q2, q3, q4;
+                tet_verts(g, tet, q1, q2, q3, q4);
+                i64 v4 = other_of_tet(q1, q2, q3, q4, v1, v2, v3);
+                bool or1 = cmp(v3, v4, F_FIELD), or2 = cmp(v2, v4, F_FIELD), or3 = cmp(v1, v4, F_FIELD);
+                if (!or1 || !or2 || !or3) {
+                    i64 pff = PT(G_FIELD, tet);
+                    if (pff != -1) {
+                        i64 w1, w2, w3;
+                        tri_verts(g, pff, w1, w2, w3);
+                        bool g1 = cmp(w3, v4, G_FIELD), g2 = cmp(w2, v4, G_FIELD), g3 = cmp(w1, v4, G_FIELD);
+                        bool h1 = cmp(w3, v4, F_FIELD), h2 = cmp(w2, v4, F_FIELD), h3 = cmp(w1, v4, F_FIELD);
+                        if ((!g1 || !g2 || !g3) && (!h1 || !h2 || !h3)) {
+                            i64 idd = PF(F_FIELD, pff);
+                            if (idd >= 0 && idd < g.E && PE(F_FIELD, idd) == pff) {  // :3288
+                                i64 e1, e2;
+                                edge_verts(g, idd, e1, e2);
+                                i64 vv3 = w3;
+                                if (w1 != e1 && e2 != w1) vv3 = w1;  // :3293
+                                else if (w2 != e1 && w2 != e2) vv3 = w2;
+                                tgt(vv3);
+                            } else if (idd >= 0 && idd < g.T && PT(F_FIELD, idd) == pff) {  // :3302
+                                i64 t1, t2, t3, t4;
+                                tet_verts(g, idd, t1, t2, t3, t4);
+                                // :3307-3309 assign the face's vertex, not the tet's
+                                if (t1 != w1 && t1 != w2 && t1 != w3) t4 = w1;
+                                else if (t2 != w1 && t2 != w2 && t2 != w3) t4 = w2;
+                                else if (t3 != w1 && t3 != w2 && t3 != w3) t4 = w3;
+                                tgt(t1);
+                                tgt(t2);
+                                tgt(t3);
+                                tgt(t4);
+                            }
+                        } else {  // :3324-3334
+                            tgt(w1);
+                            tgt(w2);
+                            tgt(w3);
+                        }
+                    } else {
+                        tgt(v4);  // :3338-3345
+                    }
+                } else {
+                    tgt(v4);  // :3351-3357
+                }
+            }
+        } else {  // false negative :3480-3981
+            i64 q = PF(G_FIELD, id);
+            if (q != -1 && q < g.E && PE(G_FIELD, q) == id) {  // :3488-3610
+                i64 edge_id2 = q;
+                i64 v1, v2;
+                edge_verts(g, edge_id2, v1, v2);
+                i64 s1 = !cmp(v1, v2, G_FIELD) ? v2 : v1;
+                i64 s2 = !cmp(v1, v2, G_FIELD) ? v1 : v2;
+                i64 tf = PE(F_FIELD, edge_id2);
+                if (tf != -1 && PF(F_FIELD, tf) == edge_id2) {  // :3522
+                    i64 a, b, v3;
+                    tri_verts(g, tf, a, b, v3);
+                    if (s1 != a && s2 != a) v3 = a;
+                    if (s1 != b && s2 != b) v3 = b;
+                    tgt(v3);
+                } else if (tf >= 0 && tf < g.N && PV(F_FIELD, tf) == edge_id2) {  // :3544
+                    tgt(v1);
+                    tgt(v2);
+                }
+            } else if (q != -1 && q < g.T && PT(G_FIELD, q) == id) {  // :3613-3980
+                i64 tet = q;
+                i64 v1, v2, v3;
+                tri_verts(g, face, v1, v2, v3);
+                i64 q1, q2, q3, q4;
+                tet_verts(g, tet, q1, q2, q3, q4);
+                i64 v4 = other_of_tet(q1, q2, q3, q4, v1, v2, v3);
+                bool f1 = cmp(v3, v4, G_FIELD), f2 = cmp(v2, v4, G_FIELD), f3 = cmp(v1, v4, G_FIELD);
+                bool o1 = cmp(v3, v4, F_FIELD), o2 = cmp(v2, v4, F_FIELD), o3 = cmp(v1, v4, F_FIELD);
+                if (f1 && f2 && f3) {
+                    // :3714-3927; the inner "else" branches are unreachable since f1,f2,f3 hold
+                    if (f1 != o1) tgt(v3);
+                    else if (f2 != o2) tgt(v2);
+                    else if (f3 != o3) tgt(v1);
+                } else {
+                    i64 true_cell = PT(F_FIELD, tet);
+                    if (true_cell >= 0 && true_cell < g.T) {  // :3937 (face id compared to T)
+                        tgt(v1);
+                        tgt(v2);
+                        tgt(v3);
+                    } else {
+                        i64 m = smallest_vertex(v1, v2, v3);
+                        if (m != -1) tgt(m);
+                        else {
+                            tgt(v1);
+                            tgt(v2);
+                            tgt(v3);
+                        }
+                    }
+                }
+            }
+        }
+    }
+
+    // fix_maximum main_3d.cu:3985-4303
+    void fix_maximum(i64 id) {
+        i64 p = PT(F_FIELD, id);
+        if (p != -1) {  // false positive
+            i64 face1 = p;
+            i64 v1, v2, v3;
+            tri_verts(g, face1, v1, v2, v3);
+            i64 q1, q2, q3, q4;
+            tet_verts(g, id, q1, q2, q3, q4);
+            i64 v4 = other_of_tet(q1, q2, q3, q4, v1, v2, v3);
+            bool o1 = cmp(v1, v4, F_FIELD), o2 = cmp(v2, v4, F_FIELD), o3 = cmp(v3, v4, F_FIELD);
+            if (o1 && o2 && o3) {
+                tgt(v4);
+            } else {
+                i64 fc = PF(G_FIELD, face1);
+                if (fc >= 0) {
+                    if (fc < g.E && PE(G_FIELD, fc) == face1) {
+                        i64 a, b;
+                        edge_verts(g, fc, a, b);
+                        tgt(cmp(a, b, G_FIELD) ? a : b);
+                    } else if (fc < g.T && PT(G_FIELD, fc) == face1) {
+                        tgt(v4);
+                    }
+                }
+            }
+        } else {  // false negative
+            i64 face1 = PT(G_FIELD, id);
+            i64 v1, v2, v3;
+            tri_verts(g, face1, v1, v2, v3);
+            i64 q1, q2, q3, q4;
+            tet_verts(g, id, q1, q2, q3, q4);
+            i64 v4 = other_of_tet(q1, q2, q3, q4, v1, v2, v3);
+            bool f1 = cmp(v1, v4, G_FIELD), f2 = cmp(v2, v4, G_FIELD), f3 = cmp(v3, v4, G_FIELD);
+            bool o1 = cmp(v1, v4, F_FIELD), o2 = cmp(v2, v4, F_FIELD), o3 = cmp(v3, v4, F_FIELD);
+            if (f1 && f2 && f3) {
+                if (!o1) tgt(v1);
+                if (!o2) tgt(v2);
+                if (!o3) tgt(v3);
+                if (o1 && o2 && o3) {
+                    i64 it = PF(F_FIELD, face1);
+                    if (it != -1) {
+                        i64 t1, t2, t3, t4;
+                        tet_verts(g, it, t1, t2, t3, t4);
+                        tgt(other_of_tet(t1, t2, t3, t4, v1, v2, v3));
+                    }
+                }
+            } else {
+                i64 pid = PF(F_FIELD, face1);
+                if (pid != -1) {
+                    if (pid >= 0 && pid < g.E && PE(F_FIELD, pid) == face1) {
+                        i64 a, b;
+                        edge_verts(g, pid, a, b);
+                        tgt(a);
+                        tgt(b);
+                    } else {
+                        i64 t1, t2, t3, t4;
+                        tet_verts(g, pid, t1, t2, t3, t4);
+                        tgt(t1);
+                        tgt(t2);
+                        tgt(t3);
+                        tgt(t4);
+                    }
+                } else {
+                    if (!o1) tgt(v1);
+                    if (!o2) tgt(v2);
+                    if (!o3) tgt(v3);
+                }
+            }
+        }
+    }
+
+    // ------------------------------------------------------------------------------------------
+    // Separatrix checks (main_3d.cu:4437-5062) and repairs (main_3d.cu:5135-5295)
+    // ------------------------------------------------------------------------------------------
+    // is_false main_3d.cu:4437-4497. Returns true and records the divergent tet if found.
+    bool is_false(i64 id, i64& divergent) {
+        i64 cur = id, old;
+        i64 steps = 0;
+        do {
+            old = cur;
+            i64 t = PT(F_FIELD, cur);
+            i64 dt = PT(G_FIELD, cur);
+            if (t == -1) break;
+            if (t != dt) {
+                divergent = cur;
+                return true;
+            }
+            i64 pft = PF(F_FIELD, t);
+            if (pft == -1 || PT(F_FIELD, pft) != t) break;
+            i64 t1, t2;
+            cell_tetra(g, t, t1, t2, literal);
+            cur = (t1 == cur) ? t2 : t1;
+            if (++steps > g.N) {
+                walk_overflow++;
+                break;
+            }
+        } while (cur != old && cur != -1);
+        return false;
+    }
+
+    // get_paired_id main_3d.cu:4499-4537 (which = field)
+    i64 paired_max(i64 id, int which) {
+        i64 cur = id, old;
+        i64 steps = 0;
+        do {
+            old = cur;
+            i64 t = PT(which, cur);
+            if (t == -1) break;
+            i64 pft = PF(which, t);
+            if (pft == -1 || PT(which, pft) != t) break;
+            i64 t1, t2;
+            cell_tetra(g, t, t1, t2, literal);
+            cur = (t1 == cur) ? t2 : t1;
+            if (++steps > g.N) {
+                walk_overflow++;
+                break;
+            }
+        } while (cur != old && cur != -1);
+        return cur;
+    }
+
+    // Cofacet tets of a face in the reference enumeration order (main_3d.cu:4567-4606).
+    int face_tets(i64 face, i64 out[2]) {
+        i64 v1, v2, v3;
+        tri_verts(g, face, v1, v2, v3);
+        i64 c[14];
+        int n = tri_cofacet_v4(g, v1, v2, v3, literal, c);
+        for (int k = 0; k < n && k < 2; k++) out[k] = tet_id(g, v1, v2, v3, c[k]);
+        return std::min(n, 2);
+    }
+
+    // get_false_as_vpath main_3d.cu:4540-4610
+    void detect_as_geometry() {
+        l_as.clear();
+        std::vector<std::vector<i64>> loc;
+#pragma omp parallel
+        {
+#pragma omp single
+            {
+#ifdef _OPENMP
+                loc.resize(omp_get_num_threads());
+#else
+                loc.resize(1);
+#endif
+            }
+#ifdef _OPENMP
+            std::vector<i64>& my = loc[omp_get_thread_num()];
+#else
+            std::vector<i64>& my = loc[0];
+#endif
+#pragma omp for schedule(dynamic, 1024)
+            for (i64 i = 0; i < g.F; i++) {
+                if (pf[0][i] != -1) continue;
+                i64 tets[2];
+                int n = face_tets(i, tets);
+                for (int k = 0; k < n; k++) {
+                    i64 dv;
+                    if (is_false(tets[k], dv)) {
+                        my.push_back(tets[k]);
+                        as_index[2 * tets[k]] = dv;  // same value from every writer
+                        as_index[2 * tets[k] + 1] = 3;
+                        break;
+                    }
+                }
+            }
+        }
+        for (auto& v : loc) l_as.insert(l_as.end(), v.begin(), v.end());
+        std::sort(l_as.begin(), l_as.end());
+    }
+
+    // get_false_ds_vpath main_3d.cu:4837-4959. The reference's dec_current_id (:4849) is never
+    // advanced, so the loop leaves at the second iteration through the v != dec_v test (:4861):
+    // only the start vertex's f and g pairings are compared. Restated with the same variables.
+    bool ds_walk_geometry(i64 start, i64& rec) {
+        i64 cur = start, dec_cur = start;
+        i64 v = start;
+        do {
+            v = cur;
+            i64 dec_v = dec_cur;
+            if (v != dec_v) break;
+            i64 ce = PV(F_FIELD, v);
+            if (ce == -1) break;
+            i64 dce = PV(G_FIELD, v);
+            if (ce != dce) {
+                rec = v;
+                return true;
+            }
+            if (PE(F_FIELD, ce) == -1) break;
+            i64 a, b;
+            edge_verts(g, ce, a, b);
+            cur = (a == cur) ? b : a;
+        } while (PV(F_FIELD, v) != -1 && PV(G_FIELD, v) != -1);
+        return false;
+    }
+    void detect_ds_geometry() {
+        l_ds.clear();
+        std::vector<std::vector<i64>> loc;
+#pragma omp parallel
+        {
+#pragma omp single
+            {
+#ifdef _OPENMP
+                loc.resize(omp_get_num_threads());
+#else
+                loc.resize(1);
+#endif
+            }
+#ifdef _OPENMP
+            std::vector<i64>& my = loc[omp_get_thread_num()];
+#else
+            std::vector<i64>& my = loc[0];
+#endif
+#pragma omp for schedule(dynamic, 1024)
+            for (i64 i = 0; i < g.E; i++) {
+                if (pe[0][i] != -1) continue;
+                i64 v1, v2;
+                edge_verts(g, i, v1, v2);
+                i64 rec;
+                if (ds_walk_geometry(v1, rec) || ds_walk_geometry(v2, rec)) {
+                    my.push_back(i);
+                    ds_index[2 * i] = rec;
+                    ds_index[2 * i + 1] = 0;
+                }
+            }
+        }
+        for (auto& v : loc) l_ds.insert(l_ds.end(), v.begin(), v.end());
+        std::sort(l_ds.begin(), l_ds.end());
+    }
+
+    // f descending labels, get_orginal_saddle_min_pairs main_3d.cu:4613-4687
+    i64 f_min_label(i64 start) {
+        i64 cur = start, v = start;
+        i64 steps = 0;
+        while (true) {
+            v = cur;
+            i64 ce = PV(F_FIELD, v);
+            if (ce == -1) break;
+            if (PE(F_FIELD, ce) == -1) break;
+            i64 a, b;
+            edge_verts(g, ce, a, b);
+            cur = (a == cur) ? b : a;
+            if (++steps > g.N) {
+                walk_overflow++;
+                break;
+            }
+        }
+        return v;
+    }
+    void compute_labels() {
+        min_label.assign(2 * g.E, -1);
+        max_label.assign(2 * g.F, -1);
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (i64 i = 0; i < g.E; i++) {
+            if (pe[0][i] != -1) continue;
+            i64 v1, v2;
+            edge_verts(g, i, v1, v2);
+            min_label[2 * i] = f_min_label(v1);
+            min_label[2 * i + 1] = f_min_label(v2);
+        }
+        // get_orginal_saddle_max_pairs main_3d.cu:4689-4759, indexed by face id (shadowing fix)
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (i64 i = 0; i < g.F; i++) {
+            if (pf[0][i] != -1) continue;
+            i64 tets[2];
+            int n = face_tets(i, tets);
+            for (int k = 0; k < n; k++) max_label[2 * i + k] = paired_max(tets[k], F_FIELD);
+        }
+    }
+
+    // get_false_ds_vpath_connectivity main_3d.cu:4961-5062
+    bool ds_walk_conn(i64 start, i64& trouble, i64& end) {
+        i64 cur = start, v = start;
+        i64 steps = 0;
+        while (true) {
+            v = cur;
+            i64 dce = PV(G_FIELD, v);
+            if (dce == -1) break;
+            i64 ce = PV(F_FIELD, v);
+            if (ce != dce && trouble == -1) trouble = v;
+            if (PE(G_FIELD, dce) == -1) break;
+            i64 a, b;
+            edge_verts(g, dce, a, b);
+            cur = (a == cur) ? b : a;
+            if (++steps > g.N) {
+                walk_overflow++;
+                break;
+            }
+        }
+        end = v;
+        return true;
+    }
+    void detect_ds_connectivity() {
+        l_ds.clear();
+        for (i64 i = 0; i < g.E; i++) {
+            if (pe[0][i] != -1) continue;
+            i64 v1, v2;
+            edge_verts(g, i, v1, v2);
+            i64 trouble = -1, end;
+            ds_walk_conn(v1, trouble, end);
+            if (end != min_label[2 * i]) {
+                l_ds.push_back(i);
+                ds_index[2 * i] = trouble;
+                ds_index[2 * i + 1] = 0;
+                continue;
+            }
+            ds_walk_conn(v2, trouble, end);
+            if (end != min_label[2 * i + 1]) {
+                l_ds.push_back(i);
+                ds_index[2 * i] = trouble;
+                ds_index[2 * i + 1] = 0;
+            }
+        }
+    }
+    // get_false_as_vpath_connectivity main_3d.cu:4761-4835 (labels indexed by face id)
+    void detect_as_connectivity() {
+        l_as.clear();
+        for (i64 i = 0; i < g.F; i++) {
+            if (pf[1][i] != -1) continue;
+            i64 tets[2];
+            int n = face_tets(i, tets);
+            for (int k = 0; k < n; k++) {
+                i64 dec_id = paired_max(tets[k], G_FIELD);
+                if (dec_id != max_label[2 * i + k]) {
+                    i64 dv;
+                    if (is_false(tets[k], dv)) {
+                        l_as.push_back(tets[k]);
+                        as_index[2 * tets[k]] = dv;
+                        as_index[2 * tets[k] + 1] = 3;
+                        break;
+                    }
+                }
+            }
+        }
+        std::sort(l_as.begin(), l_as.end());
+    }
+
+    // Deterministic walls: get_false_ds_wall main_3d.cu:5064-5133 with a private stack and
+    // visited set per start face, edges in the reference order (e1,e2,e3), LIFO.
+    void detect_ds_wall() {
+        l_dsw.clear();
+        std::vector<uint8_t> visited(g.F, 0);
+        std::vector<i64> touched, stack;
+        for (i64 i = 0; i < g.F; i++) {
+            if (pf[0][i] != -1) continue;
+            for (i64 t : touched) visited[t] = 0;
+            touched.clear();
+            stack.clear();
+            stack.push_back(i);
+            bool found = false;
+            while (!stack.empty() && !found) {
+                i64 tri = stack.back();
+                stack.pop_back();
+                if (visited[tri]) continue;
+                visited[tri] = 1;
+                touched.push_back(tri);
+                i64 v1, v2, v3;
+                tri_verts(g, tri, v1, v2, v3);
+                i64 x1, y1, z1, x2, y2, z2, x3, y3, z3;
+                g.xyz(v1, x1, y1, z1);
+                g.xyz(v2, x2, y2, z2);
+                g.xyz(v3, x3, y3, z3);
+                i64 es[3] = {edge_id(g, x1, y1, z1, x2, y2, z2), edge_id(g, x1, y1, z1, x3, y3, z3),
+                             edge_id(g, x2, y2, z2, x3, y3, z3)};
+                for (int k = 0; k < 3; k++) {
+                    i64 j = es[k];
+                    i64 pj = PE(F_FIELD, j);
+                    if (pj == -1) continue;
+                    if (pj != PE(G_FIELD, j)) {
+                        l_dsw.push_back(i);
+                        dsw_index[i] = j;
+                        found = true;
+                        break;
+                    }
+                    if (pj != tri && pj >= 0 && pj < g.F && PF(F_FIELD, pj) == j) stack.push_back(pj);
+                }
+            }
+        }
+    }
+
+    // fix_as_vpath main_3d.cu:5135-5244 (only the live else-branch, defect D14)
+    void fix_as_vpath(i64 id) {
+        i64 t_idx = as_index[2 * id];
+        i64 true_edge = PT(F_FIELD, t_idx);
+        i64 false_edge = PT(G_FIELD, t_idx);
+        if (false_edge == -1 || true_edge == -1) return;  // unreachable with matching critical sets
+        i64 pfe = PF(F_FIELD, false_edge);
+        if (pfe >= 0 && pfe < g.T && PT(F_FIELD, pfe) == false_edge) {
+            i64 v1, v2, v3;
+            tri_verts(g, false_edge, v1, v2, v3);
+            i64 q1, q2, q3, q4;
+            tet_verts(g, pfe, q1, q2, q3, q4);
+            tgt(other_of_tet(q1, q2, q3, q4, v1, v2, v3));
+        } else {
+            i64 v1, v2, v3;
+            tri_verts(g, true_edge, v1, v2, v3);
+            i64 a, b, tv3;
+            tri_verts(g, false_edge, a, b, tv3);
+            if (a != v1 && a != v2 && a != v3) tv3 = a;
+            else if (b != v1 && b != v2 && b != v3) tv3 = b;
+            tgt(tv3);
+        }
+    }
+
+    // fix_ds_vpath main_3d.cu:5246-5295 (live else-branch)
+    void fix_ds_vpath(i64 id) {
+        i64 v_idx = ds_index[2 * id];
+        if (v_idx < 0) return;  // trouble_maker == -1: the reference reads index -1
+        i64 tpe = PV(F_FIELD, v_idx);
+        if (tpe == -1) return;
+        i64 a, b;
+        edge_verts(g, tpe, a, b);
+        tgt(v_idx == a ? b : a);
+    }
+
+    // fix_ds_wall main_3d.cu:5297-5413
+    void fix_ds_wall(i64 i) {
+        i64 e = dsw_index[i];
+        i64 tf = PE(F_FIELD, e), ff = PE(G_FIELD, e);
+        if (tf == ff) return;
+        if (tf >= 0 && ff >= 0 && tf < g.F && PF(F_FIELD, tf) == e && ff < g.F && PF(G_FIELD, ff) == e) {
+            i64 v1, v2;
+            edge_verts(g, e, v1, v2);
+            i64 a, b, v3;
+            tri_verts(g, tf, a, b, v3);
+            if (a != v1 && a != v2) v3 = a;
+            if (b != v1 && b != v2) v3 = b;
+            double diff = delta(v3);
+            if (std::fabs(diff) > 1e-16 * range) tgt(v3);
+            else {
+                tgt(v1);
+                tgt(v2);
+            }
+        } else if (tf == -1 || tf >= g.F || (tf < g.F && tf >= 0 && PF(F_FIELD, tf) != e)) {
+            i64 v1, v2;
+            edge_verts(g, e, v1, v2);
+            tgt(v1);
+            tgt(v2);
+        } else if (ff == -1 || ff >= g.F || (ff < g.F && ff >= 0 && PF(G_FIELD, ff) != e)) {
+            i64 v1, v2;
+            edge_verts(g, e, v1, v2);
+            i64 a, b, v3;
+            tri_verts(g, tf, a, b, v3);
+            if (a != v1 && a != v2) v3 = a;
+            else if (b != v1 && b != v2) v3 = b;
+            tgt(v3);
+        }
+    }
+
+    // ------------------------------------------------------------------------------------------
+    // Edit buffer apply (main_3d.cu:1122-1164) and filtration (main_3d.cu:2405-2490)
+    // ------------------------------------------------------------------------------------------
+    i64 apply() {
+        i64 n = 0;
+#pragma omp parallel for schedule(static) reduction(+ : n)
+        for (i64 v = 0; v < g.N; v++) {
+            if (!target[v]) continue;
+            target[v] = 0;
+            n++;
+            double dl = delta(v);
+            if (fl.preserve_type == 0) {
+                T gv = d[v];
+                if (std::fabs(dl) > 1e-15 * range && counter[v] < 5 && Arith<T>::step(f[v], gv, dl, eps)) {
+                    d[v] = gv;
+                    counter[v] += 1;
+                } else {
+                    counter[v] = 6;
+                    d[v] = Arith<T>::snap(f[v], eps);
+                }
+            } else {
+                counter[v] = 6;
+                d[v] = Arith<T>::snap(f[v], eps);
+            }
+        }
+        return n;
+    }
+
+    void pin(i64 v) {
+        d[v] = Arith<T>::snap(f[v], eps);
+        counter[v] = 6;
+    }
+    void filtration() {
+        for (i64 v = 0; v < g.N; v++)
+            if (pv[0][v] == -1) pin(v);
+        for (i64 e = 0; e < g.E; e++)
+            if (pe[0][e] == -1) {
+                i64 a, b;
+                edge_verts(g, e, a, b);
+                pin(a);
+                pin(b);
+            }
+        for (i64 t = 0; t < g.F; t++)
+            if (pf[0][t] == -1) {
+                i64 a, b, c;
+                tri_verts(g, t, a, b, c);
+                pin(a);
+                pin(b);
+                pin(c);
+            }
+        for (i64 q = 0; q < g.T; q++)
+            if (pt[0][q] == -1) {
+                i64 a, b, c, e;
+                tet_verts(g, q, a, b, c, e);
+                pin(a);
+                pin(b);
+                pin(c);
+                pin(e);
+            }
+    }
+
+    template <class Fn> void for_list(const std::vector<i64>& l, Fn fn) {
+        i64 n = (i64)l.size();
+#pragma omp parallel for schedule(dynamic, 256)
+        for (i64 k = 0; k < n; k++) fn(l[k]);
+    }
+
+    void clear_targets() { std::fill(target.begin(), target.end(), 0); }
+
+    void detect_paths() {
+        if (fl.preserve_geometry == 1) {
+            detect_as_geometry();
+            detect_ds_geometry();
+        } else if (fl.preserve_vpath == 1) {
+            detect_as_connectivity();
+            detect_ds_connectivity();
+        }
+        if (fl.preserve_connectors == 1) detect_ds_wall();
+        else l_dsw.clear();
+    }
+};
+
+}  // namespace orc
+
+using namespace orc;
+
+// ================================================================================================
+// C ABI for ctypes
+// ================================================================================================
+typedef struct {
+    int32_t preserve_saddles, preserve_vpath, preserve_geometry, filtration, preserve_connectors,
+        preserve_type, edit_type, max_sweeps;
+} orc_flags;
+
+extern "C" int orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
+extern "C" void orc_counts(i64 W, i64 H, i64 D, i64* out4) {
+    Grid g(W, H, D);
+    out4[0] = g.N;
+    out4[1] = g.E;
+    out4[2] = g.F;
+    out4[3] = g.T;
+}
+
+extern "C" i64 orc_edge_id(i64 W, i64 H, i64 D, i64 a, i64 b) {
+    init_dirs();
+    Grid g(W, H, D);
+    i64 x, y, z, x1, y1, z1;
+    g.xyz(a, x, y, z);
+    g.xyz(b, x1, y1, z1);
+    return edge_id(g, x, y, z, x1, y1, z1);
+}
+extern "C" void orc_edge_verts(i64 W, i64 H, i64 D, i64 e, i64* out2) {
+    Grid g(W, H, D);
+    edge_verts(g, e, out2[0], out2[1]);
+}
+extern "C" i64 orc_tri_id(i64 W, i64 H, i64 D, i64 a, i64 b, i64 c) {
+    Grid g(W, H, D);
+    return tri_id(g, a, b, c);
+}
+extern "C" void orc_tri_verts(i64 W, i64 H, i64 D, i64 t, i64* out3) {
+    Grid g(W, H, D);
+    tri_verts(g, t, out3[0], out3[1], out3[2]);
+}
+extern "C" i64 orc_tet_id(i64 W, i64 H, i64 D, i64 a, i64 b, i64 c, i64 d) {
+    Grid g(W, H, D);
+    return tet_id(g, a, b, c, d);
+}
+extern "C" void orc_tet_verts(i64 W, i64 H, i64 D, i64 q, i64* out4) {
+    Grid g(W, H, D);
+    tet_verts(g, q, out4[0], out4[1], out4[2], out4[3]);
+}
+extern "C" void orc_cell_tetra(i64 W, i64 H, i64 D, i64 t, int literal, i64* out2) {
+    init_dirs();
+    Grid g(W, H, D);
+    cell_tetra(g, t, out2[0], out2[1], literal != 0);
+}
+
+// Gradient of one field. pv/pe/pf/pt: caller arrays of N/E/F/T int64.
+template <class T>
+static int gradient_impl(const T* fld, i64 W, i64 H, i64 D, int literal, i64* pv, i64* pe, i64* pf, i64* pt) {
+    init_dirs();
+    State<T> s(W, H, D);
+    s.f = fld;
+    s.d = const_cast<T*>(fld);
+    s.literal = literal != 0;
+    for (int w = 0; w < 2; w++) {
+        s.pv[w].assign(s.g.N, -1);
+        s.pe[w].assign(s.g.E, -1);
+        s.pf[w].assign(s.g.F, -1);
+        s.pt[w].assign(s.g.T, -1);
+    }
+    s.gradient(F_FIELD);
+    std::memcpy(pv, s.pv[0].data(), sizeof(i64) * s.g.N);
+    std::memcpy(pe, s.pe[0].data(), sizeof(i64) * s.g.E);
+    std::memcpy(pf, s.pf[0].data(), sizeof(i64) * s.g.F);
+    std::memcpy(pt, s.pt[0].data(), sizeof(i64) * s.g.T);
+    return 0;
+}
+extern "C" int orc_gradient_f32(const float* fld, i64 W, i64 H, i64 D, int literal, i64* pv, i64* pe, i64* pf, i64* pt) {
+    return gradient_impl<float>(fld, W, H, D, literal, pv, pe, pf, pt);
+}
+extern "C" int orc_gradient_f64(const double* fld, i64 W, i64 H, i64 D, int literal, i64* pv, i64* pe, i64* pf, i64* pt) {
+    return gradient_impl<double>(fld, W, H, D, literal, pv, pe, pf, pt);
+}
+
+// One fix phase on (f, g) without applying: targets[N] receives 0/1, counts4 the mismatch counts.
+template <class T>
+static int targets_impl(const T* f, const T* gfield, i64 W, i64 H, i64 D, double eps, double range,
+                        const orc_flags* fl, uint8_t* targets, i64* counts4) {
+    init_dirs();
+    State<T> s(W, H, D);
+    s.f = f;
+    std::vector<T> gcopy(gfield, gfield + s.g.N);
+    s.d = gcopy.data();
+    s.eps = eps;
+    s.range = range;
+    std::memcpy(&s.fl, fl, sizeof(orc_flags));
+    s.literal = false;
+    s.alloc();
+    s.gradient(F_FIELD);
+    s.gradient(G_FIELD);
+    s.detect_min();
+    s.detect_s1();
+    s.detect_s2();
+    s.detect_max();
+    counts4[0] = (i64)s.l_min.size();
+    counts4[1] = (i64)s.l_s1.size();
+    counts4[2] = (i64)s.l_s2.size();
+    counts4[3] = (i64)s.l_max.size();
+    s.for_list(s.l_min, [&](i64 id) { s.fix_minimum(id); });
+    if (fl->preserve_saddles) {
+        s.for_list(s.l_s1, [&](i64 id) { s.fix_saddle(id); });
+        s.for_list(s.l_s2, [&](i64 id) { s.fix_2saddle(id); });
+    }
+    s.for_list(s.l_max, [&](i64 id) { s.fix_maximum(id); });
+    std::memcpy(targets, s.target.data(), s.g.N);
+    return 0;
+}
+extern "C" int orc_targets_f32(const float* f, const float* g, i64 W, i64 H, i64 D, double eps, double range,
+                    const orc_flags* fl, uint8_t* targets, i64* counts4) {
+    return targets_impl<float>(f, g, W, H, D, eps, range, fl, targets, counts4);
+}
+extern "C" int orc_targets_f64(const double* f, const double* g, i64 W, i64 H, i64 D, double eps, double range,
+                    const orc_flags* fl, uint8_t* targets, i64* counts4) {
+    return targets_impl<double>(f, g, W, H, D, eps, range, fl, targets, counts4);
+}
+
+// Path mismatch lists on (f, g) as they stand (no fix): out lists sized by caller (E for ds,
+// T for as, F for walls); counts3 = {ds, as, ds_wall}; index arrays written as the reference does.
+template <class T>
+static int paths_impl(const T* f, const T* gfield, i64 W, i64 H, i64 D, const orc_flags* fl,
+                      i64* counts3, i64* ds_list, i64* ds_rec, i64* as_list, i64* as_rec, i64* dsw_list,
+                      i64* dsw_rec) {
+    init_dirs();
+    State<T> s(W, H, D);
+    s.f = f;
+    std::vector<T> gcopy(gfield, gfield + s.g.N);
+    s.d = gcopy.data();
+    s.eps = 0;
+    s.range = 0;
+    std::memcpy(&s.fl, fl, sizeof(orc_flags));
+    s.literal = false;
+    s.alloc();
+    s.gradient(F_FIELD);
+    s.gradient(G_FIELD);
+    if (fl->preserve_vpath == 1 && fl->preserve_geometry == 0) s.compute_labels();
+    s.detect_paths();
+    counts3[0] = (i64)s.l_ds.size();
+    counts3[1] = (i64)s.l_as.size();
+    counts3[2] = (i64)s.l_dsw.size();
+    for (size_t k = 0; k < s.l_ds.size(); k++) {
+        if (ds_list) ds_list[k] = s.l_ds[k];
+        if (ds_rec) ds_rec[k] = s.ds_index[2 * s.l_ds[k]];
+    }
+    for (size_t k = 0; k < s.l_as.size(); k++) {
+        if (as_list) as_list[k] = s.l_as[k];
+        if (as_rec) as_rec[k] = s.as_index[2 * s.l_as[k]];
+    }
+    for (size_t k = 0; k < s.l_dsw.size(); k++) {
+        if (dsw_list) dsw_list[k] = s.l_dsw[k];
+        if (dsw_rec) dsw_rec[k] = s.dsw_index[s.l_dsw[k]];
+    }
+    return (int)(s.walk_overflow > 0);
+}
+extern "C" int orc_paths_f32(const float* f, const float* g, i64 W, i64 H, i64 D, const orc_flags* fl, i64* counts3,
+                  i64* ds_list, i64* ds_rec, i64* as_list, i64* as_rec, i64* dsw_list, i64* dsw_rec) {
+    return paths_impl<float>(f, g, W, H, D, fl, counts3, ds_list, ds_rec, as_list, as_rec, dsw_list, dsw_rec);
+}
+extern "C" int orc_paths_f64(const double* f, const double* g, i64 W, i64 H, i64 D, const orc_flags* fl, i64* counts3,
+                  i64* ds_list, i64* ds_rec, i64* as_list, i64* as_rec, i64* dsw_list, i64* dsw_rec) {
+    return paths_impl<double>(f, g, W, H, D, fl, counts3, ds_list, ds_rec, as_list, as_rec, dsw_list, dsw_rec);
+}
+
+// Full correction loop, main_3d.cu:6521-7149.
+// counts8 = {min, saddle1, saddle2, max, ds_vpath, as_vpath, ds_wall, as_wall} at exit.
+// traj (nullable) receives 8 counts per sweep, up to traj_cap sweeps.
+// Returns 0 converged, 1 not converged within max_sweeps.
+template <class T>
+static int correct_impl(const T* f, T* gfield, i64 W, i64 H, i64 D, double eps, double range, const orc_flags* fl,
+                        i64* counts8, int32_t* counter_out, i64* sweeps_out, i64* traj, i64 traj_cap,
+                        double* phase_s) {
+    init_dirs();
+    State<T> s(W, H, D);
+    s.f = f;
+    s.d = gfield;
+    s.eps = eps;
+    s.range = range;
+    std::memcpy(&s.fl, fl, sizeof(orc_flags));
+    s.literal = false;
+    s.alloc();
+    double ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // gradient, get_cp, get_path, fix_cp, fix_path
+    auto now = []() {
+#ifdef _OPENMP
+        return omp_get_wtime();
+#else
+        return 0.0;
+#endif
+    };
+    double t0 = now();
+    s.gradient(F_FIELD);
+    if (fl->preserve_vpath == 1 && fl->preserve_geometry == 0) s.compute_labels();
+    if (fl->filtration == 1) s.filtration();
+    std::fill(s.counter.begin(), s.counter.end(), 1);  // init_counter main_3d.cu:6782
+    s.gradient(G_FIELD);
+    double t1 = now();
+    ph[0] += t1 - t0;
+    s.detect_min();
+    s.detect_s1();
+    s.detect_s2();
+    s.detect_max();
+    ph[1] += now() - t1;
+    i64 c[8] = {(i64)s.l_min.size(), (i64)s.l_s1.size(), (i64)s.l_s2.size(), (i64)s.l_max.size(), 1, 1, 1, 0};
+    i64 sweeps = 0;
+    int status = 0;
+    auto record = [&]() {
+        if (traj && sweeps - 1 < traj_cap)
+            for (int k = 0; k < 8; k++) traj[(sweeps - 1) * 8 + k] = c[k];
+    };
+    while (c[5] > 0 || c[4] > 0 || c[6] > 0 || c[2] > 0 || c[3] > 0 || c[0] > 0 || c[1] > 0) {
+        while (c[3] > 0 || c[2] > 0 || c[0] > 0 || c[1] > 0) {
+            if (sweeps >= fl->max_sweeps) {
+                status = 1;
+                goto done;
+            }
+            sweeps++;
+            double ta = now();
+            s.clear_targets();
+            if (c[0] > 0) s.for_list(s.l_min, [&](i64 id) { s.fix_minimum(id); });
+            if (c[1] > 0 && fl->preserve_saddles == 1) s.for_list(s.l_s1, [&](i64 id) { s.fix_saddle(id); });
+            if (c[2] > 0 && fl->preserve_saddles == 1) s.for_list(s.l_s2, [&](i64 id) { s.fix_2saddle(id); });
+            if (c[3] > 0) s.for_list(s.l_max, [&](i64 id) { s.fix_maximum(id); });
+            s.apply();
+            double tb = now();
+            ph[3] += tb - ta;
+            s.gradient(G_FIELD);
+            double tc = now();
+            ph[0] += tc - tb;
+            s.detect_min();
+            if (fl->preserve_saddles == 1) {
+                s.detect_s1();
+                s.detect_s2();
+            } else {
+                s.l_s1.clear();
+                s.l_s2.clear();
+            }
+            s.detect_max();
+            ph[1] += now() - tc;
+            c[0] = (i64)s.l_min.size();
+            c[1] = (i64)s.l_s1.size();
+            c[2] = (i64)s.l_s2.size();
+            c[3] = (i64)s.l_max.size();
+            record();
+        }
+        if (fl->preserve_vpath == 0) break;
+        if (sweeps >= fl->max_sweeps) {
+            status = 1;
+            goto done;
+        }
+        sweeps++;
+        {
+            double ta = now();
+            s.detect_paths();
+            double tb = now();
+            ph[2] += tb - ta;
+            i64 nas = (i64)s.l_as.size(), nds = (i64)s.l_ds.size(), ndsw = (i64)s.l_dsw.size();
+            s.clear_targets();
+            if (nas > 0) s.for_list(s.l_as, [&](i64 id) { s.fix_as_vpath(id); });
+            if (nds > 0) s.for_list(s.l_ds, [&](i64 id) { s.fix_ds_vpath(id); });
+            if (ndsw > 0 && fl->preserve_connectors == 1) s.for_list(s.l_dsw, [&](i64 id) { s.fix_ds_wall(id); });
+            s.apply();
+            double tc = now();
+            ph[4] += tc - tb;
+            s.gradient(G_FIELD);
+            double td = now();
+            ph[0] += td - tc;
+            s.detect_min();
+            s.detect_s1();
+            s.detect_s2();
+            s.detect_max();
+            c[0] = (i64)s.l_min.size();
+            c[1] = (i64)s.l_s1.size();
+            c[2] = (i64)s.l_s2.size();
+            c[3] = (i64)s.l_max.size();
+            double te = now();
+            ph[1] += te - td;
+            c[4] = c[5] = c[6] = c[7] = 0;
+            if (c[0] == 0 && c[1] == 0 && c[2] == 0 && c[3] == 0) {
+                s.detect_paths();
+                c[4] = (i64)s.l_ds.size();
+                c[5] = (i64)s.l_as.size();
+                c[6] = (i64)s.l_dsw.size();
+                ph[2] += now() - te;
+            }
+            record();
+        }
+    }
+done:
+    if (fl->preserve_vpath == 0) c[4] = c[5] = c[6] = c[7] = 0;
+    for (int k = 0; k < 8; k++) counts8[k] = c[k];
+    if (counter_out)
+        for (i64 v = 0; v < s.g.N; v++) counter_out[v] = s.counter[v];
+    if (sweeps_out) *sweeps_out = sweeps;
+    if (phase_s)
+        for (int k = 0; k < 5; k++) phase_s[k] = ph[k];
+    if (s.walk_overflow > 0) return -2;
+    return status;
+}
+extern "C" int orc_correct_f32(const float* f, float* g, i64 W, i64 H, i64 D, double eps, double range, const orc_flags* fl,
+                    i64* counts8, int32_t* counter_out, i64* sweeps_out, i64* traj, i64 traj_cap, double* phase_s) {
+    return correct_impl<float>(f, g, W, H, D, eps, range, fl, counts8, counter_out, sweeps_out, traj, traj_cap, phase_s);
+}
+extern "C" int orc_correct_f64(const double* f, double* g, i64 W, i64 H, i64 D, double eps, double range, const orc_flags* fl,
+                    i64* counts8, int32_t* counter_out, i64* sweeps_out, i64* traj, i64 traj_cap, double* phase_s) {
+    return correct_impl<double>(f, g, W, H, D, eps, range, fl, counts8, counter_out, sweeps_out, traj, traj_cap, phase_s);
+}
+
