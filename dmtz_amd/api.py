@@ -1,0 +1,236 @@
+"""Host-side interface of the correction path over the C ABI of include/dmtz.h.
+
+The reference exposes this path only as a CLI (main_3d.cu:6255-6271):
+``main_3d "name,W,H,D" rel_eb compressor preserve_saddles preserve_vpath preserve_geometry
+filtration preserve_connectors preserve_type edit_type``. ``correct`` keeps those flags with the
+same names and meanings, takes f and g as arrays instead of files, and returns the corrected
+field with the mismatch counters of main_3d.cu:68-75 instead of printing them.
+"""
+import ctypes
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import _native as N
+
+
+class DmtzError(RuntimeError):
+    pass
+
+
+@dataclass
+class CorrectionFlags:
+    preserve_saddles: int = 1
+    preserve_vpath: int = 0
+    preserve_geometry: int = 0
+    filtration: int = 0
+    preserve_connectors: int = 0
+    preserve_type: int = 0
+    edit_type: int = 0
+    max_sweeps: int = 100000
+
+    def c(self):
+        return N.Flags(self.preserve_saddles, self.preserve_vpath, self.preserve_geometry,
+                       self.filtration, self.preserve_connectors, self.preserve_type,
+                       self.edit_type, self.max_sweeps, 0, 0)
+
+
+@dataclass
+class CorrectionResult:
+    g: object
+    counts: tuple
+    sweeps: int
+    status: int
+    counter: Optional[np.ndarray] = None
+    trajectory: Optional[np.ndarray] = None
+    phases: dict = field(default_factory=dict)
+
+    @property
+    def converged(self):
+        return self.status == 0 and all(c == 0 for c in self.counts)
+
+
+class Context:
+    """A dmtz_ctx: one HIP stream and the scratch memory of the correction loop."""
+
+    def __init__(self, device=0):
+        self._lib = N.lib()
+        h = ctypes.c_void_p()
+        rc = self._lib.dmtz_ctx_create(ctypes.byref(h), int(device))
+        if rc != 0:
+            raise DmtzError("dmtz_ctx_create failed (%d)" % rc)
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.dmtz_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def error(self):
+        return self._lib.dmtz_last_error(self._h).decode()
+
+    def check(self, rc, what):
+        if rc < 0:
+            raise DmtzError("%s failed (%d): %s" % (what, rc, self.error()))
+        return rc
+
+    def trajectory(self):
+        n = self._lib.dmtz_trajectory(self._h, None, 0)
+        out = np.zeros(max(n, 1) * 8, np.int64)
+        self._lib.dmtz_trajectory(self._h, out.ctypes.data_as(ctypes.c_void_p), n)
+        return out[: n * 8].reshape(n, 8)
+
+    def phases(self):
+        out = np.zeros(6, np.float64)
+        self._lib.dmtz_phase_times(self._h, out.ctypes.data_as(ctypes.c_void_p))
+        names = ("gradient_pairing", "get_f_cp", "get_f_path", "fix_f_cp", "fix_f_vpath", "total")
+        return dict(zip(names, out.tolist()))
+
+
+_DEFAULT_CTX = None
+
+
+def default_context():
+    global _DEFAULT_CTX
+    if _DEFAULT_CTX is None:
+        _DEFAULT_CTX = Context(0)
+    return _DEFAULT_CTX
+
+
+def _tag(dtype):
+    if dtype == np.float32:
+        return "f32"
+    if dtype == np.float64:
+        return "f64"
+    raise TypeError("fields must be float32 or float64")
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def correct(f, g, eps, rng, flags=None, ctx=None, return_counter=False):
+    """Correct g toward f within eps until its discrete gradient matches f's.
+
+    f, g: numpy arrays indexed [z, y, x] (x fastest), same float dtype. Returns a
+    CorrectionResult with the corrected copy of g."""
+    ctx = ctx or default_context()
+    flags = flags or CorrectionFlags()
+    f = np.ascontiguousarray(f)
+    gout = np.array(g, dtype=f.dtype, copy=True, order="C")
+    nz, ny, nx = f.shape
+    counts = N.Counts()
+    sweeps = ctypes.c_int32(0)
+    counter = np.zeros(f.shape, np.uint8) if return_counter else None
+    fn = getattr(ctx._lib, "dmtz_correct_" + _tag(f.dtype))
+    fl = flags.c()
+    rc = fn(ctx.handle, _p(f), _p(gout), nx, ny, nz, float(eps), float(rng), ctypes.byref(fl),
+            ctypes.byref(counts), _p(counter) if counter is not None else None, ctypes.byref(sweeps))
+    ctx.check(rc, "dmtz_correct")
+    return CorrectionResult(gout, counts.as_tuple(), int(sweeps.value), rc, counter,
+                            ctx.trajectory(), ctx.phases())
+
+
+def correct_device(f, g, eps, rng, flags=None, ctx=None, counter_out=None):
+    """Correct device-resident torch tensors in place (g is modified). f, g: cuda tensors
+    [z, y, x] of float32 or float64 on the context's device. counter_out: optional uint8 cuda
+    tensor of the same shape."""
+    import torch
+
+    ctx = ctx or default_context()
+    flags = flags or CorrectionFlags()
+    assert f.is_cuda and g.is_cuda and f.is_contiguous() and g.is_contiguous()
+    assert f.dtype == g.dtype and f.shape == g.shape
+    tag = {torch.float32: "f32", torch.float64: "f64"}[f.dtype]
+    nz, ny, nx = f.shape
+    counts = N.Counts()
+    sweeps = ctypes.c_int32(0)
+    fn = getattr(ctx._lib, "dmtz_correct_dev_" + tag)
+    fl = flags.c()
+    torch.cuda.synchronize(f.device)
+    rc = fn(ctx.handle, ctypes.c_void_p(f.data_ptr()), ctypes.c_void_p(g.data_ptr()), nx, ny, nz,
+            float(eps), float(rng), ctypes.byref(fl), ctypes.byref(counts),
+            ctypes.c_void_p(counter_out.data_ptr()) if counter_out is not None else None,
+            ctypes.byref(sweeps))
+    ctx.check(rc, "dmtz_correct_dev")
+    return CorrectionResult(g, counts.as_tuple(), int(sweeps.value), rc, None, ctx.trajectory(),
+                            ctx.phases())
+
+
+def gradient(field_, ctx=None):
+    """The reference's four pairing arrays (pv, pe, pf, pt) of one field, from the device."""
+    ctx = ctx or default_context()
+    field_ = np.ascontiguousarray(field_)
+    nz, ny, nx = field_.shape
+    Nv = nx * ny * nz
+    E = ((nx - 1) * ny * nz + nx * (ny - 1) * nz + nx * ny * (nz - 1) + (nx - 1) * (ny - 1) * nz
+         + (nx - 1) * ny * (nz - 1) + nx * (ny - 1) * (nz - 1) + (nx - 1) * (ny - 1) * (nz - 1))
+    F = 2 * ((nx - 1) * (ny - 1) * nz + nx * (ny - 1) * (nz - 1) + (nx - 1) * ny * (nz - 1)) \
+        + 6 * (nx - 1) * (ny - 1) * (nz - 1)
+    T = 6 * (nx - 1) * (ny - 1) * (nz - 1)
+    outs = [np.empty(n, np.int64) for n in (Nv, E, F, T)]
+    fn = getattr(ctx._lib, "dmtz_gradient_" + _tag(field_.dtype))
+    ctx.check(fn(ctx.handle, _p(field_), nx, ny, nz, *[_p(o) for o in outs]), "dmtz_gradient")
+    return tuple(outs)
+
+
+def targets(f, g, eps, rng, flags=None, ctx=None):
+    """One detection + repair phase: (edit-set mask [z,y,x] bool, counts4)."""
+    ctx = ctx or default_context()
+    flags = flags or CorrectionFlags()
+    f = np.ascontiguousarray(f)
+    g = np.ascontiguousarray(g, dtype=f.dtype)
+    nz, ny, nx = f.shape
+    tg = np.zeros(f.shape, np.uint8)
+    c4 = np.zeros(4, np.int64)
+    fn = getattr(ctx._lib, "dmtz_targets_" + _tag(f.dtype))
+    fl = flags.c()
+    ctx.check(fn(ctx.handle, _p(f), _p(g), nx, ny, nz, float(eps), float(rng), ctypes.byref(fl),
+                 _p(tg), _p(c4)), "dmtz_targets")
+    return tg.astype(bool), tuple(int(x) for x in c4)
+
+
+def eps_from_rel(f, rel):
+    """(eps, range) as main_3d.cu:6301-6309, computed by the library."""
+    f = np.ascontiguousarray(f)
+    e = ctypes.c_double()
+    r = ctypes.c_double()
+    fn = getattr(N.lib(), "dmtz_eps_from_rel_" + _tag(f.dtype))
+    if fn(_p(f), f.size, float(rel), ctypes.byref(e), ctypes.byref(r)) != 0:
+        raise DmtzError("dmtz_eps_from_rel failed")
+    return e.value, r.value
+
+
+def paths(f, g, flags=None, ctx=None, with_targets=False):
+    """Separatrix mismatch lists on (f, g): dict(ds=(edge ids), ds_rec, as_=(tet ids), as_rec,
+    targets=edit-set mask or None)."""
+    ctx = ctx or default_context()
+    flags = flags or CorrectionFlags(preserve_vpath=1, preserve_geometry=1)
+    f = np.ascontiguousarray(f)
+    g = np.ascontiguousarray(g, dtype=f.dtype)
+    nz, ny, nx = f.shape
+    cap = 4 * f.size + 16
+    bufs = [np.zeros(cap, np.int64) for _ in range(4)]
+    c2 = np.zeros(2, np.int64)
+    tg = np.zeros(f.shape, np.uint8) if with_targets else None
+    fn = getattr(ctx._lib, "dmtz_paths_" + _tag(f.dtype))
+    fl = flags.c()
+    ctx.check(fn(ctx.handle, _p(f), _p(g), nx, ny, nz, ctypes.byref(fl), _p(c2), *[_p(b) for b in bufs],
+                 cap, _p(tg) if tg is not None else None), "dmtz_paths")
+    nds, nas = int(c2[0]), int(c2[1])
+    return dict(ds=bufs[0][:nds], ds_rec=bufs[1][:nds], as_=bufs[2][:nas], as_rec=bufs[3][:nas],
+                targets=tg.astype(bool) if tg is not None else None)

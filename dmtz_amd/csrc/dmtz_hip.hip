@@ -1,0 +1,1155 @@
+// MI355X (gfx950) implementation of the DMTz topology-correction path behind include/dmtz.h.
+//
+// Device work:
+//   k_detect      per-vertex lower stars of f and g -> mismatch records, 4 typed lists
+//                 (get_false_* main_3d.cu:2492-2550 over the gradient of main_3d.cu:1930-2316)
+//   k_fix_cp      repair rules over a record list -> edit set (main_3d.cu:2554-4303)
+//   k_apply       epsilon-checked edit of the edit set (applyDeltaBuffer1 main_3d.cu:1122-1164)
+//   k_fcrit       f-critical cells (for separatrix checks and filtration)
+//   k_ds_geo, k_as_branch + k_as_combine, k_ds_branch + k_ds_combine, k_labels
+//                 separatrix checks (main_3d.cu:4437-5062)
+//   k_fix_paths   separatrix repairs (main_3d.cu:5135-5295)
+//   k_pin         filtration (main_3d.cu:2405-2490)
+// Host: the sweep loop of main_3d.cu:6784-7149 on one HIP stream; per-sweep counts are read back
+// like the reference's cudaMemcpyFromSymbol calls (main_3d.cu:6918-6921, 6989-6992, 7096-7143).
+//
+// Records: a mismatched (or f-critical) cell is stored as (vertex << 7) | local cell, where the
+// vertex is the cell's owner in the emitting field and the local cell numbers the 75 cells of a
+// lower star (lower_star.h). Lists are filled with wave-aggregated atomics: the set is
+// deterministic, the order is not, and every consumer is order independent.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dmtz.h"
+#include "mismatch.h"
+#include "repair.h"
+
+using namespace dmtz;
+typedef unsigned long long ull;
+
+namespace {
+
+enum : int {
+    K_MIN = 0, K_S1 = 1, K_S2 = 2, K_MAX = 3,  // record lists per cell type
+    K_TGT = 4,                                  // edit set size
+    K_DS = 5, K_AS = 6,                         // separatrix mismatch lists
+    K_FC0 = 8,                                  // f-critical lists per type: 8..11
+    K_WALK = 12,                                // capped walks
+    K_NCTR = 16
+};
+
+constexpr int BLOCK = 256;
+
+// ------------------------------------------------------------------------------------------------
+// Wave64 helpers
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ int wave_excl_scan(int x, int& total) {
+    const int lane = __lane_id();
+    int inc = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
+    }
+    total = __shfl(inc, 63, 64);
+    return inc - x;
+}
+
+// Every lane of the wave must call this. Returns the lane's first slot in the list.
+__device__ __forceinline__ ull wave_alloc(ull* ctr, int n) {
+    int total;
+    int pre = wave_excl_scan(n, total);
+    ull base = 0;
+    if (__lane_id() == 0 && total > 0) base = atomicAdd(ctr, (ull)total);
+    unsigned lo = __shfl((unsigned)(base & 0xffffffffu), 0, 64);
+    unsigned hi = __shfl((unsigned)(base >> 32), 0, 64);
+    return (((ull)hi << 32) | lo) + (ull)pre;
+}
+
+struct Lists4 {
+    uint64_t* l[4];
+    ull cap;
+};
+
+// Emit the set bits of (lo, hi) as records of vertex v into the four typed lists.
+__device__ __forceinline__ void emit_records(i64 v, uint64_t lo, uint32_t hi, const Lists4& out, ull* ctr) {
+    const uint64_t m1 = (lo >> 1) & 0x3FFFull;
+    const uint64_t m2 = (lo >> 15) & ((1ull << 36) - 1);
+    const int n0 = (int)(lo & 1ull), n1 = __popcll(m1), n2 = __popcll(m2), n3 = __popc(hi);
+    ull b0 = wave_alloc(ctr + 0, n0);
+    ull b1 = wave_alloc(ctr + 1, n1);
+    ull b2 = wave_alloc(ctr + 2, n2);
+    ull b3 = wave_alloc(ctr + 3, n3);
+    const uint64_t key = (uint64_t)v << 7;
+    if (n0 && b0 < out.cap) out.l[0][b0] = key | LC_VERT;
+    uint64_t m = m1;
+    while (m) {
+        int b = __builtin_ctzll(m);
+        m &= m - 1;
+        if (b1 < out.cap) out.l[1][b1] = key | (uint64_t)(LC_EDGE + b);
+        b1++;
+    }
+    m = m2;
+    while (m) {
+        int b = __builtin_ctzll(m);
+        m &= m - 1;
+        if (b2 < out.cap) out.l[2][b2] = key | (uint64_t)(LC_TRI + b);
+        b2++;
+    }
+    uint32_t h = hi;
+    while (h) {
+        int b = __builtin_ctz(h);
+        h &= h - 1;
+        if (b3 < out.cap) out.l[3][b3] = key | (uint64_t)(LC_TET + b);
+        b3++;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Kernels
+// ------------------------------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_detect(const T* f, const T* g, Dims d, Counts c, Lists4 out, ull* ctr) {
+    const i64 v = (i64)blockIdx.x * BLOCK + threadIdx.x;
+    uint64_t lo = 0;
+    uint32_t hi = 0;
+    if (v < c.N) {
+        FieldGrad<T> F(f, d, c), G(g, d, c);
+        LowerStar lf, lg;
+        F.star_inline(v, lf);
+        G.star_inline(v, lg);
+        vertex_mismatch(F, G, v, lf, lg, lo, hi);
+    }
+    emit_records(v, lo, hi, out, ctr);
+}
+
+// f-critical cells owned by each vertex in f; type_mask selects which types are listed.
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_fcrit(const T* f, Dims d, Counts c, Lists4 out, ull* ctr, int type_mask) {
+    const i64 v = (i64)blockIdx.x * BLOCK + threadIdx.x;
+    uint64_t lo = 0;
+    uint32_t hi = 0;
+    if (v < c.N) {
+        FieldGrad<T> F(f, d, c);
+        LowerStar lf;
+        F.star_inline(v, lf);
+        crit_mask(lf, lo, hi);
+        uint64_t keep = 0;
+        if (type_mask & 1) keep |= 1ull;
+        if (type_mask & 2) keep |= 0x3FFFull << 1;
+        if (type_mask & 4) keep |= ((1ull << 36) - 1) << 15;
+        lo &= keep;
+        if (!(type_mask & 8)) hi = 0;
+    }
+    emit_records(v, lo, hi, out, ctr);
+}
+
+struct TargetSink {
+    uint32_t* stamp;
+    uint32_t epoch;
+    uint32_t* list;
+    ull* ctr;
+    ull cap;
+    __device__ void operator()(i64 v) const {
+        uint32_t old = atomicExch(stamp + v, epoch);
+        if (old != epoch) {
+            ull i = atomicAdd(ctr, 1ull);
+            if (i < cap) list[i] = (uint32_t)v;
+        }
+    }
+};
+
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_fix_cp(int type, const uint64_t* recs, ull n, const T* f, const T* g, Dims d,
+                                                  Counts c, const uint8_t* counter, double eps, double range,
+                                                  int edit_type, TargetSink sink) {
+    const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t r = recs[i];
+    const i64 v = (i64)(r >> 7);
+    const int lc = (int)(r & 127u);
+    const i64 id = local_cell_id(d, c, v, lc);
+    Repair<T, TargetSink> R(f, g, d, c, counter, eps, range, edit_type, sink);
+    switch (type) {
+        case 0: R.fix_minimum(id); break;
+        case 1: R.fix_saddle(id); break;
+        case 2: R.fix_2saddle(id); break;
+        default: R.fix_maximum(id); break;
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_apply(const uint32_t* list, ull n, const T* f, T* g, uint8_t* counter,
+                                                 double eps, double range, int preserve_type, int edit_type) {
+    const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const i64 v = list[i];
+    const T fv = f[v];
+    T gv = g[v];
+    int k = counter[v];
+    const double dl = edit_type == 1 ? halving_delta(eps, k) : EditArith<T>::delta0(fv, gv, eps);
+    if (preserve_type == 0) {
+        if (fabs_d(dl) > 1e-15 * range && k < 5 && EditArith<T>::step(fv, gv, dl, eps)) {
+            k += 1;
+        } else {
+            k = 6;
+            gv = EditArith<T>::snap(fv, eps);
+        }
+    } else {
+        k = 6;
+        gv = EditArith<T>::snap(fv, eps);
+    }
+    g[v] = gv;
+    counter[v] = (uint8_t)k;
+}
+
+// Filtration main_3d.cu:2405-2490: every vertex of an f-critical cell is pinned to f - eps.
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_pin(const uint64_t* recs, ull n, const T* f, T* g, Dims d, Counts c,
+                                               double eps) {
+    const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t r = recs[i];
+    const i64 v = (i64)(r >> 7);
+    const int lc = (int)(r & 127u);
+    i64 x, y, z;
+    vxyz(d, v, x, y, z);
+    i64 w[4] = {v, -1, -1, -1};
+    auto nb = [&](int k) { return vidx(d, x + DIR[k][0], y + DIR[k][1], z + DIR[k][2]); };
+    if (lc >= LC_EDGE && lc < LC_TRI) {
+        w[1] = nb(lc - LC_EDGE);
+    } else if (lc >= LC_TRI && lc < LC_TET) {
+        w[1] = nb(ST.tri_i[lc - LC_TRI]);
+        w[2] = nb(ST.tri_j[lc - LC_TRI]);
+    } else if (lc >= LC_TET) {
+        w[1] = nb(ST.tet_s[lc - LC_TET][0]);
+        w[2] = nb(ST.tet_s[lc - LC_TET][1]);
+        w[3] = nb(ST.tet_s[lc - LC_TET][2]);
+    }
+    for (int k = 0; k < 4; k++)
+        if (w[k] >= 0) g[w[k]] = EditArith<T>::snap(f[w[k]], eps);
+}
+
+struct PathList {
+    int64_t* id;   // cell id (edge for descending, tet for ascending)
+    int64_t* rec;  // recorded vertex / divergent tet
+    ull* ctr;
+    ull cap;
+    // Every lane of the wave must call this; lanes with has = false add nothing.
+    __device__ void push_wave(bool has, i64 a, i64 b) const {
+        ull at = wave_alloc(ctr, has ? 1 : 0);
+        if (has && at < cap) {
+            id[at] = a;
+            rec[at] = b;
+        }
+    }
+};
+
+struct NoSink {
+    __device__ void operator()(i64) const {}
+};
+
+// The separatrix checks run in two steps: one thread per (start cell, branch) walks and writes
+// its result at a fixed position; a combine step per start cell applies the reference's
+// "first branch that fails" rule and appends to the list in convergent code.
+
+// get_false_ds_vpath main_3d.cu:4837-4959 over the f-critical edges.
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_ds_geo(const uint64_t* fce, ull n, const T* f, const T* g, Dims d, Counts c,
+                                                  PathList out) {
+    const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    bool has = false;
+    i64 e = -1, rec = -1;
+    if (i < n) {
+        const uint64_t r = fce[i];
+        e = local_cell_id(d, c, (i64)(r >> 7), (int)(r & 127u));
+        Repair<T, NoSink> R(f, g, d, c, nullptr, 0, 0, 0, NoSink{});
+        has = R.ds_geometry(e, rec);
+    }
+    out.push_wave(has, e, rec);
+}
+
+// Ascending branch walks (is_false main_3d.cu:4437-4497), thread = (f-critical face, cofacet k).
+// Connectivity mode (maxlab != null) first compares g's reached maximum with f's label
+// (main_3d.cu:4821-4826). res[2*j] = start tet if the branch fails else -1, res[2*j+1] = the
+// divergent tet.
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_as_branch(const uint64_t* fcf, ull n, const int64_t* maxlab, const T* f,
+                                                     const T* g, Dims d, Counts c, int64_t* res, ull* walk_ctr) {
+    const ull j = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= 2 * n) return;
+    const ull i = j >> 1;
+    const int k = (int)(j & 1);
+    const uint64_t r = fcf[i];
+    const i64 t = local_cell_id(d, c, (i64)(r >> 7), (int)(r & 127u));
+    Repair<T, NoSink> R(f, g, d, c, nullptr, 0, 0, 0, NoSink{});
+    i64 q0, q1;
+    R.cofacets(t, q0, q1);
+    const i64 q = k ? q1 : q0;
+    bool ovf = false, found = false;
+    i64 dv = -1;
+    bool check = q >= 0;
+    if (check && maxlab) check = R.paired_max(q, 1, c.N, ovf) != maxlab[2 * i + k];
+    if (check) R.as_walk(q, c.N, found, dv, ovf);
+    res[2 * j] = found ? q : -1;
+    res[2 * j + 1] = dv;
+    if (ovf) atomicAdd(walk_ctr, 1ull);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_as_combine(const int64_t* res, ull n, PathList out) {
+    const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    bool has = false;
+    i64 a = -1, b = -1;
+    if (i < n) {
+        const int64_t* p = res + 4 * i;
+        has = p[0] >= 0 || p[2] >= 0;
+        a = p[0] >= 0 ? p[0] : p[2];
+        b = p[0] >= 0 ? p[1] : p[3];
+    }
+    out.push_wave(has, a, b);
+}
+
+// Endpoint labels of f (get_orginal_saddle_min_pairs / _max_pairs, main_3d.cu:4613-4759), one
+// thread per (f-critical edge, endpoint) and per (f-critical face, cofacet); ascending labels
+// are indexed by face (resolution of the shadowed index at main_3d.cu:4750).
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_labels(const uint64_t* fce, ull ne, const uint64_t* fcf, ull nf, const T* f,
+                                                  Dims d, Counts c, int64_t* minlab, int64_t* maxlab, ull* walk_ctr) {
+    const ull j = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    Repair<T, NoSink> R(f, f, d, c, nullptr, 0, 0, 0, NoSink{});
+    bool ovf = false;
+    if (j < 2 * ne) {
+        const uint64_t r = fce[j >> 1];
+        const i64 e = local_cell_id(d, c, (i64)(r >> 7), (int)(r & 127u));
+        i64 a, b;
+        edge_verts(d, c, e, a, b);
+        minlab[j] = R.f_min_label((j & 1) ? b : a, c.N, ovf);
+    } else if (j < 2 * ne + 2 * nf) {
+        const ull jj = j - 2 * ne;
+        const uint64_t r = fcf[jj >> 1];
+        const i64 t = local_cell_id(d, c, (i64)(r >> 7), (int)(r & 127u));
+        i64 q0, q1;
+        R.cofacets(t, q0, q1);
+        const i64 q = (jj & 1) ? q1 : q0;
+        maxlab[jj] = q >= 0 ? R.paired_max(q, 0, c.N, ovf) : -1;
+    }
+    if (ovf) atomicAdd(walk_ctr, 1ull);
+}
+
+// get_false_ds_vpath_connectivity main_3d.cu:4961-5062: thread = (f-critical edge, endpoint)
+// walks g's descending path; res[4*i + 2*k] = reached g minimum, res[4*i + 2*k + 1] = first
+// vertex on the walk whose f and g pairings differ.
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_ds_branch(const uint64_t* fce, ull n, const T* f, const T* g, Dims d,
+                                                     Counts c, int64_t* res, ull* walk_ctr) {
+    const ull j = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= 2 * n) return;
+    const uint64_t r = fce[j >> 1];
+    const i64 e = local_cell_id(d, c, (i64)(r >> 7), (int)(r & 127u));
+    Repair<T, NoSink> R(f, g, d, c, nullptr, 0, 0, 0, NoSink{});
+    i64 a, b;
+    edge_verts(d, c, e, a, b);
+    i64 trouble = -1;
+    bool ovf = false;
+    res[2 * j] = R.g_min_walk((j & 1) ? b : a, trouble, c.N, ovf);
+    res[2 * j + 1] = trouble;
+    if (ovf) atomicAdd(walk_ctr, 1ull);
+}
+
+// The reference carries trouble_maker from the first endpoint's walk into the second.
+__global__ __launch_bounds__(BLOCK) void k_ds_combine(const uint64_t* fce, ull n, const int64_t* res,
+                                                      const int64_t* minlab, Dims d, Counts c, PathList out) {
+    const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    bool has = false;
+    i64 e = -1, rec = -1;
+    if (i < n) {
+        const uint64_t r = fce[i];
+        e = local_cell_id(d, c, (i64)(r >> 7), (int)(r & 127u));
+        const int64_t* p = res + 4 * i;
+        bool bad1 = p[0] != minlab[2 * i];
+        bool bad2 = p[2] != minlab[2 * i + 1];
+        has = bad1 || bad2;
+        rec = bad1 ? p[1] : (p[1] != -1 ? p[1] : p[3]);
+    }
+    out.push_wave(has, e, rec);
+}
+
+// fix_as_vpath / fix_ds_vpath main_3d.cu:5135-5295
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_fix_paths(const int64_t* as_rec, ull n_as, const int64_t* ds_rec, ull n_ds,
+                                                     const T* f, const T* g, Dims d, Counts c, const uint8_t* counter,
+                                                     double eps, double range, int edit_type, TargetSink sink) {
+    const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    Repair<T, TargetSink> R(f, g, d, c, counter, eps, range, edit_type, sink);
+    if (i < n_as) R.fix_as_vpath(as_rec[i]);
+    else if (i < n_as + n_ds) R.fix_ds_vpath(ds_rec[i - n_as]);
+}
+
+// Dense pairing arrays for tests (dmtz_gradient_*): one thread per cell of each dimension.
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_pairing(const T* fld, Dims d, Counts c, int dim, ull n, int64_t* out) {
+    const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    FieldGrad<T> X(fld, d, c);
+    switch (dim) {
+        case 0: out[i] = X.pv((i64)i); break;
+        case 1: out[i] = X.pe((i64)i); break;
+        case 2: out[i] = X.pf((i64)i); break;
+        default: out[i] = X.pt((i64)i); break;
+    }
+}
+
+inline unsigned nblocks(ull n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
+
+}  // namespace
+
+// ================================================================================================
+// Context
+// ================================================================================================
+struct dmtz_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // device buffers
+    void* d_f = nullptr;
+    void* d_g = nullptr;
+    size_t field_bytes = 0;
+    uint8_t* d_counter = nullptr;
+    uint32_t* d_stamp = nullptr;
+    uint32_t* d_targets = nullptr;
+    uint64_t* d_lists = nullptr;  // 4 record lists + 4 f-critical lists
+    int64_t* d_path = nullptr;    // ds id/rec, as id/rec
+    int64_t* d_labels = nullptr;  // min labels (2 per f-critical edge), max labels (2 per face)
+    int64_t* d_walk = nullptr;    // per-branch walk results (4 per start cell)
+    ull* d_ctr = nullptr;
+    ull* h_ctr = nullptr;
+    i64 cap_n = 0;   // vertices the buffers are sized for
+    ull list_cap = 0;
+    uint32_t epoch = 0;
+    std::vector<int64_t> traj;
+    double phase[6] = {0, 0, 0, 0, 0, 0};
+};
+
+namespace {
+
+void free_bufs(dmtz_ctx* ctx) {
+    (void)hipFree(ctx->d_f);
+    (void)hipFree(ctx->d_g);
+    (void)hipFree(ctx->d_counter);
+    (void)hipFree(ctx->d_stamp);
+    (void)hipFree(ctx->d_targets);
+    (void)hipFree(ctx->d_lists);
+    (void)hipFree(ctx->d_path);
+    (void)hipFree(ctx->d_labels);
+    (void)hipFree(ctx->d_walk);
+    ctx->d_walk = nullptr;
+    ctx->d_f = ctx->d_g = nullptr;
+    ctx->d_counter = nullptr;
+    ctx->d_stamp = ctx->d_targets = nullptr;
+    ctx->d_lists = nullptr;
+    ctx->d_path = ctx->d_labels = nullptr;
+    ctx->field_bytes = 0;
+    ctx->cap_n = 0;
+}
+
+#define CK(x)                                                                         \
+    do {                                                                              \
+        hipError_t e_ = (x);                                                          \
+        if (e_ != hipSuccess) {                                                       \
+            ctx->err = std::string(#x) + " failed: " + hipGetErrorString(e_);         \
+            return DMTZ_E_HIP;                                                        \
+        }                                                                             \
+    } while (0)
+
+int ensure(dmtz_ctx* ctx, i64 N, size_t elem, bool want_fields) {
+    if (N > ctx->cap_n) {
+        free_bufs(ctx);
+        ull cap = (ull)N + 4096;
+        CK(hipMalloc(&ctx->d_counter, (size_t)N));
+        CK(hipMalloc(&ctx->d_stamp, sizeof(uint32_t) * (size_t)N));
+        CK(hipMemsetAsync(ctx->d_stamp, 0, sizeof(uint32_t) * (size_t)N, ctx->stream));
+        CK(hipMalloc(&ctx->d_targets, sizeof(uint32_t) * (size_t)N));
+        CK(hipMalloc(&ctx->d_lists, sizeof(uint64_t) * 8 * cap));
+        CK(hipMalloc(&ctx->d_path, sizeof(int64_t) * 4 * cap));
+        CK(hipMalloc(&ctx->d_labels, sizeof(int64_t) * 4 * cap));
+        CK(hipMalloc(&ctx->d_walk, sizeof(int64_t) * 4 * cap));
+        ctx->cap_n = N;
+        ctx->list_cap = cap;
+        ctx->epoch = 0;
+    }
+    if (want_fields && ctx->field_bytes < (size_t)N * elem) {
+        (void)hipFree(ctx->d_f);
+        (void)hipFree(ctx->d_g);
+        ctx->d_f = ctx->d_g = nullptr;
+        CK(hipMalloc(&ctx->d_f, (size_t)N * elem));
+        CK(hipMalloc(&ctx->d_g, (size_t)N * elem));
+        ctx->field_bytes = (size_t)N * elem;
+    }
+    return 0;
+}
+
+int check_dims(dmtz_ctx* ctx, i64 nx, i64 ny, i64 nz) {
+    if (!ctx) return DMTZ_E_ARG;
+    if (nx < 3 || ny < 2 || nz < 2) {
+        ctx->err = "grid must have nx >= 3, ny >= 2, nz >= 2";
+        return DMTZ_E_ARG;
+    }
+    if (nx * ny * nz >= ((i64)1 << 31)) {
+        ctx->err = "grid has 2^31 or more vertices";
+        return DMTZ_E_ARG;
+    }
+    return 0;
+}
+
+Lists4 rec_lists(dmtz_ctx* ctx) {
+    Lists4 L;
+    for (int k = 0; k < 4; k++) L.l[k] = ctx->d_lists + (size_t)k * ctx->list_cap;
+    L.cap = ctx->list_cap;
+    return L;
+}
+Lists4 fcrit_lists(dmtz_ctx* ctx) {
+    Lists4 L;
+    for (int k = 0; k < 4; k++) L.l[k] = ctx->d_lists + (size_t)(4 + k) * ctx->list_cap;
+    L.cap = ctx->list_cap;
+    return L;
+}
+
+int sync_ctr(dmtz_ctx* ctx) {
+    CK(hipMemcpyAsync(ctx->h_ctr, ctx->d_ctr, sizeof(ull) * K_NCTR, hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int overflow_check(dmtz_ctx* ctx) {
+    const int checked[10] = {K_MIN, K_S1, K_S2, K_MAX, K_DS, K_AS, K_FC0, K_FC0 + 1, K_FC0 + 2, K_FC0 + 3};
+    for (int k : checked)
+        if (ctx->h_ctr[k] > ctx->list_cap) {
+            ctx->err = "mismatch list capacity exceeded";
+            return DMTZ_E_CAPACITY;
+        }
+    if (ctx->h_ctr[K_TGT] > (ull)ctx->cap_n) {
+        ctx->err = "edit set capacity exceeded";
+        return DMTZ_E_CAPACITY;
+    }
+    return 0;
+}
+
+using Clock = std::chrono::steady_clock;
+double secs(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+template <class T>
+struct Driver {
+    dmtz_ctx* ctx;
+    const T* f;
+    T* g;
+    Dims d;
+    Counts c;
+    double eps, range;
+    dmtz_flags fl;
+
+    // Detection of the four critical-cell mismatch kinds into the record lists.
+    int detect() {
+        CK(hipMemsetAsync(ctx->d_ctr + K_MIN, 0, sizeof(ull) * 4, ctx->stream));
+        k_detect<T><<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(f, g, d, c, rec_lists(ctx), ctx->d_ctr + K_MIN);
+        CK(hipGetLastError());
+        int rc = sync_ctr(ctx);
+        if (rc) return rc;
+        return overflow_check(ctx);
+    }
+
+    TargetSink sink() {
+        TargetSink s;
+        s.stamp = ctx->d_stamp;
+        s.epoch = ctx->epoch;
+        s.list = ctx->d_targets;
+        s.ctr = ctx->d_ctr + K_TGT;
+        s.cap = (ull)ctx->cap_n;
+        return s;
+    }
+
+    int begin_edits() {
+        ctx->epoch++;
+        if (ctx->epoch == 0) {  // stamp wrap: clear and restart
+            CK(hipMemsetAsync(ctx->d_stamp, 0, sizeof(uint32_t) * (size_t)c.N, ctx->stream));
+            ctx->epoch = 1;
+        }
+        CK(hipMemsetAsync(ctx->d_ctr + K_TGT, 0, sizeof(ull), ctx->stream));
+        return 0;
+    }
+
+    // Repair phase of one critical-point sweep (main_3d.cu:6801-6833) + apply (:6837).
+    int fix_cp(const ull counts[4]) {
+        int rc = begin_edits();
+        if (rc) return rc;
+        Lists4 L = rec_lists(ctx);
+        for (int ty = 0; ty < 4; ty++) {
+            if (counts[ty] == 0) continue;
+            if ((ty == 1 || ty == 2) && fl.preserve_saddles != 1) continue;
+            k_fix_cp<T><<<nblocks(counts[ty]), BLOCK, 0, ctx->stream>>>(ty, L.l[ty], counts[ty], f, g, d, c,
+                                                                         ctx->d_counter, eps, range, fl.edit_type,
+                                                                         sink());
+            CK(hipGetLastError());
+        }
+        return apply();
+    }
+
+    int apply() {
+        int rc = sync_ctr(ctx);
+        if (rc) return rc;
+        rc = overflow_check(ctx);
+        if (rc) return rc;
+        ull nt = ctx->h_ctr[K_TGT];
+        if (nt) {
+            k_apply<T><<<nblocks(nt), BLOCK, 0, ctx->stream>>>(ctx->d_targets, nt, f, g, ctx->d_counter, eps, range,
+                                                               fl.preserve_type, fl.edit_type);
+            CK(hipGetLastError());
+        }
+        return 0;
+    }
+
+    PathList path_list(int which) {
+        PathList p;
+        ull cap = ctx->list_cap;
+        p.id = ctx->d_path + (size_t)(2 * which) * cap;
+        p.rec = ctx->d_path + (size_t)(2 * which + 1) * cap;
+        p.ctr = ctx->d_ctr + (which == 0 ? K_DS : K_AS);
+        p.cap = cap;
+        return p;
+    }
+
+    ull nfce = 0, nfcf = 0;
+
+    int prepare_paths() {
+        Lists4 FL = fcrit_lists(ctx);
+        CK(hipMemsetAsync(ctx->d_ctr + K_FC0, 0, sizeof(ull) * 4, ctx->stream));
+        k_fcrit<T><<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(f, d, c, FL, ctx->d_ctr + K_FC0, 2 | 4);
+        CK(hipGetLastError());
+        int rc = sync_ctr(ctx);
+        if (rc) return rc;
+        rc = overflow_check(ctx);
+        if (rc) return rc;
+        nfce = ctx->h_ctr[K_FC0 + 1];
+        nfcf = ctx->h_ctr[K_FC0 + 2];
+        if (fl.preserve_vpath == 1 && fl.preserve_geometry == 0 && nfce + nfcf) {
+            k_labels<T><<<nblocks(2 * (nfce + nfcf)), BLOCK, 0, ctx->stream>>>(
+                FL.l[1], nfce, FL.l[2], nfcf, f, d, c, ctx->d_labels, ctx->d_labels + 2 * ctx->list_cap,
+                ctx->d_ctr + K_WALK);
+            CK(hipGetLastError());
+        }
+        return 0;
+    }
+
+    // Separatrix detection (main_3d.cu:6955-6970): fills the ds / as lists.
+    int detect_paths() {
+        CK(hipMemsetAsync(ctx->d_ctr + K_DS, 0, sizeof(ull) * 2, ctx->stream));
+        Lists4 FL = fcrit_lists(ctx);
+        const bool conn = fl.preserve_geometry != 1;
+        if (nfcf) {
+            const int64_t* lab = conn ? ctx->d_labels + 2 * ctx->list_cap : nullptr;
+            k_as_branch<T><<<nblocks(2 * nfcf), BLOCK, 0, ctx->stream>>>(FL.l[2], nfcf, lab, f, g, d, c, ctx->d_walk,
+                                                                         ctx->d_ctr + K_WALK);
+            CK(hipGetLastError());
+            k_as_combine<<<nblocks(nfcf), BLOCK, 0, ctx->stream>>>(ctx->d_walk, nfcf, path_list(1));
+            CK(hipGetLastError());
+        }
+        if (nfce) {
+            if (!conn) {
+                k_ds_geo<T><<<nblocks(nfce), BLOCK, 0, ctx->stream>>>(FL.l[1], nfce, f, g, d, c, path_list(0));
+                CK(hipGetLastError());
+            } else {
+                k_ds_branch<T><<<nblocks(2 * nfce), BLOCK, 0, ctx->stream>>>(FL.l[1], nfce, f, g, d, c, ctx->d_walk,
+                                                                             ctx->d_ctr + K_WALK);
+                CK(hipGetLastError());
+                k_ds_combine<<<nblocks(nfce), BLOCK, 0, ctx->stream>>>(FL.l[1], nfce, ctx->d_walk, ctx->d_labels, d, c,
+                                                                       path_list(0));
+                CK(hipGetLastError());
+            }
+        }
+        int rc = sync_ctr(ctx);
+        if (rc) return rc;
+        return overflow_check(ctx);
+    }
+
+    int fix_paths(ull nds, ull nas) {
+        int rc = begin_edits();
+        if (rc) return rc;
+        if (nds + nas) {
+            k_fix_paths<T><<<nblocks(nds + nas), BLOCK, 0, ctx->stream>>>(
+                path_list(1).rec, nas, path_list(0).rec, nds, f, g, d, c, ctx->d_counter, eps, range, fl.edit_type,
+                sink());
+            CK(hipGetLastError());
+        }
+        return apply();
+    }
+
+    int filtration() {
+        Lists4 FL = fcrit_lists(ctx);
+        CK(hipMemsetAsync(ctx->d_ctr + K_FC0, 0, sizeof(ull) * 4, ctx->stream));
+        k_fcrit<T><<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(f, d, c, FL, ctx->d_ctr + K_FC0, 15);
+        CK(hipGetLastError());
+        int rc = sync_ctr(ctx);
+        if (rc) return rc;
+        rc = overflow_check(ctx);
+        if (rc) return rc;
+        for (int ty = 0; ty < 4; ty++) {
+            ull n = ctx->h_ctr[K_FC0 + ty];
+            if (!n) continue;
+            k_pin<T><<<nblocks(n), BLOCK, 0, ctx->stream>>>(FL.l[ty], n, f, g, d, c, eps);
+            CK(hipGetLastError());
+        }
+        return 0;
+    }
+
+    // The loop of main_3d.cu:6521-7149.
+    int run(dmtz_counts* out, int32_t* sweeps_out) {
+        ctx->traj.clear();
+        for (double& p : ctx->phase) p = 0;
+        auto t_start = Clock::now();
+        int maxs = fl.max_sweeps > 0 ? fl.max_sweeps : 100000;
+        CK(hipMemsetAsync(ctx->d_ctr, 0, sizeof(ull) * K_NCTR, ctx->stream));
+        if (fl.filtration == 1) {
+            int rc = filtration();
+            if (rc) return rc;
+        }
+        CK(hipMemsetAsync(ctx->d_counter, 1, (size_t)c.N, ctx->stream));  // init_counter :6782
+        bool paths = fl.preserve_vpath == 1;
+        if (paths) {
+            int rc = prepare_paths();
+            if (rc) return rc;
+        }
+        auto t0 = Clock::now();
+        int rc = detect();
+        if (rc) return rc;
+        auto t1 = Clock::now();
+        ctx->phase[0] += secs(t0, t1);
+        i64 cnt[8] = {(i64)ctx->h_ctr[0], (i64)ctx->h_ctr[1], (i64)ctx->h_ctr[2], (i64)ctx->h_ctr[3], 1, 1, 1, 0};
+        i64 sweeps = 0;
+        int status = 0;
+        auto record = [&]() {
+            for (int k = 0; k < 8; k++) ctx->traj.push_back(cnt[k]);
+        };
+        while (cnt[5] > 0 || cnt[4] > 0 || cnt[6] > 0 || cnt[2] > 0 || cnt[3] > 0 || cnt[0] > 0 || cnt[1] > 0) {
+            while (cnt[3] > 0 || cnt[2] > 0 || cnt[0] > 0 || cnt[1] > 0) {
+                if (sweeps >= maxs) {
+                    status = DMTZ_NOT_CONVERGED;
+                    goto done;
+                }
+                sweeps++;
+                auto ta = Clock::now();
+                ull cc[4] = {(ull)cnt[0], (ull)cnt[1], (ull)cnt[2], (ull)cnt[3]};
+                rc = fix_cp(cc);
+                if (rc) return rc;
+                auto tb = Clock::now();
+                ctx->phase[3] += secs(ta, tb);
+                rc = detect();
+                if (rc) return rc;
+                auto tc = Clock::now();
+                ctx->phase[1] += secs(tb, tc);
+                cnt[0] = (i64)ctx->h_ctr[0];
+                cnt[1] = fl.preserve_saddles == 1 ? (i64)ctx->h_ctr[1] : 0;
+                cnt[2] = fl.preserve_saddles == 1 ? (i64)ctx->h_ctr[2] : 0;
+                cnt[3] = (i64)ctx->h_ctr[3];
+                record();
+            }
+            if (!paths) break;
+            if (sweeps >= maxs) {
+                status = DMTZ_NOT_CONVERGED;
+                goto done;
+            }
+            sweeps++;
+            {
+                auto ta = Clock::now();
+                rc = detect_paths();
+                if (rc) return rc;
+                auto tb = Clock::now();
+                ctx->phase[2] += secs(ta, tb);
+                ull nds = ctx->h_ctr[K_DS], nas = ctx->h_ctr[K_AS];
+                rc = fix_paths(nds, nas);
+                if (rc) return rc;
+                auto tc = Clock::now();
+                ctx->phase[4] += secs(tb, tc);
+                rc = detect();
+                if (rc) return rc;
+                auto td = Clock::now();
+                ctx->phase[1] += secs(tc, td);
+                cnt[0] = (i64)ctx->h_ctr[0];
+                cnt[1] = (i64)ctx->h_ctr[1];
+                cnt[2] = (i64)ctx->h_ctr[2];
+                cnt[3] = (i64)ctx->h_ctr[3];
+                cnt[4] = cnt[5] = cnt[6] = cnt[7] = 0;
+                if (cnt[0] == 0 && cnt[1] == 0 && cnt[2] == 0 && cnt[3] == 0) {
+                    rc = detect_paths();
+                    if (rc) return rc;
+                    cnt[4] = (i64)ctx->h_ctr[K_DS];
+                    cnt[5] = (i64)ctx->h_ctr[K_AS];
+                    ctx->phase[2] += secs(td, Clock::now());
+                }
+                record();
+            }
+        }
+    done:
+        if (!paths) cnt[4] = cnt[5] = cnt[6] = cnt[7] = 0;
+        CK(hipStreamSynchronize(ctx->stream));
+        ctx->phase[5] = secs(t_start, Clock::now());
+        if (out) {
+            out->min = cnt[0];
+            out->saddle1 = cnt[1];
+            out->saddle2 = cnt[2];
+            out->max = cnt[3];
+            out->ds_vpath = cnt[4];
+            out->as_vpath = cnt[5];
+            out->ds_wall = cnt[6];
+            out->as_wall = cnt[7];
+        }
+        if (sweeps_out) *sweeps_out = (int32_t)sweeps;
+        CK(hipMemcpyAsync(ctx->h_ctr, ctx->d_ctr, sizeof(ull) * K_NCTR, hipMemcpyDeviceToHost, ctx->stream));
+        CK(hipStreamSynchronize(ctx->stream));
+        if (ctx->h_ctr[K_WALK] > 0) {
+            ctx->err = "a separatrix walk exceeded N steps (cyclic gradient path)";
+            return DMTZ_E_WALK;
+        }
+        return status;
+    }
+};
+
+template <class T>
+int correct_dev(dmtz_ctx* ctx, const T* d_f, T* d_g, i64 nx, i64 ny, i64 nz, double eps, double range,
+                const dmtz_flags* flags, dmtz_counts* out, uint8_t* d_counter_out, int32_t* sweeps_out) {
+    int rc = check_dims(ctx, nx, ny, nz);
+    if (rc) return rc;
+    if (!flags || !d_f || !d_g) {
+        ctx->err = "null argument";
+        return DMTZ_E_ARG;
+    }
+    if (flags->preserve_connectors == 1) {
+        ctx->err = "preserve_connectors (2-saddle walls) is not supported on the device path";
+        return DMTZ_E_UNSUPPORTED;
+    }
+    if (flags->mode != 0) {
+        ctx->err = "unknown mode";
+        return DMTZ_E_ARG;
+    }
+    Dims d{nx, ny, nz};
+    Counts c = make_counts(d);
+    rc = ensure(ctx, c.N, sizeof(T), false);
+    if (rc) return rc;
+    Driver<T> drv{ctx, d_f, d_g, d, c, eps, range, *flags};
+    rc = drv.run(out, sweeps_out);
+    if (rc < 0) return rc;
+    if (d_counter_out) {
+        hipError_t e = hipMemcpyAsync(d_counter_out, ctx->d_counter, (size_t)c.N, hipMemcpyDeviceToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            ctx->err = hipGetErrorString(e);
+            return DMTZ_E_HIP;
+        }
+    }
+    return rc;
+}
+
+template <class T>
+int correct_host(dmtz_ctx* ctx, const T* f, T* g, i64 nx, i64 ny, i64 nz, double eps, double range,
+                 const dmtz_flags* flags, dmtz_counts* out, uint8_t* counter_out, int32_t* sweeps_out) {
+    int rc = check_dims(ctx, nx, ny, nz);
+    if (rc) return rc;
+    if (!f || !g) {
+        ctx->err = "null argument";
+        return DMTZ_E_ARG;
+    }
+    i64 N = nx * ny * nz;
+    rc = ensure(ctx, N, sizeof(T), true);
+    if (rc) return rc;
+    CK(hipMemcpyAsync(ctx->d_f, f, sizeof(T) * (size_t)N, hipMemcpyHostToDevice, ctx->stream));
+    CK(hipMemcpyAsync(ctx->d_g, g, sizeof(T) * (size_t)N, hipMemcpyHostToDevice, ctx->stream));
+    int st = correct_dev<T>(ctx, (const T*)ctx->d_f, (T*)ctx->d_g, nx, ny, nz, eps, range, flags, out, nullptr,
+                            sweeps_out);
+    if (st < 0) return st;
+    CK(hipMemcpyAsync(g, ctx->d_g, sizeof(T) * (size_t)N, hipMemcpyDeviceToHost, ctx->stream));
+    if (counter_out)
+        CK(hipMemcpyAsync(counter_out, ctx->d_counter, (size_t)N, hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    return st;
+}
+
+template <class T>
+int gradient_host(dmtz_ctx* ctx, const T* fld, i64 nx, i64 ny, i64 nz, int64_t* pv, int64_t* pe, int64_t* pf,
+                  int64_t* pt) {
+    int rc = check_dims(ctx, nx, ny, nz);
+    if (rc) return rc;
+    Dims d{nx, ny, nz};
+    Counts c = make_counts(d);
+    rc = ensure(ctx, c.N, sizeof(T), true);
+    if (rc) return rc;
+    CK(hipMemcpyAsync(ctx->d_f, fld, sizeof(T) * (size_t)c.N, hipMemcpyHostToDevice, ctx->stream));
+    int64_t* outs[4] = {pv, pe, pf, pt};
+    i64 sizes[4] = {c.N, c.E, c.F, c.T};
+    for (int k = 0; k < 4; k++) {
+        int64_t* dbuf = nullptr;
+        CK(hipMalloc(&dbuf, sizeof(int64_t) * (size_t)(sizes[k] > 0 ? sizes[k] : 1)));
+        if (sizes[k] > 0) {
+            k_pairing<T><<<nblocks(sizes[k]), BLOCK, 0, ctx->stream>>>((const T*)ctx->d_f, d, c, k, (ull)sizes[k], dbuf);
+            hipError_t e = hipGetLastError();
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(outs[k], dbuf, sizeof(int64_t) * (size_t)sizes[k], hipMemcpyDeviceToHost, ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            (void)hipFree(dbuf);
+            if (e != hipSuccess) {
+                ctx->err = hipGetErrorString(e);
+                return DMTZ_E_HIP;
+            }
+        } else {
+            (void)hipFree(dbuf);
+        }
+    }
+    return 0;
+}
+
+template <class T>
+int targets_host(dmtz_ctx* ctx, const T* f, const T* g, i64 nx, i64 ny, i64 nz, double eps, double range,
+                 const dmtz_flags* flags, uint8_t* targets, int64_t* counts4) {
+    int rc = check_dims(ctx, nx, ny, nz);
+    if (rc) return rc;
+    Dims d{nx, ny, nz};
+    Counts c = make_counts(d);
+    rc = ensure(ctx, c.N, sizeof(T), true);
+    if (rc) return rc;
+    CK(hipMemcpyAsync(ctx->d_f, f, sizeof(T) * (size_t)c.N, hipMemcpyHostToDevice, ctx->stream));
+    CK(hipMemcpyAsync(ctx->d_g, g, sizeof(T) * (size_t)c.N, hipMemcpyHostToDevice, ctx->stream));
+    CK(hipMemsetAsync(ctx->d_counter, 1, (size_t)c.N, ctx->stream));
+    CK(hipMemsetAsync(ctx->d_ctr, 0, sizeof(ull) * K_NCTR, ctx->stream));
+    Driver<T> drv{ctx, (const T*)ctx->d_f, (T*)ctx->d_g, d, c, eps, range, *flags};
+    rc = drv.detect();
+    if (rc) return rc;
+    ull cc[4] = {ctx->h_ctr[0], ctx->h_ctr[1], ctx->h_ctr[2], ctx->h_ctr[3]};
+    for (int k = 0; k < 4; k++) counts4[k] = (int64_t)cc[k];
+    rc = drv.begin_edits();
+    if (rc) return rc;
+    Lists4 L = rec_lists(ctx);
+    for (int ty = 0; ty < 4; ty++) {
+        if (cc[ty] == 0) continue;
+        if ((ty == 1 || ty == 2) && flags->preserve_saddles != 1) continue;
+        k_fix_cp<T><<<nblocks(cc[ty]), BLOCK, 0, ctx->stream>>>(ty, L.l[ty], cc[ty], (const T*)ctx->d_f,
+                                                                 (const T*)ctx->d_g, d, c, ctx->d_counter, eps, range,
+                                                                 flags->edit_type, drv.sink());
+        CK(hipGetLastError());
+    }
+    rc = sync_ctr(ctx);
+    if (rc) return rc;
+    ull nt = ctx->h_ctr[K_TGT];
+    std::vector<uint32_t> tl(nt);
+    if (nt) CK(hipMemcpyAsync(tl.data(), ctx->d_targets, sizeof(uint32_t) * nt, hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    std::memset(targets, 0, (size_t)c.N);
+    for (ull k = 0; k < nt; k++) targets[tl[k]] = 1;
+    return 0;
+}
+
+template <class T>
+int paths_host(dmtz_ctx* ctx, const T* f, const T* g, i64 nx, i64 ny, i64 nz, const dmtz_flags* flags,
+               int64_t* counts2, int64_t* ds_id, int64_t* ds_rec, int64_t* as_id, int64_t* as_rec, int64_t cap,
+               uint8_t* targets) {
+    int rc = check_dims(ctx, nx, ny, nz);
+    if (rc) return rc;
+    if (!flags || !counts2) {
+        ctx->err = "null argument";
+        return DMTZ_E_ARG;
+    }
+    Dims d{nx, ny, nz};
+    Counts c = make_counts(d);
+    rc = ensure(ctx, c.N, sizeof(T), true);
+    if (rc) return rc;
+    CK(hipMemcpyAsync(ctx->d_f, f, sizeof(T) * (size_t)c.N, hipMemcpyHostToDevice, ctx->stream));
+    CK(hipMemcpyAsync(ctx->d_g, g, sizeof(T) * (size_t)c.N, hipMemcpyHostToDevice, ctx->stream));
+    CK(hipMemsetAsync(ctx->d_counter, 1, (size_t)c.N, ctx->stream));
+    CK(hipMemsetAsync(ctx->d_ctr, 0, sizeof(ull) * K_NCTR, ctx->stream));
+    dmtz_flags fl = *flags;
+    fl.preserve_vpath = 1;
+    Driver<T> drv{ctx, (const T*)ctx->d_f, (T*)ctx->d_g, d, c, 0.0, 0.0, fl};
+    rc = drv.prepare_paths();
+    if (rc) return rc;
+    rc = drv.detect_paths();
+    if (rc) return rc;
+    ull nds = ctx->h_ctr[K_DS], nas = ctx->h_ctr[K_AS];
+    counts2[0] = (int64_t)nds;
+    counts2[1] = (int64_t)nas;
+    PathList pd = drv.path_list(0), pa = drv.path_list(1);
+    ull kd = nds < (ull)cap ? nds : (ull)cap, ka = nas < (ull)cap ? nas : (ull)cap;
+    if (kd && ds_id) CK(hipMemcpyAsync(ds_id, pd.id, sizeof(int64_t) * kd, hipMemcpyDeviceToHost, ctx->stream));
+    if (kd && ds_rec) CK(hipMemcpyAsync(ds_rec, pd.rec, sizeof(int64_t) * kd, hipMemcpyDeviceToHost, ctx->stream));
+    if (ka && as_id) CK(hipMemcpyAsync(as_id, pa.id, sizeof(int64_t) * ka, hipMemcpyDeviceToHost, ctx->stream));
+    if (ka && as_rec) CK(hipMemcpyAsync(as_rec, pa.rec, sizeof(int64_t) * ka, hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    if (targets) {
+        rc = drv.begin_edits();
+        if (rc) return rc;
+        if (nds + nas) {
+            k_fix_paths<T><<<nblocks(nds + nas), BLOCK, 0, ctx->stream>>>(pa.rec, nas, pd.rec, nds, (const T*)ctx->d_f,
+                                                                          (const T*)ctx->d_g, d, c, ctx->d_counter, 0.0,
+                                                                          0.0, fl.edit_type, drv.sink());
+            CK(hipGetLastError());
+        }
+        rc = sync_ctr(ctx);
+        if (rc) return rc;
+        ull nt = ctx->h_ctr[K_TGT];
+        std::vector<uint32_t> tl(nt);
+        if (nt) CK(hipMemcpyAsync(tl.data(), ctx->d_targets, sizeof(uint32_t) * nt, hipMemcpyDeviceToHost, ctx->stream));
+        CK(hipStreamSynchronize(ctx->stream));
+        std::memset(targets, 0, (size_t)c.N);
+        for (ull k = 0; k < nt; k++) targets[tl[k]] = 1;
+    }
+    CK(hipMemcpyAsync(ctx->h_ctr, ctx->d_ctr, sizeof(ull) * K_NCTR, hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    if (ctx->h_ctr[K_WALK] > 0) {
+        ctx->err = "a separatrix walk exceeded N steps (cyclic gradient path)";
+        return DMTZ_E_WALK;
+    }
+    return 0;
+}
+
+template <class T>
+int eps_from_rel(const T* f, int64_t n, double rel, double* eps_out, double* range_out) {
+    if (!f || n <= 0) return DMTZ_E_ARG;
+    double lo = (double)f[0], hi = (double)f[0];
+    for (int64_t i = 1; i < n; i++) {
+        double x = (double)f[i];
+        lo = x < lo ? x : lo;
+        hi = x > hi ? x : hi;
+    }
+    double range = hi - lo;
+    if (range_out) *range_out = range;
+    if (eps_out) *eps_out = range * rel;  // bound1 = (maxValue - minValue) * range1, :6307
+    return 0;
+}
+
+}  // namespace
+
+// ================================================================================================
+// C ABI
+// ================================================================================================
+extern "C" {
+
+int dmtz_ctx_create(dmtz_ctx** out, int device) {
+    if (!out) return DMTZ_E_ARG;
+    *out = nullptr;
+    dmtz_ctx* ctx = new dmtz_ctx();
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&ctx->d_ctr, sizeof(ull) * K_NCTR);
+    if (e == hipSuccess) e = hipHostMalloc(&ctx->h_ctr, sizeof(ull) * K_NCTR);
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "dmtz_ctx_create: %s\n", hipGetErrorString(e));
+        delete ctx;
+        return DMTZ_E_HIP;
+    }
+    *out = ctx;
+    return 0;
+}
+
+void dmtz_ctx_destroy(dmtz_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_bufs(ctx);
+    (void)hipFree(ctx->d_ctr);
+    (void)hipHostFree(ctx->h_ctr);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* dmtz_last_error(const dmtz_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int dmtz_eps_from_rel_f32(const float* f, int64_t n, double rel, double* e, double* r) {
+    return eps_from_rel(f, n, rel, e, r);
+}
+int dmtz_eps_from_rel_f64(const double* f, int64_t n, double rel, double* e, double* r) {
+    return eps_from_rel(f, n, rel, e, r);
+}
+
+int dmtz_correct_f32(dmtz_ctx* ctx, const float* f, float* g, int64_t nx, int64_t ny, int64_t nz, double eps,
+                     double range, const dmtz_flags* fl, dmtz_counts* out, uint8_t* counter_out, int32_t* sweeps) {
+    if (!ctx) return DMTZ_E_ARG;
+    (void)hipSetDevice(ctx->device);
+    return correct_host<float>(ctx, f, g, nx, ny, nz, eps, range, fl, out, counter_out, sweeps);
+}
+int dmtz_correct_f64(dmtz_ctx* ctx, const double* f, double* g, int64_t nx, int64_t ny, int64_t nz, double eps,
+                     double range, const dmtz_flags* fl, dmtz_counts* out, uint8_t* counter_out, int32_t* sweeps) {
+    if (!ctx) return DMTZ_E_ARG;
+    (void)hipSetDevice(ctx->device);
+    return correct_host<double>(ctx, f, g, nx, ny, nz, eps, range, fl, out, counter_out, sweeps);
+}
+int dmtz_correct_dev_f32(dmtz_ctx* ctx, const float* f, float* g, int64_t nx, int64_t ny, int64_t nz, double eps,
+                         double range, const dmtz_flags* fl, dmtz_counts* out, uint8_t* counter_out, int32_t* sweeps) {
+    if (!ctx) return DMTZ_E_ARG;
+    (void)hipSetDevice(ctx->device);
+    return correct_dev<float>(ctx, f, g, nx, ny, nz, eps, range, fl, out, counter_out, sweeps);
+}
+int dmtz_correct_dev_f64(dmtz_ctx* ctx, const double* f, double* g, int64_t nx, int64_t ny, int64_t nz, double eps,
+                         double range, const dmtz_flags* fl, dmtz_counts* out, uint8_t* counter_out, int32_t* sweeps) {
+    if (!ctx) return DMTZ_E_ARG;
+    (void)hipSetDevice(ctx->device);
+    return correct_dev<double>(ctx, f, g, nx, ny, nz, eps, range, fl, out, counter_out, sweeps);
+}
+
+int64_t dmtz_trajectory(const dmtz_ctx* ctx, int64_t* out, int64_t cap) {
+    if (!ctx) return 0;
+    int64_t n = (int64_t)ctx->traj.size() / 8;
+    if (out)
+        for (int64_t i = 0; i < n && i < cap; i++)
+            for (int k = 0; k < 8; k++) out[i * 8 + k] = ctx->traj[(size_t)i * 8 + k];
+    return n;
+}
+
+int dmtz_phase_times(const dmtz_ctx* ctx, double* out6) {
+    if (!ctx || !out6) return DMTZ_E_ARG;
+    for (int k = 0; k < 6; k++) out6[k] = ctx->phase[k];
+    return 0;
+}
+
+int dmtz_gradient_f32(dmtz_ctx* ctx, const float* fld, int64_t nx, int64_t ny, int64_t nz, int64_t* pv, int64_t* pe,
+                      int64_t* pf, int64_t* pt) {
+    if (!ctx) return DMTZ_E_ARG;
+    (void)hipSetDevice(ctx->device);
+    return gradient_host<float>(ctx, fld, nx, ny, nz, pv, pe, pf, pt);
+}
+int dmtz_gradient_f64(dmtz_ctx* ctx, const double* fld, int64_t nx, int64_t ny, int64_t nz, int64_t* pv, int64_t* pe,
+                      int64_t* pf, int64_t* pt) {
+    if (!ctx) return DMTZ_E_ARG;
+    (void)hipSetDevice(ctx->device);
+    return gradient_host<double>(ctx, fld, nx, ny, nz, pv, pe, pf, pt);
+}
+
+int dmtz_targets_f32(dmtz_ctx* ctx, const float* f, const float* g, int64_t nx, int64_t ny, int64_t nz, double eps,
+                     double range, const dmtz_flags* fl, uint8_t* targets, int64_t* counts4) {
+    if (!ctx) return DMTZ_E_ARG;
+    (void)hipSetDevice(ctx->device);
+    return targets_host<float>(ctx, f, g, nx, ny, nz, eps, range, fl, targets, counts4);
+}
+int dmtz_targets_f64(dmtz_ctx* ctx, const double* f, const double* g, int64_t nx, int64_t ny, int64_t nz, double eps,
+                     double range, const dmtz_flags* fl, uint8_t* targets, int64_t* counts4) {
+    if (!ctx) return DMTZ_E_ARG;
+    (void)hipSetDevice(ctx->device);
+    return targets_host<double>(ctx, f, g, nx, ny, nz, eps, range, fl, targets, counts4);
+}
+
+int dmtz_paths_f32(dmtz_ctx* ctx, const float* f, const float* g, int64_t nx, int64_t ny, int64_t nz,
+                   const dmtz_flags* fl, int64_t* counts2, int64_t* ds_id, int64_t* ds_rec, int64_t* as_id,
+                   int64_t* as_rec, int64_t cap, uint8_t* targets) {
+    if (!ctx) return DMTZ_E_ARG;
+    (void)hipSetDevice(ctx->device);
+    return paths_host<float>(ctx, f, g, nx, ny, nz, fl, counts2, ds_id, ds_rec, as_id, as_rec, cap, targets);
+}
+int dmtz_paths_f64(dmtz_ctx* ctx, const double* f, const double* g, int64_t nx, int64_t ny, int64_t nz,
+                   const dmtz_flags* fl, int64_t* counts2, int64_t* ds_id, int64_t* ds_rec, int64_t* as_id,
+                   int64_t* as_rec, int64_t cap, uint8_t* targets) {
+    if (!ctx) return DMTZ_E_ARG;
+    (void)hipSetDevice(ctx->device);
+    return paths_host<double>(ctx, f, g, nx, ny, nz, fl, counts2, ds_id, ds_rec, as_id, as_rec, cap, targets);
+}
+
+}  // extern "C"

@@ -34,6 +34,15 @@ DMTZ_HD void gather_star(const T* fld, const Dims& d, i64 v, T* val) {
     val[14] = fld[v];
 }
 
+// Out-of-line lower star of vertex v, shared by all pairing lookups (one copy per field type
+// instead of one inlined copy per lookup site).
+template <class T>
+DMTZ_HD_CALL void lower_star_at(const T* fld, Dims d, i64 v, LowerStar* ls) {
+    T val[15];
+    gather_star(fld, d, v, val);
+    lower_star(val, *ls);
+}
+
 template <class T>
 struct FieldGrad {
     const T* fld;
@@ -47,7 +56,9 @@ struct FieldGrad {
         T va = fld[a], vb = fld[b];
         return va > vb || (va == vb && a > b);
     }
-    DMTZ_HD void star(i64 v, LowerStar& ls) const {
+    DMTZ_HD void star(i64 v, LowerStar& ls) const { lower_star_at<T>(fld, d, v, &ls); }
+    // Inlined evaluation for the dense per-vertex kernels.
+    DMTZ_HD void star_inline(i64 v, LowerStar& ls) const {
         T val[15];
         gather_star(fld, d, v, val);
         lower_star(val, ls);

@@ -460,113 +460,118 @@ struct Repair {
         }
     }
 
-    // is_false main_3d.cu:4437-4497: walk the f ascending path from tet id; true with the
-    // first tet whose f and g partner faces differ. Walks are capped at cap steps.
-    DMTZ_HD bool is_false(i64 id, i64& divergent, i64 cap, bool& overflow) const {
-        i64 cur = id, old;
+    // Walks are written as one loop with a completion flag and no early exit from nested
+    // loops: the shape the device compiler handles reliably for these long divergent loops.
+
+    // is_false main_3d.cu:4437-4497: walk f's ascending path from tet id; on the first tet
+    // whose f and g partner faces differ, found = true and divergent = that tet. Capped at cap
+    // steps (overflow = true).
+    DMTZ_HD void as_walk(i64 id, i64 cap, bool& found, i64& divergent, bool& overflow) const {
+        i64 cur = id;
+        bool done = id < 0;
+        found = false;
+        divergent = -1;
         i64 steps = 0;
-        do {
-            old = cur;
+        while (!done) {
             i64 t = F.pt(cur);
-            if (t == -1) break;
-            i64 dt = G.pt(cur);
-            if (t != dt) {
+            i64 dt = t == -1 ? -1 : G.pt(cur);
+            i64 pft = (t == -1 || t != dt) ? -1 : F.pf(t);
+            bool chain = pft != -1 && F.pt(pft) == t;
+            i64 t1 = -1, t2 = -1;
+            if (chain) cofacets(t, t1, t2);
+            i64 nxt = (t1 == cur) ? t2 : t1;
+            if (t != -1 && t != dt) {
+                found = true;
                 divergent = cur;
-                return true;
             }
-            i64 pft = F.pf(t);
-            if (pft == -1 || F.pt(pft) != t) break;
-            i64 t1, t2;
-            cofacets(t, t1, t2);
-            cur = (t1 == cur) ? t2 : t1;
-            if (++steps > cap) {
+            done = t == -1 || t != dt || !chain || nxt == cur || nxt == -1;
+            cur = nxt;
+            if (!done && ++steps > cap) {
                 overflow = true;
-                break;
+                done = true;
             }
-        } while (cur != old && cur != -1);
-        return false;
+        }
     }
 
-    // get_paired_id main_3d.cu:4499-4537 on field which (0 = f, 1 = g).
+    // get_paired_id main_3d.cu:4499-4537 on field which (0 = f, 1 = g): the tet reached by
+    // following the ascending pairing from tet id.
     DMTZ_HD i64 paired_max(i64 id, int which, i64 cap, bool& overflow) const {
         const FieldGrad<T>& X = which == 0 ? F : G;
-        i64 cur = id, old;
+        i64 cur = id;
+        bool done = id < 0;
         i64 steps = 0;
-        do {
-            old = cur;
+        while (!done) {
             i64 t = X.pt(cur);
-            if (t == -1) break;
-            i64 pft = X.pf(t);
-            if (pft == -1 || X.pt(pft) != t) break;
-            i64 t1, t2;
-            cofacets(t, t1, t2);
-            cur = (t1 == cur) ? t2 : t1;
-            if (++steps > cap) {
+            i64 pft = t == -1 ? -1 : X.pf(t);
+            bool chain = pft != -1 && X.pt(pft) == t;
+            i64 t1 = -1, t2 = -1;
+            if (chain) cofacets(t, t1, t2);
+            i64 nxt = (t1 == cur) ? t2 : t1;
+            done = !chain || nxt == cur || nxt == -1;
+            if (chain) cur = nxt;  // leaving through the boundary yields -1, as the reference
+            if (!done && ++steps > cap) {
                 overflow = true;
-                break;
+                done = true;
             }
-        } while (cur != old && cur != -1);
+        }
         return cur;
     }
 
     // get_false_ds_vpath main_3d.cu:4837-4959 for one f-critical edge. The reference's
-    // dec_current_id (:4849) never advances, so each walk compares only its start vertex.
+    // dec_current_id (:4849) never advances, so each walk compares only its start vertex:
+    // v1's f and g pairings, then v2's; the terminal f minimum is not compared.
     DMTZ_HD bool ds_geometry(i64 e, i64& rec) const {
         i64 v1, v2;
         edge_verts(F.d, F.c, e, v1, v2);
-        i64 ends[2] = {v1, v2};
-        for (int w = 0; w < 2; w++) {
-            i64 v = ends[w];
-            i64 ce = F.pv(v);
-            if (ce == -1) continue;
-            if (ce != G.pv(v)) {
-                rec = v;
-                return true;
-            }
-        }
-        return false;
+        i64 c1 = F.pv(v1), c2 = F.pv(v2);
+        bool b1 = c1 != -1 && c1 != G.pv(v1);
+        bool b2 = c2 != -1 && c2 != G.pv(v2);
+        rec = b1 ? v1 : v2;
+        return b1 || b2;
     }
 
-    // get_orginal_saddle_min_pairs main_3d.cu:4613-4687: f minimum reached from v.
+    // get_orginal_saddle_min_pairs main_3d.cu:4613-4687: the f minimum reached from start.
     DMTZ_HD i64 f_min_label(i64 start, i64 cap, bool& overflow) const {
-        i64 cur = start, v = start;
+        i64 cur = start;
+        bool done = start < 0;
         i64 steps = 0;
-        while (true) {
-            v = cur;
-            i64 ce = F.pv(v);
-            if (ce == -1) break;
-            if (F.pe(ce) == -1) break;
-            i64 a, b;
-            edge_verts(F.d, F.c, ce, a, b);
-            cur = (a == cur) ? b : a;
-            if (++steps > cap) {
+        while (!done) {
+            i64 ce = F.pv(cur);
+            bool chain = ce != -1 && F.pe(ce) != -1;
+            i64 a = -1, b = -1;
+            if (chain) edge_verts(F.d, F.c, ce, a, b);
+            done = !chain;
+            if (chain) cur = (a == cur) ? b : a;
+            if (!done && ++steps > cap) {
                 overflow = true;
-                break;
+                done = true;
             }
         }
-        return v;
+        return cur;
     }
 
     // get_false_ds_vpath_connectivity main_3d.cu:4961-5062, one endpoint: walk g's descending
-    // path, remember the first vertex whose f and g pairings differ, return the g minimum.
+    // path, keep the first vertex whose f and g pairings differ (trouble stays if already set),
+    // return the g minimum reached.
     DMTZ_HD i64 g_min_walk(i64 start, i64& trouble, i64 cap, bool& overflow) const {
-        i64 cur = start, v = start;
+        i64 cur = start;
+        bool done = start < 0;
         i64 steps = 0;
-        while (true) {
-            v = cur;
-            i64 dce = G.pv(v);
-            if (dce == -1) break;
-            if (F.pv(v) != dce && trouble == -1) trouble = v;
-            if (G.pe(dce) == -1) break;
-            i64 a, b;
-            edge_verts(F.d, F.c, dce, a, b);
-            cur = (a == cur) ? b : a;
-            if (++steps > cap) {
+        while (!done) {
+            i64 dce = G.pv(cur);
+            i64 fce = dce == -1 ? -1 : F.pv(cur);
+            if (dce != -1 && fce != dce && trouble == -1) trouble = cur;
+            bool chain = dce != -1 && G.pe(dce) != -1;
+            i64 a = -1, b = -1;
+            if (chain) edge_verts(F.d, F.c, dce, a, b);
+            done = !chain;
+            if (chain) cur = (a == cur) ? b : a;
+            if (!done && ++steps > cap) {
                 overflow = true;
-                break;
+                done = true;
             }
         }
-        return v;
+        return cur;
     }
 };
 

@@ -15,8 +15,10 @@
 
 #if defined(__HIPCC__)
 #define DMTZ_HD __host__ __device__ __forceinline__
+#define DMTZ_HD_CALL __host__ __device__ __attribute__((noinline))
 #else
 #define DMTZ_HD inline
+#define DMTZ_HD_CALL __attribute__((noinline))
 #endif
 
 namespace dmtz {

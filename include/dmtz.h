@@ -113,6 +113,19 @@ int dmtz_targets_f32(dmtz_ctx* ctx, const float* f, const float* g, int64_t nx, 
 int dmtz_targets_f64(dmtz_ctx* ctx, const double* f, const double* g, int64_t nx, int64_t ny, int64_t nz,
                      double eps, double range, const dmtz_flags* flags, uint8_t* targets, int64_t* counts4);
 
+/* Separatrix detection on (f, g) as they stand (main_3d.cu:4437-5062 in the mode the flags
+ * select: geometry = get_false_as_vpath / get_false_ds_vpath, connectivity = the _connectivity
+ * variants with f's endpoint labels of main_3d.cu:4613-4759). counts2 = {descending, ascending};
+ * ds_id/ds_rec receive (f-critical edge, recorded vertex) and as_id/as_rec (tet, divergent tet)
+ * pairs, as many as the caller's capacity cap allows (same order). When targets is non-null it
+ * receives the edit set of fix_as_vpath / fix_ds_vpath (main_3d.cu:5135-5295) over those lists. */
+int dmtz_paths_f32(dmtz_ctx* ctx, const float* f, const float* g, int64_t nx, int64_t ny, int64_t nz,
+                   const dmtz_flags* flags, int64_t* counts2, int64_t* ds_id, int64_t* ds_rec, int64_t* as_id,
+                   int64_t* as_rec, int64_t cap, uint8_t* targets);
+int dmtz_paths_f64(dmtz_ctx* ctx, const double* f, const double* g, int64_t nx, int64_t ny, int64_t nz,
+                   const dmtz_flags* flags, int64_t* counts2, int64_t* ds_id, int64_t* ds_rec, int64_t* as_id,
+                   int64_t* as_rec, int64_t cap, uint8_t* targets);
+
 #ifdef __cplusplus
 }
 #endif

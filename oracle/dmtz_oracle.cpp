@@ -1902,3 +1902,34 @@ extern "C" int orc_correct_f64(const double* f, double* g, i64 W, i64 H, i64 D, 
     return correct_impl<double>(f, g, W, H, D, eps, range, fl, counts8, counter_out, sweeps_out, traj, traj_cap, phase_s);
 }
 
+
+// Separatrix detection + repair phase on (f, g) without applying: the edit set of
+// fix_as_vpath / fix_ds_vpath (main_3d.cu:5135-5295) over the lists of detect_paths.
+template <class T>
+static int path_targets_impl(const T* f, const T* gfield, i64 W, i64 H, i64 D, double eps, double range,
+                             const orc_flags* fl, uint8_t* targets, i64* counts2) {
+    init_dirs();
+    State<T> s(W, H, D);
+    s.f = f;
+    std::vector<T> gcopy(gfield, gfield + s.g.N);
+    s.d = gcopy.data();
+    s.eps = eps;
+    s.range = range;
+    std::memcpy(&s.fl, fl, sizeof(orc_flags));
+    s.literal = false;
+    s.alloc();
+    s.gradient(F_FIELD);
+    s.gradient(G_FIELD);
+    if (fl->preserve_vpath == 1 && fl->preserve_geometry == 0) s.compute_labels();
+    s.detect_paths();
+    counts2[0] = (i64)s.l_ds.size();
+    counts2[1] = (i64)s.l_as.size();
+    s.for_list(s.l_as, [&](i64 id) { s.fix_as_vpath(id); });
+    s.for_list(s.l_ds, [&](i64 id) { s.fix_ds_vpath(id); });
+    std::memcpy(targets, s.target.data(), s.g.N);
+    return 0;
+}
+extern "C" int orc_path_targets_f32(const float* f, const float* g, i64 W, i64 H, i64 D, double eps, double range,
+                                    const orc_flags* fl, uint8_t* targets, i64* counts2) {
+    return path_targets_impl<float>(f, g, W, H, D, eps, range, fl, targets, counts2);
+}

@@ -97,3 +97,89 @@ extern "C" int hc_targets_f64(const double* f, const double* g, i64 W, i64 H, i6
                               int ps, int et, uint8_t* targets, i64* counts4) {
     return targets_impl<double>(f, g, W, H, D, eps, range, ps, et, targets, counts4);
 }
+
+// Separatrix mismatch lists (geometry or connectivity mode) on (f, g) as they stand, with the
+// same per-cell logic as the device kernels: descending over f-critical edges, ascending over
+// f-critical faces. Outputs (id, recorded) pairs; returns -1 when a walk hit the cap.
+struct NoSinkH {
+    void operator()(i64) const {}
+};
+template <class T>
+static int paths_impl(const T* f, const T* g, i64 W, i64 H, i64 D, int geometry, i64* counts2, i64* ds_id,
+                      i64* ds_rec, i64* as_id, i64* as_rec) {
+    Dims d{W, H, D};
+    Counts c = make_counts(d);
+    Repair<T, NoSinkH> R(f, g, d, c, nullptr, 0, 0, 0, NoSinkH{});
+    Repair<T, NoSinkH> RF(f, f, d, c, nullptr, 0, 0, 0, NoSinkH{});
+    FieldGrad<T> Fg(f, d, c);
+    bool ovf = false;
+    i64 nds = 0, nas = 0;
+    for (i64 v = 0; v < c.N; v++) {
+        LowerStar ls;
+        Fg.star(v, ls);
+        uint64_t lo;
+        uint32_t hi;
+        crit_mask(ls, lo, hi);
+        for (int lc = 1; lc < LC_TET; lc++) {
+            if (!lc_bit(lo, hi, lc)) continue;
+            i64 id = local_cell_id(d, c, v, lc);
+            if (lc < LC_TRI) {
+                i64 rec = -1;
+                if (geometry) {
+                    if (R.ds_geometry(id, rec)) { ds_id[nds] = id; ds_rec[nds++] = rec; }
+                } else {
+                    i64 a, b;
+                    edge_verts(d, c, id, a, b);
+                    i64 la = RF.f_min_label(a, c.N, ovf), lb = RF.f_min_label(b, c.N, ovf);
+                    i64 trouble = -1;
+                    i64 end = R.g_min_walk(a, trouble, c.N, ovf);
+                    if (end != la) { ds_id[nds] = id; ds_rec[nds++] = trouble; }
+                    else {
+                        end = R.g_min_walk(b, trouble, c.N, ovf);
+                        if (end != lb) { ds_id[nds] = id; ds_rec[nds++] = trouble; }
+                    }
+                }
+            } else {
+                i64 q[2];
+                R.cofacets(id, q[0], q[1]);
+                for (int k = 0; k < 2; k++) {
+                    if (q[k] < 0) continue;
+                    if (!geometry) {
+                        i64 lab = RF.paired_max(q[k], 0, c.N, ovf);
+                        if (R.paired_max(q[k], 1, c.N, ovf) == lab) continue;
+                    }
+                    i64 dv;
+                    bool found;
+                    R.as_walk(q[k], c.N, found, dv, ovf);
+                    if (found) { as_id[nas] = q[k]; as_rec[nas++] = dv; break; }
+                }
+            }
+        }
+    }
+    counts2[0] = nds;
+    counts2[1] = nas;
+    return ovf ? -1 : 0;
+}
+extern "C" int hc_paths_f32(const float* f, const float* g, i64 W, i64 H, i64 D, int geometry, i64* counts2,
+                            i64* ds_id, i64* ds_rec, i64* as_id, i64* as_rec) {
+    return paths_impl<float>(f, g, W, H, D, geometry, counts2, ds_id, ds_rec, as_id, as_rec);
+}
+
+// Edit set of the separatrix repairs over given (tet, divergent tet) and (edge, vertex) lists.
+template <class T>
+static int path_targets_impl(const T* f, const T* g, i64 W, i64 H, i64 D, double eps, double range, int edit_type,
+                             const i64* as_rec, i64 nas, const i64* ds_rec, i64 nds, uint8_t* targets) {
+    Dims d{W, H, D};
+    Counts c = make_counts(d);
+    std::vector<uint8_t> tg(c.N, 0);
+    std::vector<uint8_t> counter(c.N, 1);
+    Repair<T, VecSink> R(f, g, d, c, counter.data(), eps, range, edit_type, VecSink{&tg});
+    for (i64 k = 0; k < nas; k++) R.fix_as_vpath(as_rec[k]);
+    for (i64 k = 0; k < nds; k++) R.fix_ds_vpath(ds_rec[k]);
+    std::memcpy(targets, tg.data(), c.N);
+    return 0;
+}
+extern "C" int hc_path_targets_f32(const float* f, const float* g, i64 W, i64 H, i64 D, double eps, double range,
+                                   int et, const i64* as_rec, i64 nas, const i64* ds_rec, i64 nds, uint8_t* targets) {
+    return path_targets_impl<float>(f, g, W, H, D, eps, range, et, as_rec, nas, ds_rec, nds, targets);
+}

@@ -1,0 +1,155 @@
+"""GPU parity of the HIP path against the oracle, through the C ABI (include/dmtz.h).
+
+Integer / index work is bit-exact: pairing arrays, mismatch counts, edit sets, per-sweep count
+trajectories, delta counters and the corrected field g' (float bits) must equal the oracle's on
+the same seeded inputs.
+"""
+import numpy as np
+import pytest
+
+import dmtz_amd
+from dmtz_amd import fields
+
+pytestmark = pytest.mark.gpu
+
+
+def _gradient_cases():
+    out = []
+    for seed in range(3):
+        for shape in [(7, 6, 5), (5, 8, 9), (9, 9, 9), (4, 3, 6)]:
+            out.append(fields.gaussian_mixture(shape, 3, seed=seed, noise=0.05))
+    rng = np.random.default_rng(1)
+    for k in range(3):
+        out.append(rng.integers(0, 3, size=(6, 7, 8)).astype(np.float32))
+        out.append(rng.integers(0, 2, size=(5, 5, 5)).astype(np.float64))
+    out.append(fields.gaussian_mixture((17, 19, 23), 6, seed=9, noise=0.02))
+    return out
+
+
+def test_gradient_bit_exact(oracle, dev):
+    for f in _gradient_cases():
+        print("gradient", f.shape, f.dtype, flush=True)
+        A = oracle.gradient(f)
+        B = dmtz_amd.gradient(f, ctx=dev)
+        for name, a, b in zip(("vertices", "edges", "faces", "tets"), A, B):
+            assert (a == b).all(), (f.shape, name, np.nonzero(a != b)[0][:5])
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("edit_type", [0, 1])
+def test_edit_set_bit_exact(oracle, dev, dtype, edit_type):
+    for seed in range(3):
+        for shape, rel in [((8, 9, 10), 1e-2), ((16, 16, 16), 3e-2), ((6, 7, 9), 5e-2)]:
+            f = fields.gaussian_mixture(shape, 4, seed=seed, noise=0.02).astype(dtype)
+            eps, rng = fields.eps_from_rel(f, rel)
+            g = fields.quantize(f, eps)
+            print("targets", shape, dtype.__name__, edit_type, seed, flush=True)
+            want, wc = oracle.targets(f, g, eps, rng, oracle.flags(edit_type=edit_type))
+            got, gc = dmtz_amd.targets(f, g, eps, rng, dmtz_amd.CorrectionFlags(edit_type=edit_type),
+                                       ctx=dev)
+            assert gc == wc
+            assert (got == want).all()
+
+
+FLAG_CASES = [
+    ("cp", {}),
+    ("geometry", dict(preserve_vpath=1, preserve_geometry=1)),
+    ("connectivity", dict(preserve_vpath=1, preserve_geometry=0)),
+    ("edit_type1", dict(edit_type=1, preserve_vpath=1, preserve_geometry=1)),
+    ("filtration", dict(filtration=1)),
+    ("preserve_type", dict(preserve_type=1)),
+    # without saddle repairs these inputs do not converge (the reference has no sweep cap and
+    # would loop forever); both sides must stop at the cap in the same state
+    ("no_saddles", dict(preserve_saddles=0, max_sweeps=300)),
+]
+
+
+def _compare_run(oracle, dev, f, g, eps, rng, fl):
+    want = oracle.correct(f, g, eps, rng, oracle.flags(**fl))
+    got = dmtz_amd.correct(f, g, eps, rng, dmtz_amd.CorrectionFlags(**fl), ctx=dev,
+                           return_counter=True)
+    assert got.status == want["status"]
+    assert got.sweeps == want["sweeps"]
+    assert got.counts == want["counts"]
+    assert (got.trajectory == want["traj"]).all()
+    assert (got.counter == want["counter"]).all()
+    assert got.g.tobytes() == want["g"].tobytes()
+    return got
+
+
+@pytest.mark.parametrize("name,fl", FLAG_CASES, ids=[c[0] for c in FLAG_CASES])
+def test_correction_bit_exact(oracle, dev, name, fl):
+    for seed, shape, rel in [(0, (16, 16, 16), 1e-2), (1, (24, 20, 12), 2e-2), (2, (13, 17, 11), 5e-2)]:
+        f = fields.gaussian_mixture(shape, 6, seed=seed, noise=0.02)
+        eps, rng = fields.eps_from_rel(f, rel)
+        g = fields.quantize(f, eps)
+        print("correct", name, shape, seed, flush=True)
+        got = _compare_run(oracle, dev, f, g, eps, rng, fl)
+        if "max_sweeps" not in fl:
+            assert got.counts == (0,) * 8
+
+
+def test_correction_bit_exact_f64(oracle, dev):
+    f = fields.gaussian_mixture((20, 18, 16), 6, seed=4, noise=0.02).astype(np.float64)
+    eps, rng = fields.eps_from_rel(f, 2e-2)
+    g = fields.quantize(f, eps)
+    for fl in ({}, dict(preserve_vpath=1, preserve_geometry=1)):
+        _compare_run(oracle, dev, f, g, eps, rng, fl)
+
+
+def test_correction_c1_64cubed(oracle, dev):
+    """The CPU-runnable workload: 64^3 Gaussian mixture (8 bumps), eps = 1e-3 range, geometry."""
+    f = fields.gaussian_mixture((64, 64, 64), 8, seed=0)
+    eps, rng = fields.eps_from_rel(f, 1e-3)
+    g = fields.quantize(f, eps)
+    got = _compare_run(oracle, dev, f, g, eps, rng, dict(preserve_vpath=1, preserve_geometry=1))
+    assert got.counts == (0,) * 8
+
+
+def test_nonconvergence_is_reported(dev):
+    f = fields.gaussian_mixture((16, 16, 16), 6, seed=0, noise=0.02)
+    eps, rng = fields.eps_from_rel(f, 2e-2)
+    g = fields.quantize(f, eps)
+    r = dmtz_amd.correct(f, g, eps, rng, dmtz_amd.CorrectionFlags(max_sweeps=1), ctx=dev)
+    assert r.status == 1 and r.sweeps == 1 and any(c > 0 for c in r.counts)
+
+
+def test_bad_arguments_raise(dev):
+    f = np.zeros((4, 4, 2), np.float32)  # nx = 2 is below the minimum grid
+    with pytest.raises(dmtz_amd.DmtzError):
+        dmtz_amd.correct(f, f, 0.1, 1.0, ctx=dev)
+    f = np.zeros((4, 4, 4), np.float32)
+    with pytest.raises(dmtz_amd.DmtzError):
+        dmtz_amd.correct(f, f, 0.1, 1.0, dmtz_amd.CorrectionFlags(preserve_connectors=1), ctx=dev)
+
+
+@pytest.mark.parametrize("geometry", [1, 0])
+def test_separatrix_lists_bit_exact(oracle, dev, geometry):
+    """Separatrix detection lists and the path-repair edit set, on g whose critical cells
+    already match f's (the state in which the reference runs these checks)."""
+    import ctypes
+
+    L = oracle.lib()
+    P = ctypes.c_void_p
+    I = ctypes.c_int64
+    L.orc_path_targets_f32.argtypes = [P, P, I, I, I, ctypes.c_double, ctypes.c_double, P, P, P]
+    for seed in range(4):
+        for shape, rel in [((12, 12, 12), 1e-2), ((10, 14, 9), 3e-2), ((16, 16, 16), 5e-3)]:
+            f = fields.gaussian_mixture(shape, 5, seed=seed, noise=0.02)
+            eps, rng = fields.eps_from_rel(f, rel)
+            g = oracle.correct(f, fields.quantize(f, eps), eps, rng, oracle.flags())["g"]
+            fl = dict(preserve_vpath=1, preserve_geometry=geometry)
+            print("paths", shape, seed, geometry, flush=True)
+            want = oracle.paths(f, g, oracle.flags(**fl))
+            got = dmtz_amd.paths(f, g, dmtz_amd.CorrectionFlags(**fl), ctx=dev, with_targets=True)
+            assert sorted(zip(got["ds"].tolist(), got["ds_rec"].tolist())) == \
+                sorted(zip(want["ds"].tolist(), want["ds_rec"].tolist()))
+            assert sorted(zip(got["as_"].tolist(), got["as_rec"].tolist())) == \
+                sorted(zip(want["as_"].tolist(), want["as_rec"].tolist()))
+            D, H, W = shape
+            tg = np.zeros(W * H * D, np.uint8)
+            c2 = np.zeros(2, np.int64)
+            ofl = oracle.flags(**fl)
+            L.orc_path_targets_f32(f.ctypes.data_as(P), g.ctypes.data_as(P), W, H, D, eps, rng,
+                                   ctypes.byref(ofl), tg.ctypes.data_as(P), c2.ctypes.data_as(P))
+            assert (got["targets"].ravel() == tg.astype(bool)).all()

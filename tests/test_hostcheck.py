@@ -1,0 +1,88 @@
+"""The product's device-agnostic headers (lower_star.h, gradient.h, mismatch.h, repair.h), built
+for the host with g++ as a test harness, against the oracle: pairing arrays bit-exact, mismatch
+counts and repair edit sets identical. This runs the same classifier and repair code the HIP
+kernels run, without a GPU."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from dmtz_amd import fields
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SRC = os.path.join(HERE, "native", "hostcheck.cpp")
+SO = os.path.join(HERE, "native", "libhostcheck.so")
+HDRS = [os.path.join(ROOT, "dmtz_amd", "csrc", h) for h in
+        ("stencil.h", "lower_star.h", "gradient.h", "mismatch.h", "repair.h", "edit.h")]
+
+P = ctypes.c_void_p
+I = ctypes.c_int64
+
+
+@pytest.fixture(scope="module")
+def hc():
+    newest = max(os.path.getmtime(p) for p in HDRS + [SRC])
+    if not os.path.exists(SO) or os.path.getmtime(SO) < newest:
+        subprocess.check_call(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+                               "-I" + os.path.join(ROOT, "dmtz_amd", "csrc"), "-o", SO, SRC])
+    L = ctypes.CDLL(SO)
+    L.hc_gradient_f32.argtypes = [P, I, I, I, P, P, P, P]
+    L.hc_gradient_f64.argtypes = [P, I, I, I, P, P, P, P]
+    for t in ("f32", "f64"):
+        getattr(L, "hc_targets_" + t).argtypes = [P, P, I, I, I, ctypes.c_double, ctypes.c_double,
+                                                 ctypes.c_int, ctypes.c_int, P, P]
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(P)
+
+
+def hc_gradient(L, f, oracle):
+    D, H, W = f.shape
+    outs = [np.empty(n, np.int64) for n in oracle.counts(W, H, D)]
+    fn = L.hc_gradient_f32 if f.dtype == np.float32 else L.hc_gradient_f64
+    fn(_p(f), W, H, D, *[_p(o) for o in outs])
+    return outs
+
+
+def gradient_cases():
+    cases = []
+    for seed in range(4):
+        for shape in [(7, 6, 5), (5, 8, 9), (9, 9, 9), (4, 3, 6)]:
+            cases.append(fields.gaussian_mixture(shape, 3, seed=seed, noise=0.05))
+    rng = np.random.default_rng(1)
+    for k in range(4):
+        cases.append(rng.integers(0, 3, size=(6, 7, 8)).astype(np.float32))
+        cases.append(rng.integers(0, 2, size=(5, 5, 5)).astype(np.float64))
+    cases.append(np.zeros((3, 4, 5), np.float32))
+    return cases
+
+
+def test_lower_star_gradient_matches_oracle(hc, oracle):
+    for f in gradient_cases():
+        A = oracle.gradient(f)
+        B = hc_gradient(hc, f, oracle)
+        for name, a, b in zip(("vertices", "edges", "faces", "tets"), A, B):
+            assert (a == b).all(), (f.shape, f.dtype, name, np.nonzero(a != b)[0][:5])
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("edit_type", [0, 1])
+def test_repair_edit_set_matches_oracle(hc, oracle, dtype, edit_type):
+    for seed in range(5):
+        for shape, rel in [((8, 9, 10), 1e-2), ((12, 12, 12), 3e-2), ((6, 7, 9), 5e-2)]:
+            f = fields.gaussian_mixture(shape, 4, seed=seed, noise=0.02).astype(dtype)
+            eps, rng = fields.eps_from_rel(f, rel)
+            g = fields.quantize(f, eps)
+            want, wc = oracle.targets(f, g, eps, rng, oracle.flags(edit_type=edit_type))
+            D, H, W = shape
+            tg = np.zeros(W * H * D, np.uint8)
+            c4 = np.zeros(4, np.int64)
+            fn = hc.hc_targets_f32 if dtype == np.float32 else hc.hc_targets_f64
+            fn(_p(f), _p(g), W, H, D, eps, rng, 1, edit_type, _p(tg), _p(c4))
+            assert tuple(c4) == wc
+            assert (tg.reshape(shape).astype(bool) == want).all()

@@ -1,0 +1,221 @@
+"""Pins of the CPU oracle (oracle/dmtz_oracle.cpp) against what the mathematics fixes.
+
+The reference ships no tests, fixtures or golden vectors, so the oracle is pinned by: closed-form
+simplex counts (main_3d.cu:6401-6415), ID bijections and clique structure of the implicit complex,
+the Euler characteristic of any discrete gradient, matching validity, the literal 14^k
+brute-force passes against the direct enumeration, special cases with a known answer (linear
+fields), invariance under strictly monotone value maps, and end-state invariants of the loop.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from dmtz_amd import fields
+
+DIRS = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (-1, 1, 0), (1, -1, 0),
+        (0, 1, 1), (0, -1, -1), (-1, 0, 1), (1, 0, -1), (-1, 1, 1), (1, -1, -1)]
+
+
+def closed_counts(W, H, D):
+    E = ((W - 1) * H * D + W * (H - 1) * D + W * H * (D - 1) + (W - 1) * (H - 1) * D
+         + (W - 1) * H * (D - 1) + W * (H - 1) * (D - 1) + (W - 1) * (H - 1) * (D - 1))
+    F = 2 * ((W - 1) * (H - 1) * D + W * (H - 1) * (D - 1) + (W - 1) * H * (D - 1)) \
+        + 6 * (W - 1) * (H - 1) * (D - 1)
+    return W * H * D, E, F, 6 * (W - 1) * (H - 1) * (D - 1)
+
+
+def test_counts_closed_form(oracle):
+    assert oracle.counts(5, 6, 7) == (210, 1077, 1588, 720)
+    # the survey's table (sizes at the configured workloads)
+    assert oracle.counts(64, 64, 64) == (262144, 1786239, 3024378, 1500282)
+    assert oracle.counts(256, 256, 256) == (16777216, 116655615, 199366650, 99488250)
+    assert oracle.counts(512, 512, 512) == (134217728, 936381439, 1602760698, 800596986)
+    for dims in [(3, 2, 2), (9, 7, 5), (4, 11, 3)]:
+        assert oracle.counts(*dims) == closed_counts(*dims)
+        N, E, F, T = closed_counts(*dims)
+        assert N - E + F - T == 1  # Euler characteristic of a ball
+
+
+def _adjacent(a, b):
+    return tuple(q - p for p, q in zip(a, b)) in set(DIRS)
+
+
+@pytest.mark.parametrize("dims", [(5, 6, 7), (4, 3, 5), (3, 4, 3)])
+def test_complex_is_clique_complex(oracle, dims):
+    """Edges, triangles and tets of the numbering are exactly the 2-, 3- and 4-cliques of the
+    14-neighbour graph, and every ID round-trips."""
+    W, H, D = dims
+    N, E, F, T = oracle.counts(W, H, D)
+    xyz = [(v % W, (v // W) % H, v // (W * H)) for v in range(N)]
+    edges = set()
+    for v in range(N):
+        for d in DIRS:
+            x, y, z = xyz[v]
+            n = (x + d[0], y + d[1], z + d[2])
+            if 0 <= n[0] < W and 0 <= n[1] < H and 0 <= n[2] < D:
+                u = n[0] + n[1] * W + n[2] * W * H
+                edges.add((max(u, v), min(u, v)))
+    assert len(edges) == E
+    got = set(oracle.edge_verts(dims, e) for e in range(E))
+    assert got == edges
+    for e in range(E):
+        a, b = oracle.edge_verts(dims, e)
+        assert oracle.edge_id(dims, a, b) == e and oracle.edge_id(dims, b, a) == e
+    adj = {v: set() for v in range(N)}
+    for a, b in edges:
+        adj[a].add(b)
+        adj[b].add(a)
+    tris = set()
+    for a, b in edges:
+        for c in adj[a] & adj[b]:
+            tris.add(tuple(sorted((a, b, c), reverse=True)))
+    assert len(tris) == F
+    got = set(oracle.tri_verts(dims, t) for t in range(F))
+    assert got == tris
+    for t in range(F):
+        a, b, c = oracle.tri_verts(dims, t)
+        for p in itertools.permutations((a, b, c)):
+            assert oracle.tri_id(dims, *p) == t
+    tets = set()
+    for a, b, c in tris:
+        for d in adj[a] & adj[b] & adj[c]:
+            tets.add(tuple(sorted((a, b, c, d), reverse=True)))
+    assert len(tets) == T
+    got = set(oracle.tet_verts(dims, q) for q in range(T))
+    assert got == tets
+    for q in range(T):
+        vs = oracle.tet_verts(dims, q)
+        assert oracle.tet_id(dims, *vs[::-1]) == q
+
+
+def test_cell_tetra_cofacets(oracle):
+    """get_cell_tetra: every face has 1 or 2 cofacet tets, 2 in the interior; literal search and
+    direct enumeration agree."""
+    dims = (5, 4, 6)
+    N, E, F, T = oracle.counts(*dims)
+    seen = np.zeros(T, np.int64)
+    for t in range(F):
+        x, y = oracle.cell_tetra(dims, t, literal=True)
+        assert (x, y) == oracle.cell_tetra(dims, t, literal=False)
+        for q in (x, y):
+            if q >= 0:
+                assert set(oracle.tri_verts(dims, t)) <= set(oracle.tet_verts(dims, q))
+                seen[q] += 1
+    assert (seen == 4).all()  # every tet has 4 faces
+
+
+def _fields_for_gradient():
+    out = []
+    for seed in range(3):
+        out.append(fields.gaussian_mixture((6, 7, 8), 3, seed=seed, noise=0.05))
+    rng = np.random.default_rng(7)
+    out.append(rng.integers(0, 3, size=(5, 6, 7)).astype(np.float32))  # heavy ties
+    out.append(rng.integers(0, 2, size=(4, 5, 6)).astype(np.float64))
+    out.append(np.zeros((4, 4, 5), np.float32))  # all ties
+    return out
+
+
+def _check_matching(pv, pe, pf, pt, dims, oracle):
+    N, E, F, T = oracle.counts(*dims)
+    for v in np.nonzero(pv >= 0)[0]:
+        e = pv[v]
+        assert pe[e] == v and v in oracle.edge_verts(dims, e)
+    for e in range(E):
+        p = pe[e]
+        if p < 0:
+            continue
+        if p < N and pv[p] == e:
+            continue
+        assert pf[p] == e
+        assert set(oracle.edge_verts(dims, e)) <= set(oracle.tri_verts(dims, p))
+    for t in range(F):
+        p = pf[t]
+        if p < 0:
+            continue
+        if p < E and pe[p] == t:
+            continue
+        assert pt[p] == t
+        assert set(oracle.tri_verts(dims, t)) <= set(oracle.tet_verts(dims, p))
+
+
+@pytest.mark.parametrize("k", range(6))
+def test_gradient_literal_equals_direct_and_is_matching(oracle, k):
+    f = _fields_for_gradient()[k]
+    D, H, W = f.shape
+    A = oracle.gradient(f, literal=True)
+    B = oracle.gradient(f, literal=False)
+    for a, b in zip(A, B):
+        assert (a == b).all()
+    pv, pe, pf, pt = A
+    crit = [(pv == -1).sum(), (pe == -1).sum(), (pf == -1).sum(), (pt == -1).sum()]
+    # Euler characteristic: #min - #1-saddle + #2-saddle - #max = 1 for any discrete gradient
+    assert crit[0] - crit[1] + crit[2] - crit[3] == 1
+    _check_matching(pv, pe, pf, pt, (W, H, D), oracle)
+
+
+@pytest.mark.parametrize("shape", [(5, 6, 7), (8, 4, 9)])
+def test_gradient_linear_field_has_single_minimum(oracle, shape):
+    """A strictly increasing linear function has one critical cell: the minimum vertex."""
+    D, H, W = shape
+    z, y, x = np.meshgrid(np.arange(D), np.arange(H), np.arange(W), indexing="ij")
+    for f in (x + 3.0 * y + 11.0 * z, -(2.0 * x + y + 5.0 * z)):
+        pv, pe, pf, pt = oracle.gradient(f.astype(np.float64))
+        assert (pe == -1).sum() == 0 and (pf == -1).sum() == 0 and (pt == -1).sum() == 0
+        mins = np.nonzero(pv == -1)[0]
+        assert len(mins) == 1 and mins[0] == int(np.argmin(f.ravel()))
+
+
+def test_gradient_invariant_under_monotone_map(oracle):
+    rng = np.random.default_rng(3)
+    f = rng.integers(-5, 6, size=(6, 5, 7)).astype(np.float64)
+    g = 7.0 * f ** 3 + 2.0  # strictly increasing on integers, exact in double
+    for a, b in zip(oracle.gradient(f), oracle.gradient(g)):
+        assert (a == b).all()
+
+
+def test_gradient_float_and_double_agree_on_float_values(oracle):
+    f = fields.gaussian_mixture((7, 6, 8), 4, seed=5)
+    for a, b in zip(oracle.gradient(f), oracle.gradient(f.astype(np.float64))):
+        assert (a == b).all()
+
+
+def _crit_sets(grad):
+    return tuple(frozenset(np.nonzero(a == -1)[0].tolist()) for a in grad)
+
+
+@pytest.mark.parametrize("case", [
+    dict(shape=(12, 12, 12), rel=1e-2, fl={}),
+    dict(shape=(10, 12, 14), rel=3e-2, fl=dict(edit_type=1)),
+    dict(shape=(12, 10, 9), rel=2e-2, fl=dict(preserve_vpath=1, preserve_geometry=1)),
+    dict(shape=(9, 11, 10), rel=2e-2, fl=dict(preserve_vpath=1, preserve_geometry=0)),
+    dict(shape=(10, 10, 10), rel=2e-2, fl=dict(filtration=1)),
+    dict(shape=(10, 9, 11), rel=2e-2, fl=dict(preserve_type=1)),
+])
+def test_correction_end_state(oracle, case):
+    f = fields.gaussian_mixture(case["shape"], 5, seed=11, noise=0.02)
+    eps, rng = fields.eps_from_rel(f, case["rel"])
+    g = fields.quantize(f, eps)
+    r = oracle.correct(f, g, eps, rng, oracle.flags(**case["fl"]))
+    assert r["status"] == 0
+    assert r["counts"] == (0,) * 8
+    gp = r["g"].astype(np.float64)
+    assert (np.abs(gp - f) <= eps).all()
+    assert (r["g"] <= g).all()  # every edit moves toward f - eps
+    assert r["counter"].min() >= 1 and r["counter"].max() <= 6
+    assert _crit_sets(oracle.gradient(r["g"])) == _crit_sets(oracle.gradient(f))
+
+
+def test_snap_is_tightest_float_within_eps(oracle):
+    """The float32 snap is the smallest float32 x with f - x <= eps (oracle Arith<float>)."""
+    f = fields.gaussian_mixture((8, 8, 8), 3, seed=2)
+    eps, rng = fields.eps_from_rel(f, 1e-3)
+    g = fields.quantize(f, eps)
+    r = oracle.correct(f, g, eps, rng, oracle.flags(preserve_type=1))
+    snapped = r["counter"] == 6
+    assert snapped.any()
+    x = r["g"][snapped].astype(np.float64)
+    fs = f[snapped].astype(np.float64)
+    assert (fs - x <= eps).all()
+    below = np.nextafter(r["g"][snapped], np.float32(-np.inf)).astype(np.float64)
+    assert (fs - below > eps).all()
