@@ -1,0 +1,146 @@
+#!/usr/bin/env python3
+"""Benchmark: voxels/s to zero topology mismatch within eps, one MI355X.
+
+Workload (default): 256^3 float32 Gaussian mixture with 64 bumps + 0.01 N(0,1) noise, eps =
+1e-2 * range, g = SZ-style quantisation of f, corrected until critical cells and separatrices
+(path geometry) match f. A step = one full correction of the uncorrected g (reset from a device
+copy at the start of the step) with f and g resident in HBM. value = voxels * steps / wall time.
+
+The JSON line also carries the HBM roofline of the dominant kernel (dense mismatch detection),
+measured with HIP events inside the library on the stream the kernel runs on, and the CPU
+baseline: the oracle (C++/OpenMP restatement of the reference routines) timed on this host on a
+bounded crop of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--bumps", type=int, default=64)
+    ap.add_argument("--rel", type=float, default=1e-2)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--geometry", type=int, default=1, help="1: separatrix geometry, 0: critical cells only")
+    ap.add_argument("--cpu-sample", type=int, default=64, help="edge of the CPU-baseline crop, 0 = skip")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--dtype", default="f32", choices=("f32", "f64"),
+                    help="arithmetic of the correction (f64 = the reference's double precision)")
+    ap.add_argument("--max-sweeps", type=int, default=5000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (--gpus %d requested); the path would shard "
+              "by z-slabs with one-plane halos" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import dmtz_amd
+    from dmtz_amd import fields
+
+    n = args.size
+    shape = (n, n, n)
+    log("generating %s Gaussian mixture (%d bumps)" % (shape, args.bumps))
+    f = fields.gaussian_mixture(shape, args.bumps, seed=args.seed, noise=0.01)
+    eps, rng = fields.eps_from_rel(f, args.rel)
+    g0 = fields.quantize(f, eps)
+    flags = dmtz_amd.CorrectionFlags(preserve_saddles=1, preserve_vpath=1 if args.geometry else 0,
+                                     preserve_geometry=1 if args.geometry else 0, max_sweeps=args.max_sweeps)
+    dev = torch.device("cuda:0")
+    npdt = np.float64 if args.dtype == "f64" else np.float32
+    f_d = torch.from_numpy(f.astype(npdt)).to(dev)
+    g0_d = torch.from_numpy(g0.astype(npdt)).to(dev)
+    g_d = torch.empty_like(g0_d)
+    ctx = dmtz_amd.Context(0)
+
+    def step():
+        g_d.copy_(g0_d)
+        return dmtz_amd.correct_device(f_d, g_d, eps, rng, flags, ctx=ctx)
+
+    for w in range(args.warmup):
+        r = step()
+        log("warmup %d: %d sweeps, counts %s, %.3f s" % (w, r.sweeps, r.counts, r.phases["total"]))
+    ctx.kernel_stats(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = None
+    for s in range(args.steps):
+        res = step()
+        log("step %d: %d sweeps, counts %s" % (s, res.sweeps, res.counts))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    assert res.converged, res.counts
+    err = (g_d.double() - f_d.double()).abs().max().item()
+    assert err <= eps, (err, eps)
+    N = n * n * n
+    value = N * args.steps / elapsed
+    ks = ctx.kernel_stats()
+    launches = max(ks["launches"], 1.0)
+    avg_ms = ks["ms"] / launches
+    bytes_per_launch = ks["bytes"] / launches
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    roofline = dict(bound="hbm", kernel="k_detect (dense mismatch detection)", achieved=round(achieved, 2),
+                    peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None,
+                    bytes_per_launch=int(bytes_per_launch), avg_launch_ms=round(avg_ms, 4),
+                    launches=int(ks["launches"]))
+
+    cpu = None
+    if args.cpu_sample > 0:
+        from oracle import oracle as O
+
+        O.build()
+        threads = O.set_threads(min(args.cpu_threads, os.cpu_count() or 1))
+        s = min(args.cpu_sample, n)
+        fc = np.ascontiguousarray(f[:s, :s, :s])
+        gc = np.ascontiguousarray(g0[:s, :s, :s])
+        log("cpu baseline: oracle on a %d^3 crop, %d threads" % (s, threads))
+        c0 = time.perf_counter()
+        rc = O.correct(fc, gc, eps, rng, O.flags(preserve_saddles=1, preserve_vpath=flags.preserve_vpath,
+                                                  preserve_geometry=flags.preserve_geometry))
+        c1 = time.perf_counter()
+        cpu = dict(value=round(s ** 3 / (c1 - c0), 1), unit="voxels/s", cores=threads, kind="port",
+                   sample="%d^3 crop (corner) of the same field, same eps and flags: %d sweeps in %.2f s"
+                          % (s, rc["sweeps"], c1 - c0))
+
+    out = {
+        "metric": "voxels/s to zero topology mismatch within eps (plain GPU form, 1 GPU)",
+        "value": round(value, 1),
+        "unit": "voxels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic",
+        "config": {"workload": "gaussian-mixture %d^3 float32, %d bumps, eps=%g*range, %s" % (
+            n, args.bumps, args.rel, "critical cells + separatrix geometry" if args.geometry else "critical cells"),
+                   "size": [n, n, n], "bumps": args.bumps, "rel_eps": args.rel, "seed": args.seed,
+                   "sweeps": res.sweeps, "form": "plain"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -69,6 +69,7 @@ def lib():
     L.dmtz_trajectory.argtypes = [P, P, I64]
     L.dmtz_trajectory.restype = I64
     L.dmtz_phase_times.argtypes = [P, P]
+    L.dmtz_kernel_stats.argtypes = [P, P, ctypes.c_int]
     _LIB = L
     return L
 
@@ -78,7 +79,7 @@ EXPORTS = (
     "dmtz_ctx_create", "dmtz_ctx_destroy", "dmtz_last_error",
     "dmtz_eps_from_rel_f32", "dmtz_eps_from_rel_f64",
     "dmtz_correct_f32", "dmtz_correct_f64", "dmtz_correct_dev_f32", "dmtz_correct_dev_f64",
-    "dmtz_trajectory", "dmtz_phase_times",
+    "dmtz_trajectory", "dmtz_phase_times", "dmtz_kernel_stats",
     "dmtz_gradient_f32", "dmtz_gradient_f64",
     "dmtz_targets_f32", "dmtz_targets_f64",
     "dmtz_paths_f32", "dmtz_paths_f64",

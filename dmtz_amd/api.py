@@ -94,6 +94,12 @@ class Context:
         self._lib.dmtz_trajectory(self._h, out.ctypes.data_as(ctypes.c_void_p), n)
         return out[: n * 8].reshape(n, 8)
 
+    def kernel_stats(self, reset=False):
+        """Dense detection kernel: dict(launches, ms, bytes, voxels) measured with HIP events."""
+        out = np.zeros(4, np.float64)
+        self._lib.dmtz_kernel_stats(self._h, out.ctypes.data_as(ctypes.c_void_p), 1 if reset else 0)
+        return dict(zip(("launches", "ms", "bytes", "voxels"), out.tolist()))
+
     def phases(self):
         out = np.zeros(6, np.float64)
         self._lib.dmtz_phase_times(self._h, out.ctypes.data_as(ctypes.c_void_p))

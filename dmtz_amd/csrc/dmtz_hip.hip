@@ -21,6 +21,7 @@
 
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -40,6 +41,7 @@ enum : int {
     K_DS = 5, K_AS = 6,                         // separatrix mismatch lists
     K_FC0 = 8,                                  // f-critical lists per type: 8..11
     K_WALK = 12,                                // capped walks
+    K_CHG = 13, K_DIRTY = 14,                   // changed vertices, dirty stars
     K_NCTR = 16
 };
 
@@ -113,32 +115,48 @@ __device__ __forceinline__ void emit_records(i64 v, uint64_t lo, uint32_t hi, co
 // ------------------------------------------------------------------------------------------------
 // Kernels
 // ------------------------------------------------------------------------------------------------
+// Star codes of every vertex (dense) or of the listed vertices.
 template <class T>
-__global__ __launch_bounds__(BLOCK) void k_detect(const T* f, const T* g, Dims d, Counts c, Lists4 out, ull* ctr) {
+__global__ __launch_bounds__(BLOCK) void k_code(const T* fld, Dims d, Counts c, StarCode* code) {
+    const i64 v = (i64)blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= c.N) return;
+    LowerStar ls;
+    star_of(fld, d, v, ls);
+    encode_star(ls, code[v]);
+}
+
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_code_list(const T* fld, Dims d, Counts c, StarCode* code,
+                                                     const uint32_t* list, ull n) {
+    const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const i64 v = list[i];
+    LowerStar ls;
+    star_of(fld, d, v, ls);
+    encode_star(ls, code[v]);
+}
+
+// Mismatch detection from the star codes of f and g.
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_detect(const T* f, const StarCode* cf, const T* g, const StarCode* cg,
+                                                  Dims d, Counts c, Lists4 out, ull* ctr) {
     const i64 v = (i64)blockIdx.x * BLOCK + threadIdx.x;
     uint64_t lo = 0;
     uint32_t hi = 0;
     if (v < c.N) {
-        FieldGrad<T> F(f, d, c), G(g, d, c);
-        LowerStar lf, lg;
-        F.star_inline(v, lf);
-        G.star_inline(v, lg);
-        vertex_mismatch(F, G, v, lf, lg, lo, hi);
+        FieldGrad<T> F(f, cf, d, c), G(g, cg, d, c);
+        vertex_mismatch(F, G, v, lo, hi);
     }
     emit_records(v, lo, hi, out, ctr);
 }
 
 // f-critical cells owned by each vertex in f; type_mask selects which types are listed.
-template <class T>
-__global__ __launch_bounds__(BLOCK) void k_fcrit(const T* f, Dims d, Counts c, Lists4 out, ull* ctr, int type_mask) {
+__global__ __launch_bounds__(BLOCK) void k_fcrit(const StarCode* cf, Counts c, Lists4 out, ull* ctr, int type_mask) {
     const i64 v = (i64)blockIdx.x * BLOCK + threadIdx.x;
-    uint64_t lo = 0;
-    uint32_t hi = 0;
+    uint64_t lo = 0, ilo;
+    uint32_t hi = 0, ihi;
     if (v < c.N) {
-        FieldGrad<T> F(f, d, c);
-        LowerStar lf;
-        F.star_inline(v, lf);
-        crit_mask(lf, lo, hi);
+        code_masks(cf[v], lo, hi, ilo, ihi);
         uint64_t keep = 0;
         if (type_mask & 1) keep |= 1ull;
         if (type_mask & 2) keep |= 0x3FFFull << 1;
@@ -165,16 +183,17 @@ struct TargetSink {
 };
 
 template <class T>
-__global__ __launch_bounds__(BLOCK) void k_fix_cp(int type, const uint64_t* recs, ull n, const T* f, const T* g, Dims d,
-                                                  Counts c, const uint8_t* counter, double eps, double range,
-                                                  int edit_type, TargetSink sink) {
+__global__ __launch_bounds__(BLOCK) void k_fix_cp(int type, const uint64_t* recs, ull n, const T* f, const StarCode* cf,
+                                                  const T* g, const StarCode* cg, Dims d, Counts c,
+                                                  const uint8_t* counter, double eps, double range, int edit_type,
+                                                  TargetSink sink) {
     const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     const uint64_t r = recs[i];
     const i64 v = (i64)(r >> 7);
     const int lc = (int)(r & 127u);
     const i64 id = local_cell_id(d, c, v, lc);
-    Repair<T, TargetSink> R(f, g, d, c, counter, eps, range, edit_type, sink);
+    Repair<T, TargetSink> R(f, cf, g, cg, d, c, counter, eps, range, edit_type, sink);
     switch (type) {
         case 0: R.fix_minimum(id); break;
         case 1: R.fix_saddle(id); break;
@@ -185,27 +204,60 @@ __global__ __launch_bounds__(BLOCK) void k_fix_cp(int type, const uint64_t* recs
 
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_apply(const uint32_t* list, ull n, const T* f, T* g, uint8_t* counter,
-                                                 double eps, double range, int preserve_type, int edit_type) {
+                                                 double eps, double range, int preserve_type, int edit_type,
+                                                 uint32_t* changed, ull* chg_ctr) {
     const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const i64 v = list[i];
-    const T fv = f[v];
-    T gv = g[v];
-    int k = counter[v];
-    const double dl = edit_type == 1 ? halving_delta(eps, k) : EditArith<T>::delta0(fv, gv, eps);
-    if (preserve_type == 0) {
-        if (fabs_d(dl) > 1e-15 * range && k < 5 && EditArith<T>::step(fv, gv, dl, eps)) {
-            k += 1;
+    bool ch = false;
+    i64 v = 0;
+    if (i < n) {
+        v = list[i];
+        const T fv = f[v];
+        const T g0 = g[v];
+        T gv = g0;
+        int k = counter[v];
+        const double dl = edit_type == 1 ? halving_delta(eps, k) : EditArith<T>::delta0(fv, gv, eps);
+        if (preserve_type == 0) {
+            if (fabs_d(dl) > 1e-15 * range && k < 5 && EditArith<T>::step(fv, gv, dl, eps)) {
+                k += 1;
+            } else {
+                k = 6;
+                gv = EditArith<T>::snap(fv, eps);
+            }
         } else {
             k = 6;
             gv = EditArith<T>::snap(fv, eps);
         }
-    } else {
-        k = 6;
-        gv = EditArith<T>::snap(fv, eps);
+        g[v] = gv;
+        counter[v] = (uint8_t)k;
+        ch = !(gv == g0);  // the order (hence every lower star) only sees value comparisons
     }
-    g[v] = gv;
-    counter[v] = (uint8_t)k;
+    ull at = wave_alloc(chg_ctr, ch ? 1 : 0);
+    if (ch) changed[at] = (uint32_t)v;
+}
+
+// Closed stars of the changed vertices: their lower stars must be re-evaluated. One thread per
+// (changed vertex, star slot); each vertex enters the dirty list once per epoch.
+__global__ __launch_bounds__(BLOCK) void k_dirty(const uint32_t* changed, ull n, Dims d, uint32_t* stamp,
+                                                 uint32_t epoch, uint32_t* dirty, ull* dirty_ctr) {
+    const ull j = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    bool fresh = false;
+    i64 w = 0;
+    if (j < 15 * n) {
+        const i64 v = changed[j / 15];
+        const int slot = (int)(j % 15);
+        i64 x, y, z;
+        vxyz(d, v, x, y, z);
+        bool ok = true;
+        w = v;
+        if (slot < 14) {
+            i64 nx = x + DIR[slot][0], ny = y + DIR[slot][1], nz = z + DIR[slot][2];
+            ok = inb(d, nx, ny, nz);
+            w = ok ? vidx(d, nx, ny, nz) : 0;
+        }
+        if (ok) fresh = atomicExch(stamp + w, epoch) != epoch;
+    }
+    ull at = wave_alloc(dirty_ctr, fresh ? 1 : 0);
+    if (fresh) dirty[at] = (uint32_t)w;
 }
 
 // Filtration main_3d.cu:2405-2490: every vertex of an f-critical cell is pinned to f - eps.
@@ -260,15 +312,15 @@ struct NoSink {
 
 // get_false_ds_vpath main_3d.cu:4837-4959 over the f-critical edges.
 template <class T>
-__global__ __launch_bounds__(BLOCK) void k_ds_geo(const uint64_t* fce, ull n, const T* f, const T* g, Dims d, Counts c,
-                                                  PathList out) {
+__global__ __launch_bounds__(BLOCK) void k_ds_geo(const uint64_t* fce, ull n, const T* f, const StarCode* cf, const T* g,
+                                                  const StarCode* cg, Dims d, Counts c, PathList out) {
     const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
     bool has = false;
     i64 e = -1, rec = -1;
     if (i < n) {
         const uint64_t r = fce[i];
         e = local_cell_id(d, c, (i64)(r >> 7), (int)(r & 127u));
-        Repair<T, NoSink> R(f, g, d, c, nullptr, 0, 0, 0, NoSink{});
+        Repair<T, NoSink> R(f, cf, g, cg, d, c, nullptr, 0, 0, 0, NoSink{});
         has = R.ds_geometry(e, rec);
     }
     out.push_wave(has, e, rec);
@@ -280,14 +332,15 @@ __global__ __launch_bounds__(BLOCK) void k_ds_geo(const uint64_t* fce, ull n, co
 // divergent tet.
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_as_branch(const uint64_t* fcf, ull n, const int64_t* maxlab, const T* f,
-                                                     const T* g, Dims d, Counts c, int64_t* res, ull* walk_ctr) {
+                                                     const StarCode* cf, const T* g, const StarCode* cg, Dims d,
+                                                     Counts c, int64_t* res, ull* walk_ctr) {
     const ull j = (ull)blockIdx.x * BLOCK + threadIdx.x;
     if (j >= 2 * n) return;
     const ull i = j >> 1;
     const int k = (int)(j & 1);
     const uint64_t r = fcf[i];
     const i64 t = local_cell_id(d, c, (i64)(r >> 7), (int)(r & 127u));
-    Repair<T, NoSink> R(f, g, d, c, nullptr, 0, 0, 0, NoSink{});
+    Repair<T, NoSink> R(f, cf, g, cg, d, c, nullptr, 0, 0, 0, NoSink{});
     i64 q0, q1;
     R.cofacets(t, q0, q1);
     const i64 q = k ? q1 : q0;
@@ -319,9 +372,10 @@ __global__ __launch_bounds__(BLOCK) void k_as_combine(const int64_t* res, ull n,
 // are indexed by face (resolution of the shadowed index at main_3d.cu:4750).
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_labels(const uint64_t* fce, ull ne, const uint64_t* fcf, ull nf, const T* f,
-                                                  Dims d, Counts c, int64_t* minlab, int64_t* maxlab, ull* walk_ctr) {
+                                                  const StarCode* cf, Dims d, Counts c, int64_t* minlab,
+                                                  int64_t* maxlab, ull* walk_ctr) {
     const ull j = (ull)blockIdx.x * BLOCK + threadIdx.x;
-    Repair<T, NoSink> R(f, f, d, c, nullptr, 0, 0, 0, NoSink{});
+    Repair<T, NoSink> R(f, cf, f, cf, d, c, nullptr, 0, 0, 0, NoSink{});
     bool ovf = false;
     if (j < 2 * ne) {
         const uint64_t r = fce[j >> 1];
@@ -345,13 +399,14 @@ __global__ __launch_bounds__(BLOCK) void k_labels(const uint64_t* fce, ull ne, c
 // walks g's descending path; res[4*i + 2*k] = reached g minimum, res[4*i + 2*k + 1] = first
 // vertex on the walk whose f and g pairings differ.
 template <class T>
-__global__ __launch_bounds__(BLOCK) void k_ds_branch(const uint64_t* fce, ull n, const T* f, const T* g, Dims d,
-                                                     Counts c, int64_t* res, ull* walk_ctr) {
+__global__ __launch_bounds__(BLOCK) void k_ds_branch(const uint64_t* fce, ull n, const T* f, const StarCode* cf,
+                                                     const T* g, const StarCode* cg, Dims d, Counts c, int64_t* res,
+                                                     ull* walk_ctr) {
     const ull j = (ull)blockIdx.x * BLOCK + threadIdx.x;
     if (j >= 2 * n) return;
     const uint64_t r = fce[j >> 1];
     const i64 e = local_cell_id(d, c, (i64)(r >> 7), (int)(r & 127u));
-    Repair<T, NoSink> R(f, g, d, c, nullptr, 0, 0, 0, NoSink{});
+    Repair<T, NoSink> R(f, cf, g, cg, d, c, nullptr, 0, 0, 0, NoSink{});
     i64 a, b;
     edge_verts(d, c, e, a, b);
     i64 trouble = -1;
@@ -382,20 +437,22 @@ __global__ __launch_bounds__(BLOCK) void k_ds_combine(const uint64_t* fce, ull n
 // fix_as_vpath / fix_ds_vpath main_3d.cu:5135-5295
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_fix_paths(const int64_t* as_rec, ull n_as, const int64_t* ds_rec, ull n_ds,
-                                                     const T* f, const T* g, Dims d, Counts c, const uint8_t* counter,
-                                                     double eps, double range, int edit_type, TargetSink sink) {
+                                                     const T* f, const StarCode* cf, const T* g, const StarCode* cg,
+                                                     Dims d, Counts c, const uint8_t* counter, double eps, double range,
+                                                     int edit_type, TargetSink sink) {
     const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
-    Repair<T, TargetSink> R(f, g, d, c, counter, eps, range, edit_type, sink);
+    Repair<T, TargetSink> R(f, cf, g, cg, d, c, counter, eps, range, edit_type, sink);
     if (i < n_as) R.fix_as_vpath(as_rec[i]);
     else if (i < n_as + n_ds) R.fix_ds_vpath(ds_rec[i - n_as]);
 }
 
 // Dense pairing arrays for tests (dmtz_gradient_*): one thread per cell of each dimension.
 template <class T>
-__global__ __launch_bounds__(BLOCK) void k_pairing(const T* fld, Dims d, Counts c, int dim, ull n, int64_t* out) {
+__global__ __launch_bounds__(BLOCK) void k_pairing(const T* fld, const StarCode* code, Dims d, Counts c, int dim, ull n,
+                                                   int64_t* out) {
     const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
-    FieldGrad<T> X(fld, d, c);
+    FieldGrad<T> X(fld, code, d, c);
     switch (dim) {
         case 0: out[i] = X.pv((i64)i); break;
         case 1: out[i] = X.pe((i64)i); break;
@@ -420,19 +477,30 @@ struct dmtz_ctx {
     void* d_g = nullptr;
     size_t field_bytes = 0;
     uint8_t* d_counter = nullptr;
-    uint32_t* d_stamp = nullptr;
-    uint32_t* d_targets = nullptr;
-    uint64_t* d_lists = nullptr;  // 4 record lists + 4 f-critical lists
-    int64_t* d_path = nullptr;    // ds id/rec, as id/rec
+    uint32_t* d_stamp = nullptr;    // edit-set dedup stamps
+    uint32_t* d_dstamp = nullptr;   // dirty-star dedup stamps
+    uint32_t* d_targets = nullptr;  // edit set
+    uint32_t* d_changed = nullptr;  // edited vertices whose value changed
+    uint32_t* d_dirty = nullptr;    // vertices whose g star code must be re-evaluated
+    StarCode* d_code_f = nullptr;   // star codes of f and g
+    StarCode* d_code_g = nullptr;
+    uint64_t* d_rec = nullptr;    // 4 mismatch record lists of rec_cap entries
+    uint64_t* d_fc = nullptr;     // 4 f-critical lists of fc_cap entries
+    int64_t* d_path = nullptr;    // ds id/rec, as id/rec (fc_cap each)
     int64_t* d_labels = nullptr;  // min labels (2 per f-critical edge), max labels (2 per face)
     int64_t* d_walk = nullptr;    // per-branch walk results (4 per start cell)
     ull* d_ctr = nullptr;
     ull* h_ctr = nullptr;
     i64 cap_n = 0;   // vertices the buffers are sized for
-    ull list_cap = 0;
+    ull rec_cap = 0;
+    ull fc_cap = 0;
     uint32_t epoch = 0;
     std::vector<int64_t> traj;
     double phase[6] = {0, 0, 0, 0, 0, 0};
+    // live timing of the dense detection kernel on the context stream (HIP events)
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double det_launches = 0, det_ms = 0, det_bytes = 0, det_voxels = 0;
+    double dirty_total = 0;  // star codes re-evaluated incrementally in the last correction
 };
 
 namespace {
@@ -442,19 +510,34 @@ void free_bufs(dmtz_ctx* ctx) {
     (void)hipFree(ctx->d_g);
     (void)hipFree(ctx->d_counter);
     (void)hipFree(ctx->d_stamp);
+    (void)hipFree(ctx->d_dstamp);
     (void)hipFree(ctx->d_targets);
-    (void)hipFree(ctx->d_lists);
+    (void)hipFree(ctx->d_changed);
+    (void)hipFree(ctx->d_dirty);
+    (void)hipFree(ctx->d_code_f);
+    (void)hipFree(ctx->d_code_g);
+    ctx->d_f = ctx->d_g = nullptr;
+    ctx->d_counter = nullptr;
+    ctx->d_stamp = ctx->d_dstamp = ctx->d_targets = ctx->d_changed = ctx->d_dirty = nullptr;
+    ctx->d_code_f = ctx->d_code_g = nullptr;
+    ctx->field_bytes = 0;
+    ctx->cap_n = 0;
+}
+
+void free_rec(dmtz_ctx* ctx) {
+    (void)hipFree(ctx->d_rec);
+    ctx->d_rec = nullptr;
+    ctx->rec_cap = 0;
+}
+
+void free_fc(dmtz_ctx* ctx) {
+    (void)hipFree(ctx->d_fc);
     (void)hipFree(ctx->d_path);
     (void)hipFree(ctx->d_labels);
     (void)hipFree(ctx->d_walk);
-    ctx->d_walk = nullptr;
-    ctx->d_f = ctx->d_g = nullptr;
-    ctx->d_counter = nullptr;
-    ctx->d_stamp = ctx->d_targets = nullptr;
-    ctx->d_lists = nullptr;
-    ctx->d_path = ctx->d_labels = nullptr;
-    ctx->field_bytes = 0;
-    ctx->cap_n = 0;
+    ctx->d_fc = nullptr;
+    ctx->d_path = ctx->d_labels = ctx->d_walk = nullptr;
+    ctx->fc_cap = 0;
 }
 
 #define CK(x)                                                                         \
@@ -469,17 +552,18 @@ void free_bufs(dmtz_ctx* ctx) {
 int ensure(dmtz_ctx* ctx, i64 N, size_t elem, bool want_fields) {
     if (N > ctx->cap_n) {
         free_bufs(ctx);
-        ull cap = (ull)N + 4096;
-        CK(hipMalloc(&ctx->d_counter, (size_t)N));
-        CK(hipMalloc(&ctx->d_stamp, sizeof(uint32_t) * (size_t)N));
-        CK(hipMemsetAsync(ctx->d_stamp, 0, sizeof(uint32_t) * (size_t)N, ctx->stream));
-        CK(hipMalloc(&ctx->d_targets, sizeof(uint32_t) * (size_t)N));
-        CK(hipMalloc(&ctx->d_lists, sizeof(uint64_t) * 8 * cap));
-        CK(hipMalloc(&ctx->d_path, sizeof(int64_t) * 4 * cap));
-        CK(hipMalloc(&ctx->d_labels, sizeof(int64_t) * 4 * cap));
-        CK(hipMalloc(&ctx->d_walk, sizeof(int64_t) * 4 * cap));
+        const size_t n = (size_t)N;
+        CK(hipMalloc(&ctx->d_counter, n));
+        CK(hipMalloc(&ctx->d_stamp, sizeof(uint32_t) * n));
+        CK(hipMalloc(&ctx->d_dstamp, sizeof(uint32_t) * n));
+        CK(hipMemsetAsync(ctx->d_stamp, 0, sizeof(uint32_t) * n, ctx->stream));
+        CK(hipMemsetAsync(ctx->d_dstamp, 0, sizeof(uint32_t) * n, ctx->stream));
+        CK(hipMalloc(&ctx->d_targets, sizeof(uint32_t) * n));
+        CK(hipMalloc(&ctx->d_changed, sizeof(uint32_t) * n));
+        CK(hipMalloc(&ctx->d_dirty, sizeof(uint32_t) * n));
+        CK(hipMalloc(&ctx->d_code_f, sizeof(StarCode) * n));
+        CK(hipMalloc(&ctx->d_code_g, sizeof(StarCode) * n));
         ctx->cap_n = N;
-        ctx->list_cap = cap;
         ctx->epoch = 0;
     }
     if (want_fields && ctx->field_bytes < (size_t)N * elem) {
@@ -490,6 +574,29 @@ int ensure(dmtz_ctx* ctx, i64 N, size_t elem, bool want_fields) {
         CK(hipMalloc(&ctx->d_g, (size_t)N * elem));
         ctx->field_bytes = (size_t)N * elem;
     }
+    return 0;
+}
+
+// Record lists grow on demand: detection is recomputed from f and g, so an overflowing launch is
+// simply repeated after the resize.
+int grow_rec(dmtz_ctx* ctx, ull need) {
+    if (need <= ctx->rec_cap) return 0;
+    free_rec(ctx);
+    ull cap = need + need / 4 + 4096;
+    CK(hipMalloc(&ctx->d_rec, sizeof(uint64_t) * 4 * cap));
+    ctx->rec_cap = cap;
+    return 0;
+}
+
+int grow_fc(dmtz_ctx* ctx, ull need) {
+    if (need <= ctx->fc_cap) return 0;
+    free_fc(ctx);
+    ull cap = need + need / 4 + 4096;
+    CK(hipMalloc(&ctx->d_fc, sizeof(uint64_t) * 4 * cap));
+    CK(hipMalloc(&ctx->d_path, sizeof(int64_t) * 4 * cap));
+    CK(hipMalloc(&ctx->d_labels, sizeof(int64_t) * 4 * cap));
+    CK(hipMalloc(&ctx->d_walk, sizeof(int64_t) * 4 * cap));
+    ctx->fc_cap = cap;
     return 0;
 }
 
@@ -508,15 +615,21 @@ int check_dims(dmtz_ctx* ctx, i64 nx, i64 ny, i64 nz) {
 
 Lists4 rec_lists(dmtz_ctx* ctx) {
     Lists4 L;
-    for (int k = 0; k < 4; k++) L.l[k] = ctx->d_lists + (size_t)k * ctx->list_cap;
-    L.cap = ctx->list_cap;
+    for (int k = 0; k < 4; k++) L.l[k] = ctx->d_rec + (size_t)k * ctx->rec_cap;
+    L.cap = ctx->rec_cap;
     return L;
 }
 Lists4 fcrit_lists(dmtz_ctx* ctx) {
     Lists4 L;
-    for (int k = 0; k < 4; k++) L.l[k] = ctx->d_lists + (size_t)(4 + k) * ctx->list_cap;
-    L.cap = ctx->list_cap;
+    for (int k = 0; k < 4; k++) L.l[k] = ctx->d_fc + (size_t)k * ctx->fc_cap;
+    L.cap = ctx->fc_cap;
     return L;
+}
+
+ull max4(const ull* x) {
+    ull m = x[0];
+    for (int k = 1; k < 4; k++) m = x[k] > m ? x[k] : m;
+    return m;
 }
 
 int sync_ctr(dmtz_ctx* ctx) {
@@ -526,12 +639,15 @@ int sync_ctr(dmtz_ctx* ctx) {
 }
 
 int overflow_check(dmtz_ctx* ctx) {
-    const int checked[10] = {K_MIN, K_S1, K_S2, K_MAX, K_DS, K_AS, K_FC0, K_FC0 + 1, K_FC0 + 2, K_FC0 + 3};
-    for (int k : checked)
-        if (ctx->h_ctr[k] > ctx->list_cap) {
+    for (int k = 0; k < 4; k++)
+        if (ctx->h_ctr[K_MIN + k] > ctx->rec_cap || ctx->h_ctr[K_FC0 + k] > ctx->fc_cap) {
             ctx->err = "mismatch list capacity exceeded";
             return DMTZ_E_CAPACITY;
         }
+    if (ctx->h_ctr[K_DS] > ctx->fc_cap || ctx->h_ctr[K_AS] > ctx->fc_cap) {
+        ctx->err = "separatrix list capacity exceeded";
+        return DMTZ_E_CAPACITY;
+    }
     if (ctx->h_ctr[K_TGT] > (ull)ctx->cap_n) {
         ctx->err = "edit set capacity exceeded";
         return DMTZ_E_CAPACITY;
@@ -554,12 +670,39 @@ struct Driver {
 
     // Detection of the four critical-cell mismatch kinds into the record lists.
     int detect() {
+        if (ctx->rec_cap == 0) {
+            int rc0 = grow_rec(ctx, (ull)(c.N / 4));
+            if (rc0) return rc0;
+        }
+        for (;;) {
+            int rc = detect_once();
+            if (rc) return rc;
+            ull m = max4(ctx->h_ctr + K_MIN);
+            if (m <= ctx->rec_cap) break;
+            rc = grow_rec(ctx, m);
+            if (rc) return rc;
+        }
+        return overflow_check(ctx);
+    }
+
+    int detect_once() {
         CK(hipMemsetAsync(ctx->d_ctr + K_MIN, 0, sizeof(ull) * 4, ctx->stream));
-        k_detect<T><<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(f, g, d, c, rec_lists(ctx), ctx->d_ctr + K_MIN);
+        CK(hipEventRecord(ctx->ev0, ctx->stream));
+        k_detect<T><<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(f, ctx->d_code_f, g, ctx->d_code_g, d, c, rec_lists(ctx),
+                                                              ctx->d_ctr + K_MIN);
         CK(hipGetLastError());
+        CK(hipEventRecord(ctx->ev1, ctx->stream));
         int rc = sync_ctr(ctx);
         if (rc) return rc;
-        return overflow_check(ctx);
+        float ms = 0;
+        CK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        // algorithmic bytes: f and g read once + one 8-byte record per mismatch emitted
+        ull m = ctx->h_ctr[0] + ctx->h_ctr[1] + ctx->h_ctr[2] + ctx->h_ctr[3];
+        ctx->det_launches += 1;
+        ctx->det_ms += ms;
+        ctx->det_bytes += (double)c.N * 2.0 * sizeof(T) + 8.0 * (double)m;
+        ctx->det_voxels += (double)c.N;
+        return 0;
     }
 
     TargetSink sink() {
@@ -576,6 +719,7 @@ struct Driver {
         ctx->epoch++;
         if (ctx->epoch == 0) {  // stamp wrap: clear and restart
             CK(hipMemsetAsync(ctx->d_stamp, 0, sizeof(uint32_t) * (size_t)c.N, ctx->stream));
+            CK(hipMemsetAsync(ctx->d_dstamp, 0, sizeof(uint32_t) * (size_t)c.N, ctx->stream));
             ctx->epoch = 1;
         }
         CK(hipMemsetAsync(ctx->d_ctr + K_TGT, 0, sizeof(ull), ctx->stream));
@@ -590,31 +734,57 @@ struct Driver {
         for (int ty = 0; ty < 4; ty++) {
             if (counts[ty] == 0) continue;
             if ((ty == 1 || ty == 2) && fl.preserve_saddles != 1) continue;
-            k_fix_cp<T><<<nblocks(counts[ty]), BLOCK, 0, ctx->stream>>>(ty, L.l[ty], counts[ty], f, g, d, c,
-                                                                         ctx->d_counter, eps, range, fl.edit_type,
-                                                                         sink());
+            k_fix_cp<T><<<nblocks(counts[ty]), BLOCK, 0, ctx->stream>>>(ty, L.l[ty], counts[ty], f, ctx->d_code_f, g,
+                                                                         ctx->d_code_g, d, c, ctx->d_counter, eps,
+                                                                         range, fl.edit_type, sink());
             CK(hipGetLastError());
         }
         return apply();
     }
 
+    // applyDeltaBuffer1 (main_3d.cu:1122-1164) over the edit set, then re-evaluation of the g star
+    // codes of the closed stars of the vertices whose value changed (the g gradient of
+    // main_3d.cu:6864-6874 restricted to where it can change).
     int apply() {
         int rc = sync_ctr(ctx);
         if (rc) return rc;
         rc = overflow_check(ctx);
         if (rc) return rc;
         ull nt = ctx->h_ctr[K_TGT];
-        if (nt) {
-            k_apply<T><<<nblocks(nt), BLOCK, 0, ctx->stream>>>(ctx->d_targets, nt, f, g, ctx->d_counter, eps, range,
-                                                               fl.preserve_type, fl.edit_type);
+        if (!nt) return 0;
+        CK(hipMemsetAsync(ctx->d_ctr + K_CHG, 0, sizeof(ull) * 2, ctx->stream));
+        k_apply<T><<<nblocks(nt), BLOCK, 0, ctx->stream>>>(ctx->d_targets, nt, f, g, ctx->d_counter, eps, range,
+                                                           fl.preserve_type, fl.edit_type, ctx->d_changed,
+                                                           ctx->d_ctr + K_CHG);
+        CK(hipGetLastError());
+        rc = sync_ctr(ctx);
+        if (rc) return rc;
+        ull nc = ctx->h_ctr[K_CHG];
+        if (!nc) return 0;
+        k_dirty<<<nblocks(15 * nc), BLOCK, 0, ctx->stream>>>(ctx->d_changed, nc, d, ctx->d_dstamp, ctx->epoch,
+                                                            ctx->d_dirty, ctx->d_ctr + K_DIRTY);
+        CK(hipGetLastError());
+        rc = sync_ctr(ctx);
+        if (rc) return rc;
+        ull nd = ctx->h_ctr[K_DIRTY];
+        if (nd) {
+            k_code_list<T><<<nblocks(nd), BLOCK, 0, ctx->stream>>>(g, d, c, ctx->d_code_g, ctx->d_dirty, nd);
             CK(hipGetLastError());
         }
+        ctx->dirty_total += (double)nd;
+        return 0;
+    }
+
+    // Star codes of every vertex of a field.
+    int codes(const T* fld, StarCode* out) {
+        k_code<T><<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(fld, d, c, out);
+        CK(hipGetLastError());
         return 0;
     }
 
     PathList path_list(int which) {
         PathList p;
-        ull cap = ctx->list_cap;
+        ull cap = ctx->fc_cap;
         p.id = ctx->d_path + (size_t)(2 * which) * cap;
         p.rec = ctx->d_path + (size_t)(2 * which + 1) * cap;
         p.ctr = ctx->d_ctr + (which == 0 ? K_DS : K_AS);
@@ -624,20 +794,36 @@ struct Driver {
 
     ull nfce = 0, nfcf = 0;
 
+    // f-critical cells of the types in type_mask into the f-critical lists (grown on demand).
+    int fcrit(int type_mask) {
+        if (ctx->fc_cap == 0) {
+            int rc0 = grow_fc(ctx, (ull)(c.N / 16));
+            if (rc0) return rc0;
+        }
+        for (;;) {
+            Lists4 FL = fcrit_lists(ctx);
+            CK(hipMemsetAsync(ctx->d_ctr + K_FC0, 0, sizeof(ull) * 4, ctx->stream));
+            k_fcrit<<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(ctx->d_code_f, c, FL, ctx->d_ctr + K_FC0, type_mask);
+            CK(hipGetLastError());
+            int rc = sync_ctr(ctx);
+            if (rc) return rc;
+            ull m = max4(ctx->h_ctr + K_FC0);
+            if (m <= ctx->fc_cap) break;
+            rc = grow_fc(ctx, m);
+            if (rc) return rc;
+        }
+        return 0;
+    }
+
     int prepare_paths() {
+        int rc = fcrit(2 | 4);
+        if (rc) return rc;
         Lists4 FL = fcrit_lists(ctx);
-        CK(hipMemsetAsync(ctx->d_ctr + K_FC0, 0, sizeof(ull) * 4, ctx->stream));
-        k_fcrit<T><<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(f, d, c, FL, ctx->d_ctr + K_FC0, 2 | 4);
-        CK(hipGetLastError());
-        int rc = sync_ctr(ctx);
-        if (rc) return rc;
-        rc = overflow_check(ctx);
-        if (rc) return rc;
         nfce = ctx->h_ctr[K_FC0 + 1];
         nfcf = ctx->h_ctr[K_FC0 + 2];
         if (fl.preserve_vpath == 1 && fl.preserve_geometry == 0 && nfce + nfcf) {
             k_labels<T><<<nblocks(2 * (nfce + nfcf)), BLOCK, 0, ctx->stream>>>(
-                FL.l[1], nfce, FL.l[2], nfcf, f, d, c, ctx->d_labels, ctx->d_labels + 2 * ctx->list_cap,
+                FL.l[1], nfce, FL.l[2], nfcf, f, ctx->d_code_f, d, c, ctx->d_labels, ctx->d_labels + 2 * ctx->fc_cap,
                 ctx->d_ctr + K_WALK);
             CK(hipGetLastError());
         }
@@ -650,8 +836,9 @@ struct Driver {
         Lists4 FL = fcrit_lists(ctx);
         const bool conn = fl.preserve_geometry != 1;
         if (nfcf) {
-            const int64_t* lab = conn ? ctx->d_labels + 2 * ctx->list_cap : nullptr;
-            k_as_branch<T><<<nblocks(2 * nfcf), BLOCK, 0, ctx->stream>>>(FL.l[2], nfcf, lab, f, g, d, c, ctx->d_walk,
+            const int64_t* lab = conn ? ctx->d_labels + 2 * ctx->fc_cap : nullptr;
+            k_as_branch<T><<<nblocks(2 * nfcf), BLOCK, 0, ctx->stream>>>(FL.l[2], nfcf, lab, f, ctx->d_code_f, g,
+                                                                         ctx->d_code_g, d, c, ctx->d_walk,
                                                                          ctx->d_ctr + K_WALK);
             CK(hipGetLastError());
             k_as_combine<<<nblocks(nfcf), BLOCK, 0, ctx->stream>>>(ctx->d_walk, nfcf, path_list(1));
@@ -659,10 +846,12 @@ struct Driver {
         }
         if (nfce) {
             if (!conn) {
-                k_ds_geo<T><<<nblocks(nfce), BLOCK, 0, ctx->stream>>>(FL.l[1], nfce, f, g, d, c, path_list(0));
+                k_ds_geo<T><<<nblocks(nfce), BLOCK, 0, ctx->stream>>>(FL.l[1], nfce, f, ctx->d_code_f, g, ctx->d_code_g,
+                                                                      d, c, path_list(0));
                 CK(hipGetLastError());
             } else {
-                k_ds_branch<T><<<nblocks(2 * nfce), BLOCK, 0, ctx->stream>>>(FL.l[1], nfce, f, g, d, c, ctx->d_walk,
+                k_ds_branch<T><<<nblocks(2 * nfce), BLOCK, 0, ctx->stream>>>(FL.l[1], nfce, f, ctx->d_code_f, g,
+                                                                             ctx->d_code_g, d, c, ctx->d_walk,
                                                                              ctx->d_ctr + K_WALK);
                 CK(hipGetLastError());
                 k_ds_combine<<<nblocks(nfce), BLOCK, 0, ctx->stream>>>(FL.l[1], nfce, ctx->d_walk, ctx->d_labels, d, c,
@@ -680,22 +869,17 @@ struct Driver {
         if (rc) return rc;
         if (nds + nas) {
             k_fix_paths<T><<<nblocks(nds + nas), BLOCK, 0, ctx->stream>>>(
-                path_list(1).rec, nas, path_list(0).rec, nds, f, g, d, c, ctx->d_counter, eps, range, fl.edit_type,
-                sink());
+                path_list(1).rec, nas, path_list(0).rec, nds, f, ctx->d_code_f, g, ctx->d_code_g, d, c, ctx->d_counter,
+                eps, range, fl.edit_type, sink());
             CK(hipGetLastError());
         }
         return apply();
     }
 
     int filtration() {
+        int rc = fcrit(15);
+        if (rc) return rc;
         Lists4 FL = fcrit_lists(ctx);
-        CK(hipMemsetAsync(ctx->d_ctr + K_FC0, 0, sizeof(ull) * 4, ctx->stream));
-        k_fcrit<T><<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(f, d, c, FL, ctx->d_ctr + K_FC0, 15);
-        CK(hipGetLastError());
-        int rc = sync_ctr(ctx);
-        if (rc) return rc;
-        rc = overflow_check(ctx);
-        if (rc) return rc;
         for (int ty = 0; ty < 4; ty++) {
             ull n = ctx->h_ctr[K_FC0 + ty];
             if (!n) continue;
@@ -712,26 +896,40 @@ struct Driver {
         auto t_start = Clock::now();
         int maxs = fl.max_sweeps > 0 ? fl.max_sweeps : 100000;
         CK(hipMemsetAsync(ctx->d_ctr, 0, sizeof(ull) * K_NCTR, ctx->stream));
+        ctx->dirty_total = 0;
+        auto t0 = Clock::now();
+        int rc = codes(f, ctx->d_code_f);  // gradient of f, main_3d.cu:6539-6564
+        if (rc) return rc;
         if (fl.filtration == 1) {
-            int rc = filtration();
+            rc = filtration();
             if (rc) return rc;
         }
         CK(hipMemsetAsync(ctx->d_counter, 1, (size_t)c.N, ctx->stream));  // init_counter :6782
         bool paths = fl.preserve_vpath == 1;
         if (paths) {
-            int rc = prepare_paths();
+            rc = prepare_paths();
             if (rc) return rc;
         }
-        auto t0 = Clock::now();
-        int rc = detect();
+        rc = codes(g, ctx->d_code_g);  // gradient of g, main_3d.cu:6624-6638
         if (rc) return rc;
+        CK(hipStreamSynchronize(ctx->stream));
         auto t1 = Clock::now();
         ctx->phase[0] += secs(t0, t1);
+        rc = detect();
+        if (rc) return rc;
+        ctx->phase[1] += secs(t1, Clock::now());
         i64 cnt[8] = {(i64)ctx->h_ctr[0], (i64)ctx->h_ctr[1], (i64)ctx->h_ctr[2], (i64)ctx->h_ctr[3], 1, 1, 1, 0};
         i64 sweeps = 0;
         int status = 0;
+        const char* lg = std::getenv("DMTZ_LOG");
+        const bool log = lg && lg[0] == '1';
         auto record = [&]() {
             for (int k = 0; k < 8; k++) ctx->traj.push_back(cnt[k]);
+            if (log)
+                std::fprintf(stderr, "dmtz sweep %lld: %lld %lld %lld %lld | %lld %lld  (%.3f s, %lld edits, %.0f stars)\n",
+                             (long long)sweeps, (long long)cnt[0], (long long)cnt[1], (long long)cnt[2],
+                             (long long)cnt[3], (long long)cnt[4], (long long)cnt[5], secs(t_start, Clock::now()),
+                             (long long)ctx->h_ctr[K_TGT], ctx->dirty_total);
         };
         while (cnt[5] > 0 || cnt[4] > 0 || cnt[6] > 0 || cnt[2] > 0 || cnt[3] > 0 || cnt[0] > 0 || cnt[1] > 0) {
             while (cnt[3] > 0 || cnt[2] > 0 || cnt[0] > 0 || cnt[1] > 0) {
@@ -886,13 +1084,16 @@ int gradient_host(dmtz_ctx* ctx, const T* fld, i64 nx, i64 ny, i64 nz, int64_t* 
     rc = ensure(ctx, c.N, sizeof(T), true);
     if (rc) return rc;
     CK(hipMemcpyAsync(ctx->d_f, fld, sizeof(T) * (size_t)c.N, hipMemcpyHostToDevice, ctx->stream));
+    k_code<T><<<nblocks(c.N), BLOCK, 0, ctx->stream>>>((const T*)ctx->d_f, d, c, ctx->d_code_f);
+    CK(hipGetLastError());
     int64_t* outs[4] = {pv, pe, pf, pt};
     i64 sizes[4] = {c.N, c.E, c.F, c.T};
     for (int k = 0; k < 4; k++) {
         int64_t* dbuf = nullptr;
         CK(hipMalloc(&dbuf, sizeof(int64_t) * (size_t)(sizes[k] > 0 ? sizes[k] : 1)));
         if (sizes[k] > 0) {
-            k_pairing<T><<<nblocks(sizes[k]), BLOCK, 0, ctx->stream>>>((const T*)ctx->d_f, d, c, k, (ull)sizes[k], dbuf);
+            k_pairing<T><<<nblocks(sizes[k]), BLOCK, 0, ctx->stream>>>((const T*)ctx->d_f, ctx->d_code_f, d, c, k,
+                                                                        (ull)sizes[k], dbuf);
             hipError_t e = hipGetLastError();
             if (e == hipSuccess)
                 e = hipMemcpyAsync(outs[k], dbuf, sizeof(int64_t) * (size_t)sizes[k], hipMemcpyDeviceToHost, ctx->stream);
@@ -923,6 +1124,10 @@ int targets_host(dmtz_ctx* ctx, const T* f, const T* g, i64 nx, i64 ny, i64 nz, 
     CK(hipMemsetAsync(ctx->d_counter, 1, (size_t)c.N, ctx->stream));
     CK(hipMemsetAsync(ctx->d_ctr, 0, sizeof(ull) * K_NCTR, ctx->stream));
     Driver<T> drv{ctx, (const T*)ctx->d_f, (T*)ctx->d_g, d, c, eps, range, *flags};
+    rc = drv.codes((const T*)ctx->d_f, ctx->d_code_f);
+    if (rc) return rc;
+    rc = drv.codes((const T*)ctx->d_g, ctx->d_code_g);
+    if (rc) return rc;
     rc = drv.detect();
     if (rc) return rc;
     ull cc[4] = {ctx->h_ctr[0], ctx->h_ctr[1], ctx->h_ctr[2], ctx->h_ctr[3]};
@@ -934,8 +1139,9 @@ int targets_host(dmtz_ctx* ctx, const T* f, const T* g, i64 nx, i64 ny, i64 nz, 
         if (cc[ty] == 0) continue;
         if ((ty == 1 || ty == 2) && flags->preserve_saddles != 1) continue;
         k_fix_cp<T><<<nblocks(cc[ty]), BLOCK, 0, ctx->stream>>>(ty, L.l[ty], cc[ty], (const T*)ctx->d_f,
-                                                                 (const T*)ctx->d_g, d, c, ctx->d_counter, eps, range,
-                                                                 flags->edit_type, drv.sink());
+                                                                 ctx->d_code_f, (const T*)ctx->d_g, ctx->d_code_g, d, c,
+                                                                 ctx->d_counter, eps, range, flags->edit_type,
+                                                                 drv.sink());
         CK(hipGetLastError());
     }
     rc = sync_ctr(ctx);
@@ -970,6 +1176,10 @@ int paths_host(dmtz_ctx* ctx, const T* f, const T* g, i64 nx, i64 ny, i64 nz, co
     dmtz_flags fl = *flags;
     fl.preserve_vpath = 1;
     Driver<T> drv{ctx, (const T*)ctx->d_f, (T*)ctx->d_g, d, c, 0.0, 0.0, fl};
+    rc = drv.codes((const T*)ctx->d_f, ctx->d_code_f);
+    if (rc) return rc;
+    rc = drv.codes((const T*)ctx->d_g, ctx->d_code_g);
+    if (rc) return rc;
     rc = drv.prepare_paths();
     if (rc) return rc;
     rc = drv.detect_paths();
@@ -988,9 +1198,9 @@ int paths_host(dmtz_ctx* ctx, const T* f, const T* g, i64 nx, i64 ny, i64 nz, co
         rc = drv.begin_edits();
         if (rc) return rc;
         if (nds + nas) {
-            k_fix_paths<T><<<nblocks(nds + nas), BLOCK, 0, ctx->stream>>>(pa.rec, nas, pd.rec, nds, (const T*)ctx->d_f,
-                                                                          (const T*)ctx->d_g, d, c, ctx->d_counter, 0.0,
-                                                                          0.0, fl.edit_type, drv.sink());
+            k_fix_paths<T><<<nblocks(nds + nas), BLOCK, 0, ctx->stream>>>(
+                pa.rec, nas, pd.rec, nds, (const T*)ctx->d_f, ctx->d_code_f, (const T*)ctx->d_g, ctx->d_code_g, d, c,
+                ctx->d_counter, 0.0, 0.0, fl.edit_type, drv.sink());
             CK(hipGetLastError());
         }
         rc = sync_ctr(ctx);
@@ -1042,6 +1252,8 @@ int dmtz_ctx_create(dmtz_ctx** out, int device) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&ctx->d_ctr, sizeof(ull) * K_NCTR);
     if (e == hipSuccess) e = hipHostMalloc(&ctx->h_ctr, sizeof(ull) * K_NCTR);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
     if (e != hipSuccess) {
         std::fprintf(stderr, "dmtz_ctx_create: %s\n", hipGetErrorString(e));
         delete ctx;
@@ -1056,8 +1268,12 @@ void dmtz_ctx_destroy(dmtz_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     free_bufs(ctx);
+    free_rec(ctx);
+    free_fc(ctx);
     (void)hipFree(ctx->d_ctr);
     (void)hipHostFree(ctx->h_ctr);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1108,6 +1324,16 @@ int64_t dmtz_trajectory(const dmtz_ctx* ctx, int64_t* out, int64_t cap) {
 int dmtz_phase_times(const dmtz_ctx* ctx, double* out6) {
     if (!ctx || !out6) return DMTZ_E_ARG;
     for (int k = 0; k < 6; k++) out6[k] = ctx->phase[k];
+    return 0;
+}
+
+int dmtz_kernel_stats(dmtz_ctx* ctx, double* out4, int reset) {
+    if (!ctx || !out4) return DMTZ_E_ARG;
+    out4[0] = ctx->det_launches;
+    out4[1] = ctx->det_ms;
+    out4[2] = ctx->det_bytes;
+    out4[3] = ctx->det_voxels;
+    if (reset) ctx->det_launches = ctx->det_ms = ctx->det_bytes = ctx->det_voxels = 0;
     return 0;
 }
 

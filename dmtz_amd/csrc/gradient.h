@@ -34,34 +34,31 @@ DMTZ_HD void gather_star(const T* fld, const Dims& d, i64 v, T* val) {
     val[14] = fld[v];
 }
 
-// Out-of-line lower star of vertex v, shared by all pairing lookups (one copy per field type
-// instead of one inlined copy per lookup site).
+// Lower star of vertex v from the field values (the dense classification step).
 template <class T>
-DMTZ_HD_CALL void lower_star_at(const T* fld, Dims d, i64 v, LowerStar* ls) {
+DMTZ_HD void star_of(const T* fld, const Dims& d, i64 v, LowerStar& ls) {
     T val[15];
     gather_star(fld, d, v, val);
-    lower_star(val, *ls);
+    lower_star(val, ls);
 }
 
+// Pairing lookups over precomputed star codes (code[v] = encode_star(lower star of v)). Finding
+// a cell's highest vertex needs only the cell's own values; the partner then comes from that
+// vertex's 32-byte code.
 template <class T>
 struct FieldGrad {
     const T* fld;
+    const StarCode* code;
     Dims d;
     Counts c;
 
-    DMTZ_HD FieldGrad(const T* f, const Dims& dd, const Counts& cc) : fld(f), d(dd), c(cc) {}
+    DMTZ_HD FieldGrad(const T* f, const StarCode* cd, const Dims& dd, const Counts& cc)
+        : fld(f), code(cd), d(dd), c(cc) {}
 
     // compare_vertices main_3d.cu:1043-1049: a above b.
     DMTZ_HD bool above(i64 a, i64 b) const {
         T va = fld[a], vb = fld[b];
         return va > vb || (va == vb && a > b);
-    }
-    DMTZ_HD void star(i64 v, LowerStar& ls) const { lower_star_at<T>(fld, d, v, &ls); }
-    // Inlined evaluation for the dense per-vertex kernels.
-    DMTZ_HD void star_inline(i64 v, LowerStar& ls) const {
-        T val[15];
-        gather_star(fld, d, v, val);
-        lower_star(val, ls);
     }
     DMTZ_HD i64 nb(i64 v, int slot) const {
         i64 x, y, z;
@@ -77,9 +74,8 @@ struct FieldGrad {
     // paired_vertices[v]
     DMTZ_HD i64 pv(i64 v) const {
         if (v < 0 || v >= c.N) return -1;
-        LowerStar ls;
-        star(v, ls);
-        return ls.s < 0 ? -1 : edge_of(v, ls.s);
+        int s = code_s(code[v]);
+        return s == 15 ? -1 : edge_of(v, s);
     }
 
     // paired_edges[e]
@@ -89,14 +85,10 @@ struct FieldGrad {
         edge_verts(d, c, e, a, b);
         i64 m = above(a, b) ? a : b, o = (m == a) ? b : a;
         int sl = slot_between(d, m, o);
-        LowerStar ls;
-        star(m, ls);
-        if (sl == ls.s) return m;
-        int k = ls.e0_of(sl);
-        if (k != 15) return tri_id3(d, c, m, o, nb(m, k));
-        k = ls.e1_of(sl);
-        if (k != 15) return tri_id3(d, c, m, o, nb(m, k));
-        return -1;
+        int nib = code_edge(code[m], sl);
+        if (nib == 14) return m;
+        if (nib == 15) return -1;
+        return tri_id3(d, c, m, o, nb(m, nib));
     }
 
     // paired_faces[t]
@@ -109,19 +101,16 @@ struct FieldGrad {
         if (above(r, m)) m = r;
         i64 o1 = (m == p) ? q : p;
         i64 o2 = (m == r) ? q : r;
-        int s1 = slot_between(d, m, o1), s2 = slot_between(d, m, o2);
-        int lt = ST.tri_of[s1][s2];
-        LowerStar ls;
-        star(m, ls);
-        bool hi = (ls.thi >> lt) & 1ull;
+        int lt = ST.tri_of[slot_between(d, m, o1)][slot_between(d, m, o2)];
+        int cc = code_tri(code[m], lt);
         int i = ST.tri_i[lt], j = ST.tri_j[lt];
-        int sa = hi ? i : j, sb = hi ? j : i;
-        i64 va = nb(m, sa), vb = nb(m, sb);
-        if ((ls.tE0 >> lt) & 1ull) return edge_of(m, sa);
-        if ((ls.tT0 >> lt) & 1ull) return tet_id4(d, c, m, va, vb, nb(m, ST.tri_k[lt][(ls.tD0 >> lt) & 1ull]));
-        if ((ls.tE1 >> lt) & 1ull) return edge_of(m, sb);
-        if ((ls.tT1 >> lt) & 1ull) return tet_id4(d, c, m, va, vb, nb(m, ST.tri_k[lt][(ls.tD1 >> lt) & 1ull]));
-        return -1;
+        switch (cc) {
+            case 0: return edge_of(m, i);
+            case 1: return edge_of(m, j);
+            case 2: return tet_id4(d, c, m, nb(m, i), nb(m, j), nb(m, ST.tri_k[lt][0]));
+            case 3: return tet_id4(d, c, m, nb(m, i), nb(m, j), nb(m, ST.tri_k[lt][1]));
+            default: return -1;
+        }
     }
 
     // paired_tetrahedras[q]
@@ -129,30 +118,22 @@ struct FieldGrad {
         if (q < 0 || q >= c.T) return -1;
         i64 w[4];
         tet_verts(d, q, w[0], w[1], w[2], w[3]);
-        int mi = 0;
+        i64 m = w[0];
         for (int k = 1; k < 4; k++)
-            if (above(w[k], w[mi])) mi = k;
-        i64 m = w[mi];
-        i64 o[3];
-        int n = 0;
+            if (above(w[k], m)) m = w[k];
+        uint32_t key = 0;
         for (int k = 0; k < 4; k++)
-            if (k != mi) o[n++] = w[k];
-        // order the other three: p1 above p2 above p3
-        if (above(o[1], o[0])) { i64 x = o[0]; o[0] = o[1]; o[1] = x; }
-        if (above(o[2], o[0])) { i64 x = o[0]; o[0] = o[2]; o[2] = x; }
-        if (above(o[2], o[1])) { i64 x = o[1]; o[1] = o[2]; o[2] = x; }
-        int s1 = slot_between(d, m, o[0]), s2 = slot_between(d, m, o[1]), s3 = slot_between(d, m, o[2]);
-        uint32_t key = (1u << s1) | (1u << s2) | (1u << s3);
+            if (w[k] != m) key |= 1u << slot_between(d, m, w[k]);
         int lq = 0;
         for (int k = 0; k < 24; k++) {
             uint32_t kk = (1u << ST.tet_s[k][0]) | (1u << ST.tet_s[k][1]) | (1u << ST.tet_s[k][2]);
             lq = (kk == key) ? k : lq;
         }
-        LowerStar ls;
-        star(m, ls);
-        if ((ls.qT0 >> lq) & 1u) return tri_id3(d, c, m, o[0], o[1]);
-        if ((ls.qT1 >> lq) & 1u) return tri_id3(d, c, m, o[0], o[2]);
-        return -1;
+        int cc = code_tet(code[m], lq);
+        if (cc == 3) return -1;
+        int s0 = ST.tet_s[lq][cc == 0 ? 1 : 0];
+        int s1 = ST.tet_s[lq][cc == 2 ? 1 : 2];
+        return tri_id3(d, c, m, nb(m, s0), nb(m, s1));
     }
 };
 

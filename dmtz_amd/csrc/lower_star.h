@@ -278,3 +278,93 @@ DMTZ_HD void crit_mask(const LowerStar& r, uint64_t& lo, uint32_t& hi) {
 }
 
 }  // namespace dmtz
+
+namespace dmtz {
+
+// Compact form of a lower star's pairing, 32 bytes per vertex:
+//   w[0] bits 0..55  partner nibble of the edge to slot a (a = 0..13): 14 = the centre vertex,
+//                    0..13 = the triangle completed by that slot, 15 = critical / not in the star;
+//        bits 56..59 steepest slot s (15 = none)
+//   w[1] bits 0..13  L (slots below the centre); bits 14..61: 2 bits per star tet q: the slot of
+//                    (tet_s[q][0], [1], [2]) its partner face leaves out, 3 = critical / absent
+//   w[2] 21 triangles x 3 bits, w[3] 15 triangles x 3 bits: 0 = edge to tri_i, 1 = edge to
+//                    tri_j, 2 = tet through tri_k[t][0], 3 = tet through tri_k[t][1],
+//                    7 = critical / absent
+struct StarCode {
+    uint64_t w[4];
+};
+
+DMTZ_HD int code_edge(const StarCode& c, int a) { return (int)((c.w[0] >> (4 * a)) & 15u); }
+DMTZ_HD int code_s(const StarCode& c) { return (int)((c.w[0] >> 56) & 15u); }
+DMTZ_HD uint32_t code_L(const StarCode& c) { return (uint32_t)(c.w[1] & 0x3FFFu); }
+DMTZ_HD int code_tet(const StarCode& c, int q) { return (int)((c.w[1] >> (14 + 2 * q)) & 3u); }
+DMTZ_HD int code_tri(const StarCode& c, int t) {
+    return t < 21 ? (int)((c.w[2] >> (3 * t)) & 7u) : (int)((c.w[3] >> (3 * (t - 21))) & 7u);
+}
+
+DMTZ_HD void encode_star(const LowerStar& r, StarCode& out) {
+    uint64_t w0 = 0, w1 = (uint64_t)r.L, w2 = 0, w3 = 0;
+#pragma unroll
+    for (int a = 0; a < 14; a++) {
+        int e0 = r.e0_of(a), e1 = r.e1_of(a);
+        int nib = a == r.s ? 14 : (e0 != 15 ? e0 : (e1 != 15 ? e1 : 15));
+        nib = ((r.L >> a) & 1u) ? nib : 15;
+        w0 |= (uint64_t)nib << (4 * a);
+    }
+    w0 |= (uint64_t)(r.s < 0 ? 15 : r.s) << 56;
+#pragma unroll
+    for (int t = 0; t < 36; t++) {
+        bool hi = (r.thi >> t) & 1ull;  // tri_i above tri_j
+        int code = 7;
+        if ((r.tE0 >> t) & 1ull) code = hi ? 0 : 1;        // edge to the higher slot
+        else if ((r.tT0 >> t) & 1ull) code = 2 + (int)((r.tD0 >> t) & 1ull);
+        else if ((r.tE1 >> t) & 1ull) code = hi ? 1 : 0;   // edge to the lower slot
+        else if ((r.tT1 >> t) & 1ull) code = 2 + (int)((r.tD1 >> t) & 1ull);
+        if (t < 21) w2 |= (uint64_t)code << (3 * t);
+        else w3 |= (uint64_t)code << (3 * (t - 21));
+    }
+#pragma unroll
+    for (int q = 0; q < 24; q++) {
+        const int i = ST.tet_s[q][0], j = ST.tet_s[q][1], k = ST.tet_s[q][2];
+        bool ji = (r.blw[i] >> j) & 1u, ki = (r.blw[i] >> k) & 1u, kj = (r.blw[j] >> k) & 1u;
+        // T0: the face without the lowest slot; T1: the face without the middle slot
+        int low = (ki && kj) ? 2 : ((ji && !kj) ? 1 : 0);
+        bool midi = ji != ki, midj = (!ji) != kj;
+        int mid = midi ? 0 : (midj ? 1 : 2);
+        int code = 3;
+        if ((r.qT0 >> q) & 1u) code = low;
+        else if ((r.qT1 >> q) & 1u) code = mid;
+        w1 |= (uint64_t)code << (14 + 2 * q);
+    }
+    out.w[0] = w0;
+    out.w[1] = w1;
+    out.w[2] = w2;
+    out.w[3] = w3;
+}
+
+// Critical and membership masks over the 75 local cells, from a code.
+DMTZ_HD void code_masks(const StarCode& c, uint64_t& crit_lo, uint32_t& crit_hi, uint64_t& in_lo, uint32_t& in_hi) {
+    uint32_t L = code_L(c);
+    uint64_t elo = 0, tin = 0, tcrit = 0;
+#pragma unroll
+    for (int a = 0; a < 14; a++) elo |= (uint64_t)(((L >> a) & 1u) && code_edge(c, a) == 15) << a;
+#pragma unroll
+    for (int t = 0; t < 36; t++) {
+        bool in = ((L >> ST.tri_i[t]) & 1u) && ((L >> ST.tri_j[t]) & 1u);
+        tin |= (uint64_t)in << t;
+        tcrit |= (uint64_t)(in && code_tri(c, t) == 7) << t;
+    }
+    uint32_t qin = 0, qcrit = 0;
+#pragma unroll
+    for (int q = 0; q < 24; q++) {
+        bool in = ((L >> ST.tet_s[q][0]) & 1u) && ((L >> ST.tet_s[q][1]) & 1u) && ((L >> ST.tet_s[q][2]) & 1u);
+        qin |= (uint32_t)in << q;
+        qcrit |= (uint32_t)(in && code_tet(c, q) == 3) << q;
+    }
+    crit_lo = (uint64_t)(L == 0u) | (elo << 1) | (tcrit << 15);
+    crit_hi = qcrit;
+    in_lo = 1ull | ((uint64_t)L << 1) | (tin << 15);
+    in_hi = qin;
+}
+
+}  // namespace dmtz

@@ -52,21 +52,19 @@ DMTZ_HD bool is_crit_lookup(const FieldGrad<T>& X, int type, i64 id) {
     }
 }
 
-// Mismatched local cells of vertex v: (lo, hi) bit masks over the 75 local cells.
+// Mismatched local cells of vertex v: (lo, hi) bit masks over the 75 local cells, from the
+// star codes of v in both fields.
 template <class T>
-DMTZ_HD void vertex_mismatch(const FieldGrad<T>& F, const FieldGrad<T>& G, i64 v, const LowerStar& lf,
-                             const LowerStar& lg, uint64_t& mlo, uint32_t& mhi) {
+DMTZ_HD void vertex_mismatch(const FieldGrad<T>& F, const FieldGrad<T>& G, i64 v, uint64_t& mlo, uint32_t& mhi) {
     uint64_t cf_lo, cg_lo, if_lo, ig_lo;
     uint32_t cf_hi, cg_hi, if_hi, ig_hi;
-    crit_mask(lf, cf_lo, cf_hi);
-    crit_mask(lg, cg_lo, cg_hi);
-    member_mask(lf, if_lo, if_hi);
-    member_mask(lg, ig_lo, ig_hi);
+    code_masks(F.code[v], cf_lo, cf_hi, if_lo, if_hi);
+    code_masks(G.code[v], cg_lo, cg_hi, ig_lo, ig_hi);
     mlo = (cf_lo ^ cg_lo) & if_lo & ig_lo;
     mhi = (cf_hi ^ cg_hi) & if_hi & ig_hi;
     uint64_t xf_lo = cf_lo & ~ig_lo, xg_lo = cg_lo & ~if_lo;
     uint32_t xf_hi = cf_hi & ~ig_hi, xg_hi = cg_hi & ~if_hi;
-    // cross-owner cells: rare; resolved by a lookup in the other field
+    // cross-owner cells, resolved by a lookup in the other field
     while (xf_lo | xf_hi) {
         int lc = xf_lo ? __builtin_ctzll(xf_lo) : 51 + __builtin_ctz(xf_hi);
         if (xf_lo) xf_lo &= xf_lo - 1;

@@ -21,9 +21,9 @@ struct Repair {
     int edit_type;
     Sink sink;
 
-    DMTZ_HD Repair(const T* f, const T* g, const Dims& d, const Counts& c, const uint8_t* cnt, double e,
-                   double r, int et, Sink s)
-        : F(f, d, c), G(g, d, c), counter(cnt), eps(e), range(r), edit_type(et), sink(s) {}
+    DMTZ_HD Repair(const T* f, const StarCode* cf, const T* g, const StarCode* cg, const Dims& d, const Counts& c,
+                   const uint8_t* cnt, double e, double r, int et, Sink s)
+        : F(f, cf, d, c), G(g, cg, d, c), counter(cnt), eps(e), range(r), edit_type(et), sink(s) {}
 
     DMTZ_HD i64 N() const { return F.c.N; }
     DMTZ_HD void tgt(i64 v) {

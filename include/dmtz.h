@@ -97,6 +97,12 @@ int64_t dmtz_trajectory(const dmtz_ctx* ctx, int64_t* out, int64_t cap);
  * [2] get_f_path, [3] fix_f_cp, [4] fix_f_vpath, [5] total. */
 int dmtz_phase_times(const dmtz_ctx* ctx, double* out6);
 
+/* Live statistics of the dense mismatch-detection kernel (the per-sweep classification of
+ * main_3d.cu:6864-6874 + 6903-6909) accumulated over calls on ctx, measured with HIP events on
+ * the context stream: out4 = {launches, total milliseconds, algorithmic bytes (f and g read
+ * once + 8 bytes per emitted mismatch record), voxels classified}. reset != 0 zeroes them. */
+int dmtz_kernel_stats(dmtz_ctx* ctx, double* out4, int reset);
+
 /* The reference's pairing arrays of one field (main_3d.cu:1930-2316, the five passes of
  * main_3d.cu:6550-6564): host arrays of N, E, F, T int64 entries (reference simplex IDs,
  * -1 = critical). Evaluated on the device from per-vertex lower stars. */

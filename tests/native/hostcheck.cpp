@@ -11,10 +11,22 @@
 using namespace dmtz;
 
 template <class T>
+static std::vector<StarCode> all_codes(const T* fld, const Dims& d, const Counts& c) {
+    std::vector<StarCode> out(c.N);
+    for (i64 v = 0; v < c.N; v++) {
+        LowerStar ls;
+        star_of(fld, d, v, ls);
+        encode_star(ls, out[v]);
+    }
+    return out;
+}
+
+template <class T>
 static int grad_impl(const T* fld, i64 W, i64 H, i64 D, i64* pv, i64* pe, i64* pf, i64* pt) {
     Dims d{W, H, D};
     Counts c = make_counts(d);
-    FieldGrad<T> g(fld, d, c);
+    std::vector<StarCode> cd = all_codes(fld, d, c);
+    FieldGrad<T> g(fld, cd.data(), d, c);
     for (i64 v = 0; v < c.N; v++) pv[v] = g.pv(v);
     for (i64 e = 0; e < c.E; e++) pe[e] = g.pe(e);
     for (i64 t = 0; t < c.F; t++) pf[t] = g.pf(t);
@@ -59,20 +71,15 @@ static int targets_impl(const T* f, const T* g, i64 W, i64 H, i64 D, double eps,
     Counts c = make_counts(d);
     std::vector<uint8_t> tg(c.N, 0);
     std::vector<uint8_t> counter(c.N, 1);
-    Repair<T, VecSink> R(f, g, d, c, counter.data(), eps, range, edit_type, VecSink{&tg});
+    std::vector<StarCode> cf = all_codes(f, d, c), cg = all_codes(g, d, c);
+    Repair<T, VecSink> R(f, cf.data(), g, cg.data(), d, c, counter.data(), eps, range, edit_type, VecSink{&tg});
     for (int k = 0; k < 4; k++) counts4[k] = 0;
     std::vector<std::pair<int, i64>> recs;
-    FieldGrad<T> Fg(f, d, c), Gg(g, d, c);
+    FieldGrad<T> Fg(f, cf.data(), d, c), Gg(g, cg.data(), d, c);
     for (i64 v = 0; v < c.N; v++) {
-        T val[15];
-        LowerStar a, b;
-        gather_star(f, d, v, val);
-        lower_star(val, a);
-        gather_star(g, d, v, val);
-        lower_star(val, b);
         uint64_t lo;
         uint32_t hi;
-        vertex_mismatch(Fg, Gg, v, a, b, lo, hi);
+        vertex_mismatch(Fg, Gg, v, lo, hi);
         for (int lc = 0; lc < LC_COUNT; lc++) {
             if (!lc_bit(lo, hi, lc)) continue;
             int ty = lc_type(lc);
@@ -109,17 +116,15 @@ static int paths_impl(const T* f, const T* g, i64 W, i64 H, i64 D, int geometry,
                       i64* ds_rec, i64* as_id, i64* as_rec) {
     Dims d{W, H, D};
     Counts c = make_counts(d);
-    Repair<T, NoSinkH> R(f, g, d, c, nullptr, 0, 0, 0, NoSinkH{});
-    Repair<T, NoSinkH> RF(f, f, d, c, nullptr, 0, 0, 0, NoSinkH{});
-    FieldGrad<T> Fg(f, d, c);
+    std::vector<StarCode> cf = all_codes(f, d, c), cg = all_codes(g, d, c);
+    Repair<T, NoSinkH> R(f, cf.data(), g, cg.data(), d, c, nullptr, 0, 0, 0, NoSinkH{});
+    Repair<T, NoSinkH> RF(f, cf.data(), f, cf.data(), d, c, nullptr, 0, 0, 0, NoSinkH{});
     bool ovf = false;
     i64 nds = 0, nas = 0;
     for (i64 v = 0; v < c.N; v++) {
-        LowerStar ls;
-        Fg.star(v, ls);
-        uint64_t lo;
-        uint32_t hi;
-        crit_mask(ls, lo, hi);
+        uint64_t lo, ilo;
+        uint32_t hi, ihi;
+        code_masks(cf[v], lo, hi, ilo, ihi);
         for (int lc = 1; lc < LC_TET; lc++) {
             if (!lc_bit(lo, hi, lc)) continue;
             i64 id = local_cell_id(d, c, v, lc);
@@ -173,7 +178,8 @@ static int path_targets_impl(const T* f, const T* g, i64 W, i64 H, i64 D, double
     Counts c = make_counts(d);
     std::vector<uint8_t> tg(c.N, 0);
     std::vector<uint8_t> counter(c.N, 1);
-    Repair<T, VecSink> R(f, g, d, c, counter.data(), eps, range, edit_type, VecSink{&tg});
+    std::vector<StarCode> cf = all_codes(f, d, c), cg = all_codes(g, d, c);
+    Repair<T, VecSink> R(f, cf.data(), g, cg.data(), d, c, counter.data(), eps, range, edit_type, VecSink{&tg});
     for (i64 k = 0; k < nas; k++) R.fix_as_vpath(as_rec[k]);
     for (i64 k = 0; k < nds; k++) R.fix_ds_vpath(ds_rec[k]);
     std::memcpy(targets, tg.data(), c.N);
