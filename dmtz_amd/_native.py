@@ -12,7 +12,8 @@ D = ctypes.c_double
 
 
 class Flags(ctypes.Structure):
-    """dmtz_flags: the reference's argv flags (main_3d.cu:6264-6271) plus a sweep cap."""
+    """dmtz_flags: the reference's argv flags (main_3d.cu:6264-6271) plus a sweep cap, the mode
+    and the finisher switch."""
     _fields_ = [
         ("preserve_saddles", I32),
         ("preserve_vpath", I32),
@@ -23,7 +24,7 @@ class Flags(ctypes.Structure):
         ("edit_type", I32),
         ("max_sweeps", I32),
         ("mode", I32),
-        ("reserved", I32),
+        ("finish", I32),
     ]
 
 
@@ -70,6 +71,7 @@ def lib():
     L.dmtz_trajectory.restype = I64
     L.dmtz_phase_times.argtypes = [P, P]
     L.dmtz_kernel_stats.argtypes = [P, P, ctypes.c_int]
+    L.dmtz_run_info.argtypes = [P, P]
     _LIB = L
     return L
 
@@ -79,7 +81,7 @@ EXPORTS = (
     "dmtz_ctx_create", "dmtz_ctx_destroy", "dmtz_last_error",
     "dmtz_eps_from_rel_f32", "dmtz_eps_from_rel_f64",
     "dmtz_correct_f32", "dmtz_correct_f64", "dmtz_correct_dev_f32", "dmtz_correct_dev_f64",
-    "dmtz_trajectory", "dmtz_phase_times", "dmtz_kernel_stats",
+    "dmtz_trajectory", "dmtz_phase_times", "dmtz_kernel_stats", "dmtz_run_info",
     "dmtz_gradient_f32", "dmtz_gradient_f64",
     "dmtz_targets_f32", "dmtz_targets_f64",
     "dmtz_paths_f32", "dmtz_paths_f64",

@@ -19,6 +19,10 @@ class DmtzError(RuntimeError):
     pass
 
 
+# Non-negative status codes of a correction (include/dmtz.h).
+CONVERGED, NOT_CONVERGED, STALLED = 0, 1, 2
+
+
 @dataclass
 class CorrectionFlags:
     preserve_saddles: int = 1
@@ -29,11 +33,16 @@ class CorrectionFlags:
     preserve_type: int = 0
     edit_type: int = 0
     max_sweeps: int = 100000
+    # 1: on a stall (a fixed point with mismatches left, where the reference loops forever),
+    # snap growing boxes around the stalled edit sets to f - eps, and restore f everywhere once
+    # the boxes would cover the grid (no counterpart in the reference); the run then ends with
+    # zero mismatches.
+    finish: int = 0
 
     def c(self):
         return N.Flags(self.preserve_saddles, self.preserve_vpath, self.preserve_geometry,
                        self.filtration, self.preserve_connectors, self.preserve_type,
-                       self.edit_type, self.max_sweeps, 0, 0)
+                       self.edit_type, self.max_sweeps, 0, self.finish)
 
 
 @dataclass
@@ -45,10 +54,15 @@ class CorrectionResult:
     counter: Optional[np.ndarray] = None
     trajectory: Optional[np.ndarray] = None
     phases: dict = field(default_factory=dict)
+    info: dict = field(default_factory=dict)
 
     @property
     def converged(self):
         return self.status == 0 and all(c == 0 for c in self.counts)
+
+    @property
+    def stalled(self):
+        return self.status == STALLED
 
 
 class Context:
@@ -100,6 +114,13 @@ class Context:
         self._lib.dmtz_kernel_stats(self._h, out.ctypes.data_as(ctypes.c_void_p), 1 if reset else 0)
         return dict(zip(("launches", "ms", "bytes", "voxels"), out.tolist()))
 
+    def run_info(self):
+        """Last correction: dict(stalls, finished, radius, stars) (finisher outcome and the
+        number of lower stars re-evaluated incrementally)."""
+        out = np.zeros(4, np.int64)
+        self._lib.dmtz_run_info(self._h, out.ctypes.data_as(ctypes.c_void_p))
+        return dict(zip(("stalls", "finished", "radius", "stars"), (int(x) for x in out)))
+
     def phases(self):
         out = np.zeros(6, np.float64)
         self._lib.dmtz_phase_times(self._h, out.ctypes.data_as(ctypes.c_void_p))
@@ -148,7 +169,7 @@ def correct(f, g, eps, rng, flags=None, ctx=None, return_counter=False):
             ctypes.byref(counts), _p(counter) if counter is not None else None, ctypes.byref(sweeps))
     ctx.check(rc, "dmtz_correct")
     return CorrectionResult(gout, counts.as_tuple(), int(sweeps.value), rc, counter,
-                            ctx.trajectory(), ctx.phases())
+                            ctx.trajectory(), ctx.phases(), ctx.run_info())
 
 
 def correct_device(f, g, eps, rng, flags=None, ctx=None, counter_out=None):
@@ -174,7 +195,7 @@ def correct_device(f, g, eps, rng, flags=None, ctx=None, counter_out=None):
             ctypes.byref(sweeps))
     ctx.check(rc, "dmtz_correct_dev")
     return CorrectionResult(g, counts.as_tuple(), int(sweeps.value), rc, None, ctx.trajectory(),
-                            ctx.phases())
+                            ctx.phases(), ctx.run_info())
 
 
 def gradient(field_, ctx=None):

@@ -39,11 +39,15 @@ enum : int {
     K_MIN = 0, K_S1 = 1, K_S2 = 2, K_MAX = 3,  // record lists per cell type
     K_TGT = 4,                                  // edit set size
     K_DS = 5, K_AS = 6,                         // separatrix mismatch lists
+    K_MOVED = 7,                                // edits that changed a value or a counter
     K_FC0 = 8,                                  // f-critical lists per type: 8..11
     K_WALK = 12,                                // capped walks
     K_CHG = 13, K_DIRTY = 14,                   // changed vertices, dirty stars
+    K_FINISH = 15,                              // finisher tie-repair marks
     K_NCTR = 16
 };
+
+constexpr uint8_t FROZEN = 7;  // counter value of a vertex restored to f by the finisher
 
 constexpr int BLOCK = 256;
 
@@ -71,6 +75,12 @@ __device__ __forceinline__ ull wave_alloc(ull* ctr, int n) {
     unsigned lo = __shfl((unsigned)(base & 0xffffffffu), 0, 64);
     unsigned hi = __shfl((unsigned)(base >> 32), 0, 64);
     return (((ull)hi << 32) | lo) + (ull)pre;
+}
+
+// Every lane of the wave must call this. Adds the number of lanes with x set to *ctr.
+__device__ __forceinline__ void wave_add(ull* ctr, bool x) {
+    const ull b = __ballot(x ? 1 : 0);
+    if (__lane_id() == 0 && b) atomicAdd(ctr, (ull)__popcll(b));
 }
 
 struct Lists4 {
@@ -202,37 +212,173 @@ __global__ __launch_bounds__(BLOCK) void k_fix_cp(int type, const uint64_t* recs
     }
 }
 
+// Frozen vertices are never edited. moved counts the edits that changed a value or a counter.
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_apply(const uint32_t* list, ull n, const T* f, T* g, uint8_t* counter,
                                                  double eps, double range, int preserve_type, int edit_type,
-                                                 uint32_t* changed, ull* chg_ctr) {
+                                                 uint32_t* changed, ull* chg_ctr, ull* moved) {
     const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
-    bool ch = false;
+    bool ch = false, mv = false;
     i64 v = 0;
     if (i < n) {
         v = list[i];
-        const T fv = f[v];
-        const T g0 = g[v];
-        T gv = g0;
-        int k = counter[v];
-        const double dl = edit_type == 1 ? halving_delta(eps, k) : EditArith<T>::delta0(fv, gv, eps);
-        if (preserve_type == 0) {
-            if (fabs_d(dl) > 1e-15 * range && k < 5 && EditArith<T>::step(fv, gv, dl, eps)) {
-                k += 1;
+        const int k0 = counter[v];
+        if (k0 != FROZEN) {
+            const T fv = f[v];
+            const T g0 = g[v];
+            T gv = g0;
+            int k = k0;
+            const double dl = edit_type == 1 ? halving_delta(eps, k) : EditArith<T>::delta0(fv, gv, eps);
+            if (preserve_type == 0) {
+                if (fabs_d(dl) > 1e-15 * range && k < 5 && EditArith<T>::step(fv, gv, dl, eps)) {
+                    k += 1;
+                } else {
+                    k = 6;
+                    gv = EditArith<T>::snap(fv, eps);
+                }
             } else {
                 k = 6;
                 gv = EditArith<T>::snap(fv, eps);
             }
-        } else {
-            k = 6;
-            gv = EditArith<T>::snap(fv, eps);
+            g[v] = gv;
+            counter[v] = (uint8_t)k;
+            ch = !(gv == g0);  // the order (hence every lower star) only sees value comparisons
+            mv = ch || k != k0;
         }
-        g[v] = gv;
-        counter[v] = (uint8_t)k;
-        ch = !(gv == g0);  // the order (hence every lower star) only sees value comparisons
     }
     ull at = wave_alloc(chg_ctr, ch ? 1 : 0);
     if (ch) changed[at] = (uint32_t)v;
+    wave_add(moved, mv);
+}
+
+// Finisher zone, part 1: the vertices of the first n[k] records of the four mismatch lists, each
+// listed once (claimed through its stamp with a fresh epoch).
+__global__ __launch_bounds__(BLOCK) void k_zone_cells(Lists4 recs, ull n0, ull n1, ull n2, ull n3, Dims d,
+                                                      uint32_t* stamp, uint32_t epoch, uint32_t* list, ull* ctr) {
+    const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    i64 vs[4] = {-1, -1, -1, -1};
+    if (i < n0 + n1 + n2 + n3) {
+        const int k = i < n0 ? 0 : (i < n0 + n1 ? 1 : (i < n0 + n1 + n2 ? 2 : 3));
+        const ull j = i - (k > 0 ? n0 : 0) - (k > 1 ? n1 : 0) - (k > 2 ? n2 : 0);
+        const uint64_t r = recs.l[k][j];
+        const i64 v = (i64)(r >> 7);
+        const int lc = (int)(r & 127u);
+        i64 x, y, z;
+        vxyz(d, v, x, y, z);
+        auto nb = [&](int sl) { return vidx(d, x + DIR[sl][0], y + DIR[sl][1], z + DIR[sl][2]); };
+        vs[0] = v;
+        if (lc >= LC_EDGE && lc < LC_TRI) {
+            vs[1] = nb(lc - LC_EDGE);
+        } else if (lc >= LC_TRI && lc < LC_TET) {
+            vs[1] = nb(ST.tri_i[lc - LC_TRI]);
+            vs[2] = nb(ST.tri_j[lc - LC_TRI]);
+        } else if (lc >= LC_TET) {
+            vs[1] = nb(ST.tet_s[lc - LC_TET][0]);
+            vs[2] = nb(ST.tet_s[lc - LC_TET][1]);
+            vs[3] = nb(ST.tet_s[lc - LC_TET][2]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const bool in = vs[q] >= 0 && atomicExch(stamp + vs[q], epoch) != epoch;
+        ull at = wave_alloc(ctr, in ? 1 : 0);
+        if (in) list[at] = (uint32_t)vs[q];
+    }
+}
+
+// Finisher zone, part 2: the vertices targeted by an edit set of epoch in [from, to).
+__global__ __launch_bounds__(BLOCK) void k_zone(const uint32_t* stamp, uint32_t from, uint32_t to, i64 n,
+                                                uint32_t* list, ull* ctr) {
+    const i64 v = (i64)blockIdx.x * BLOCK + threadIdx.x;
+    bool in = false;
+    if (v < n) {
+        const uint32_t s = stamp[v];
+        in = s >= from && s < to;
+    }
+    ull at = wave_alloc(ctr, in ? 1 : 0);
+    if (in) list[at] = (uint32_t)v;
+}
+
+// The box B: every vertex within Chebyshev distance r of a zone vertex is listed once (claimed
+// through its stamp), snapped unless already frozen (g = snap(f): the terminal edit of
+// applyDeltaBuffer1, main_3d.cu:1140-1141; the reference keeps a snap of the 14 neighbours
+// commented out at main_3d.cu:1144-1151) and frozen. One work item per (zone vertex, offset in
+// the (2r+1)^3 box), grid-stride with a wave-uniform trip count.
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_snap_boxes(const uint32_t* zone, ull nz, int r, Dims d, const T* f, T* g,
+                                                      uint8_t* counter, double eps, uint32_t* stamp, uint32_t epoch,
+                                                      uint32_t* box, ull* box_ctr) {
+    const ull side = 2 * (ull)r + 1, cube = side * side * side, total = nz * cube;
+    for (ull base = (ull)blockIdx.x * BLOCK; base < total; base += (ull)gridDim.x * BLOCK) {
+        const ull j = base + threadIdx.x;
+        bool in = false;
+        i64 u = 0;
+        if (j < total) {
+            const i64 v = zone[j / cube];
+            const ull o = j % cube;
+            i64 x, y, z;
+            vxyz(d, v, x, y, z);
+            const i64 ux = x + (i64)(o % side) - r, uy = y + (i64)((o / side) % side) - r,
+                      uz = z + (i64)(o / (side * side)) - r;
+            if (inb(d, ux, uy, uz)) {
+                u = vidx(d, ux, uy, uz);
+                if (atomicExch(stamp + u, epoch) != epoch) {
+                    in = true;
+                    if (counter[u] != FROZEN) g[u] = EditArith<T>::snap(f[u], eps);
+                    counter[u] = FROZEN;
+                }
+            }
+        }
+        ull at = wave_alloc(box_ctr, in ? 1 : 0);
+        if (in) box[at] = (uint32_t)u;
+    }
+}
+
+// Tie repair, marking step of one Jacobi round: a box vertex u is marked when a neighbour w has
+// g(w) == g(u), f(w) < f(u) and w > u (the index tie-break orders the pair against f).
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_tie_mark(const uint32_t* box, ull n, Dims d, const T* f, const T* g,
+                                                    uint32_t* stamp, uint32_t epoch, ull* marked) {
+    const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    bool mk = false;
+    if (i < n) {
+        const i64 u = box[i];
+        const T fu = f[u], gu = g[u];
+        i64 x, y, z;
+        vxyz(d, u, x, y, z);
+        for (int k = 0; k < 14; k++) {
+            const i64 xx = x + DIR[k][0], yy = y + DIR[k][1], zz = z + DIR[k][2];
+            if (!inb(d, xx, yy, zz)) continue;
+            const i64 w = vidx(d, xx, yy, zz);
+            if (g[w] == gu && f[w] < fu && w > u) {
+                mk = true;
+                break;
+            }
+        }
+        if (mk) stamp[u] = epoch;
+    }
+    wave_add(marked, mk);
+}
+
+// Tie repair, raising step: marked vertices move to the next representable value.
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_tie_raise(const uint32_t* box, ull n, T* g, const uint32_t* stamp,
+                                                     uint32_t epoch) {
+    const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) {
+        const i64 u = box[i];
+        if (stamp[u] == epoch) g[u] = next_up(g[u]);
+    }
+}
+
+// Restore g = f and freeze every vertex (counter 7: never edited again).
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_freeze_all(const T* f, T* g, uint8_t* counter, i64 n) {
+    const i64 v = (i64)blockIdx.x * BLOCK + threadIdx.x;
+    if (v < n) {
+        g[v] = f[v];
+        counter[v] = FROZEN;
+    }
 }
 
 // Closed stars of the changed vertices: their lower stars must be re-evaluated. One thread per
@@ -501,6 +647,7 @@ struct dmtz_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double det_launches = 0, det_ms = 0, det_bytes = 0, det_voxels = 0;
     double dirty_total = 0;  // star codes re-evaluated incrementally in the last correction
+    int64_t stalls = 0, finished = 0, radius = 0;  // finisher outcome of the last correction
 };
 
 namespace {
@@ -715,12 +862,16 @@ struct Driver {
         return s;
     }
 
+    uint32_t zone_from = 0;  // first epoch of the unchanged sweeps since the last change (0: none)
+
     int begin_edits() {
         ctx->epoch++;
-        if (ctx->epoch == 0) {  // stamp wrap: clear and restart
+        if (ctx->epoch == 0) {  // stamp wrap (after 2^32 edit sets): clear and restart; the finisher
+                                // zone then keeps only the edit sets from here on
             CK(hipMemsetAsync(ctx->d_stamp, 0, sizeof(uint32_t) * (size_t)c.N, ctx->stream));
             CK(hipMemsetAsync(ctx->d_dstamp, 0, sizeof(uint32_t) * (size_t)c.N, ctx->stream));
             ctx->epoch = 1;
+            if (zone_from) zone_from = 1;
         }
         CK(hipMemsetAsync(ctx->d_ctr + K_TGT, 0, sizeof(ull), ctx->stream));
         return 0;
@@ -745,7 +896,10 @@ struct Driver {
     // applyDeltaBuffer1 (main_3d.cu:1122-1164) over the edit set, then re-evaluation of the g star
     // codes of the closed stars of the vertices whose value changed (the g gradient of
     // main_3d.cu:6864-6874 restricted to where it can change).
+    ull moved = 0;  // edits of the last apply that changed a value or a counter
+
     int apply() {
+        moved = 0;
         int rc = sync_ctr(ctx);
         if (rc) return rc;
         rc = overflow_check(ctx);
@@ -753,18 +907,25 @@ struct Driver {
         ull nt = ctx->h_ctr[K_TGT];
         if (!nt) return 0;
         CK(hipMemsetAsync(ctx->d_ctr + K_CHG, 0, sizeof(ull) * 2, ctx->stream));
+        CK(hipMemsetAsync(ctx->d_ctr + K_MOVED, 0, sizeof(ull), ctx->stream));
         k_apply<T><<<nblocks(nt), BLOCK, 0, ctx->stream>>>(ctx->d_targets, nt, f, g, ctx->d_counter, eps, range,
                                                            fl.preserve_type, fl.edit_type, ctx->d_changed,
-                                                           ctx->d_ctr + K_CHG);
+                                                           ctx->d_ctr + K_CHG, ctx->d_ctr + K_MOVED);
         CK(hipGetLastError());
         rc = sync_ctr(ctx);
         if (rc) return rc;
-        ull nc = ctx->h_ctr[K_CHG];
+        moved = ctx->h_ctr[K_MOVED];
+        return refresh(ctx->h_ctr[K_CHG]);
+    }
+
+    // Re-evaluate the g star codes of the closed stars of the nc vertices in the changed list
+    // (the g gradient of main_3d.cu:6864-6874 restricted to where it can change).
+    int refresh(ull nc) {
         if (!nc) return 0;
         k_dirty<<<nblocks(15 * nc), BLOCK, 0, ctx->stream>>>(ctx->d_changed, nc, d, ctx->d_dstamp, ctx->epoch,
                                                             ctx->d_dirty, ctx->d_ctr + K_DIRTY);
         CK(hipGetLastError());
-        rc = sync_ctr(ctx);
+        int rc = sync_ctr(ctx);
         if (rc) return rc;
         ull nd = ctx->h_ctr[K_DIRTY];
         if (nd) {
@@ -772,6 +933,77 @@ struct Driver {
             CK(hipGetLastError());
         }
         ctx->dirty_total += (double)nd;
+        return 0;
+    }
+
+    // Finisher step (no counterpart in the reference). The zone is the vertices of the first
+    // cnt[k] mismatch records and the vertices targeted since zone_from. With r = ctx->radius
+    // below the longest grid side and a non-empty zone: snap every vertex within Chebyshev
+    // distance r of the zone that is not frozen yet, freeze them all, and repair the ties the
+    // float snap creates by one-ulp raises. Otherwise restore g = f on every vertex and freeze it.
+    int finish_step(const i64* cnt) {
+        const int r = (int)ctx->radius;
+        int rc = begin_edits();  // fresh epoch for the cell vertices; resets the list counter
+        if (rc) return rc;
+        const uint32_t from = zone_from, to = ctx->epoch;
+        CK(hipMemsetAsync(ctx->d_ctr + K_CHG, 0, sizeof(ull) * 3, ctx->stream));  // changed, dirty, finished
+        const ull ncell = (ull)(cnt[0] + cnt[1] + cnt[2] + cnt[3]);
+        if (ncell) {
+            k_zone_cells<<<nblocks(ncell), BLOCK, 0, ctx->stream>>>(rec_lists(ctx), (ull)cnt[0], (ull)cnt[1],
+                                                                     (ull)cnt[2], (ull)cnt[3], d, ctx->d_stamp, to,
+                                                                     ctx->d_targets, ctx->d_ctr + K_TGT);
+            CK(hipGetLastError());
+        }
+        k_zone<<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(ctx->d_stamp, from, to, c.N, ctx->d_targets,
+                                                         ctx->d_ctr + K_TGT);
+        CK(hipGetLastError());
+        rc = sync_ctr(ctx);
+        if (rc) return rc;
+        const ull nz = ctx->h_ctr[K_TGT];
+        const i64 span = d.W > d.H ? (d.W > d.D ? d.W : d.D) : (d.H > d.D ? d.H : d.D);
+        if (nz == 0 || r >= span) {
+            k_freeze_all<T><<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(f, g, ctx->d_counter, c.N);
+            CK(hipGetLastError());
+            rc = codes(g, ctx->d_code_g);
+            if (rc) return rc;
+            ctx->finished += c.N;
+        } else {
+            rc = begin_edits();  // fresh epoch: each vertex is claimed once
+            if (rc) return rc;
+            const ull side = 2 * (ull)r + 1, total = nz * side * side * side;
+            const ull blocks = nblocks(total) < 65536 ? nblocks(total) : 65536;
+            k_snap_boxes<T><<<blocks, BLOCK, 0, ctx->stream>>>(ctx->d_targets, nz, r, d, f, g, ctx->d_counter, eps,
+                                                                ctx->d_stamp, ctx->epoch, ctx->d_changed,
+                                                                ctx->d_ctr + K_CHG);
+            CK(hipGetLastError());
+            rc = sync_ctr(ctx);
+            if (rc) return rc;
+            const ull nb = ctx->h_ctr[K_CHG];
+            // Jacobi rounds of the tie repair; every round raises a vertex above a lower-f
+            // neighbour, so at most nb rounds can be needed
+            for (ull round = 0;; round++) {
+                if (round > nb) {
+                    ctx->err = "finisher tie repair did not settle";
+                    return DMTZ_E_WALK;
+                }
+                rc = begin_edits();
+                if (rc) return rc;
+                CK(hipMemsetAsync(ctx->d_ctr + K_FINISH, 0, sizeof(ull), ctx->stream));
+                k_tie_mark<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(ctx->d_changed, nb, d, f, g, ctx->d_stamp,
+                                                                       ctx->epoch, ctx->d_ctr + K_FINISH);
+                CK(hipGetLastError());
+                rc = sync_ctr(ctx);
+                if (rc) return rc;
+                if (ctx->h_ctr[K_FINISH] == 0) break;
+                k_tie_raise<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(ctx->d_changed, nb, g, ctx->d_stamp,
+                                                                        ctx->epoch);
+                CK(hipGetLastError());
+            }
+            rc = refresh(nb);
+            if (rc) return rc;
+            ctx->finished += (int64_t)nb;
+        }
+        zone_from = 0;
         return 0;
     }
 
@@ -889,10 +1121,16 @@ struct Driver {
         return 0;
     }
 
-    // The loop of main_3d.cu:6521-7149.
+    // The loop of main_3d.cu:6521-7149, with an exit at a stall: a sweep is a function of
+    // (g, counter, counts, sweep kind), so a sweep that changes no value or counter and whose
+    // (counts, kind) repeats one seen since the last change starts a cycle the reference never
+    // leaves. Without the finisher the run returns DMTZ_STALLED there; with it, finish_step() is
+    // applied and the sweep's counts are re-evaluated.
     int run(dmtz_counts* out, int32_t* sweeps_out) {
         ctx->traj.clear();
         for (double& p : ctx->phase) p = 0;
+        ctx->stalls = ctx->finished = ctx->radius = 0;
+        zone_from = 0;
         auto t_start = Clock::now();
         int maxs = fl.max_sweeps > 0 ? fl.max_sweeps : 100000;
         CK(hipMemsetAsync(ctx->d_ctr, 0, sizeof(ull) * K_NCTR, ctx->stream));
@@ -931,6 +1169,75 @@ struct Driver {
                              (long long)cnt[3], (long long)cnt[4], (long long)cnt[5], secs(t_start, Clock::now()),
                              (long long)ctx->h_ctr[K_TGT], ctx->dirty_total);
         };
+        // Re-evaluation after the apply of a critical-point sweep (main_3d.cu:6864-6921).
+        auto eval_cp = [&]() -> int {
+            auto tb = Clock::now();
+            int r = detect();
+            if (r) return r;
+            ctx->phase[1] += secs(tb, Clock::now());
+            cnt[0] = (i64)ctx->h_ctr[0];
+            cnt[1] = fl.preserve_saddles == 1 ? (i64)ctx->h_ctr[1] : 0;
+            cnt[2] = fl.preserve_saddles == 1 ? (i64)ctx->h_ctr[2] : 0;
+            cnt[3] = (i64)ctx->h_ctr[3];
+            return 0;
+        };
+        // Re-evaluation after the apply of a separatrix sweep (main_3d.cu:7069-7143).
+        auto eval_path = [&]() -> int {
+            auto tc = Clock::now();
+            int r = detect();
+            if (r) return r;
+            auto td = Clock::now();
+            ctx->phase[1] += secs(tc, td);
+            cnt[0] = (i64)ctx->h_ctr[0];
+            cnt[1] = (i64)ctx->h_ctr[1];
+            cnt[2] = (i64)ctx->h_ctr[2];
+            cnt[3] = (i64)ctx->h_ctr[3];
+            cnt[4] = cnt[5] = cnt[6] = cnt[7] = 0;
+            if (cnt[0] == 0 && cnt[1] == 0 && cnt[2] == 0 && cnt[3] == 0) {
+                r = detect_paths();
+                if (r) return r;
+                cnt[4] = (i64)ctx->h_ctr[K_DS];
+                cnt[5] = (i64)ctx->h_ctr[K_AS];
+                ctx->phase[2] += secs(td, Clock::now());
+            }
+            return 0;
+        };
+        std::vector<std::vector<i64>> seen;  // (counts, kind) since the last change
+        // After the re-evaluation of a sweep of the given kind (0 critical points, 1 separatrices).
+        // Sets stalled when the run stalls without the finisher.
+        bool stalled = false;
+        auto after_sweep = [&](int kind) -> int {
+            std::vector<i64> key(cnt, cnt + 8);
+            key.push_back(kind);
+            if (moved > 0) {
+                seen.assign(1, key);
+                zone_from = 0;
+                return 0;
+            }
+            if (zone_from == 0) zone_from = ctx->epoch;  // this sweep's edit set opens the zone
+            bool any = false;
+            for (int k = 0; k < 8; k++) any = any || cnt[k] != 0;
+            if (!any) return 0;
+            for (const auto& sk : seen)
+                if (sk == key) {
+                    ctx->stalls++;
+                    if (fl.finish != 1) {
+                        stalled = true;
+                        return 0;
+                    }
+                    ctx->radius++;
+                    int r = finish_step(cnt);
+                    if (r) return r;
+                    r = kind == 0 ? eval_cp() : eval_path();
+                    if (r) return r;
+                    key.assign(cnt, cnt + 8);
+                    key.push_back(kind);
+                    seen.assign(1, key);
+                    return 0;
+                }
+            seen.push_back(key);
+            return 0;
+        };
         while (cnt[5] > 0 || cnt[4] > 0 || cnt[6] > 0 || cnt[2] > 0 || cnt[3] > 0 || cnt[0] > 0 || cnt[1] > 0) {
             while (cnt[3] > 0 || cnt[2] > 0 || cnt[0] > 0 || cnt[1] > 0) {
                 if (sweeps >= maxs) {
@@ -942,17 +1249,16 @@ struct Driver {
                 ull cc[4] = {(ull)cnt[0], (ull)cnt[1], (ull)cnt[2], (ull)cnt[3]};
                 rc = fix_cp(cc);
                 if (rc) return rc;
-                auto tb = Clock::now();
-                ctx->phase[3] += secs(ta, tb);
-                rc = detect();
+                ctx->phase[3] += secs(ta, Clock::now());
+                rc = eval_cp();
                 if (rc) return rc;
-                auto tc = Clock::now();
-                ctx->phase[1] += secs(tb, tc);
-                cnt[0] = (i64)ctx->h_ctr[0];
-                cnt[1] = fl.preserve_saddles == 1 ? (i64)ctx->h_ctr[1] : 0;
-                cnt[2] = fl.preserve_saddles == 1 ? (i64)ctx->h_ctr[2] : 0;
-                cnt[3] = (i64)ctx->h_ctr[3];
+                rc = after_sweep(0);
+                if (rc) return rc;
                 record();
+                if (stalled) {
+                    status = DMTZ_STALLED;
+                    goto done;
+                }
             }
             if (!paths) break;
             if (sweeps >= maxs) {
@@ -969,25 +1275,16 @@ struct Driver {
                 ull nds = ctx->h_ctr[K_DS], nas = ctx->h_ctr[K_AS];
                 rc = fix_paths(nds, nas);
                 if (rc) return rc;
-                auto tc = Clock::now();
-                ctx->phase[4] += secs(tb, tc);
-                rc = detect();
+                ctx->phase[4] += secs(tb, Clock::now());
+                rc = eval_path();
                 if (rc) return rc;
-                auto td = Clock::now();
-                ctx->phase[1] += secs(tc, td);
-                cnt[0] = (i64)ctx->h_ctr[0];
-                cnt[1] = (i64)ctx->h_ctr[1];
-                cnt[2] = (i64)ctx->h_ctr[2];
-                cnt[3] = (i64)ctx->h_ctr[3];
-                cnt[4] = cnt[5] = cnt[6] = cnt[7] = 0;
-                if (cnt[0] == 0 && cnt[1] == 0 && cnt[2] == 0 && cnt[3] == 0) {
-                    rc = detect_paths();
-                    if (rc) return rc;
-                    cnt[4] = (i64)ctx->h_ctr[K_DS];
-                    cnt[5] = (i64)ctx->h_ctr[K_AS];
-                    ctx->phase[2] += secs(td, Clock::now());
-                }
+                rc = after_sweep(1);
+                if (rc) return rc;
                 record();
+                if (stalled) {
+                    status = DMTZ_STALLED;
+                    goto done;
+                }
             }
         }
     done:
@@ -1334,6 +1631,15 @@ int dmtz_kernel_stats(dmtz_ctx* ctx, double* out4, int reset) {
     out4[2] = ctx->det_bytes;
     out4[3] = ctx->det_voxels;
     if (reset) ctx->det_launches = ctx->det_ms = ctx->det_bytes = ctx->det_voxels = 0;
+    return 0;
+}
+
+int dmtz_run_info(const dmtz_ctx* ctx, int64_t* out4) {
+    if (!ctx || !out4) return DMTZ_E_ARG;
+    out4[0] = ctx->stalls;
+    out4[1] = ctx->finished;
+    out4[2] = ctx->radius;
+    out4[3] = (int64_t)ctx->dirty_total;
     return 0;
 }
 

@@ -24,6 +24,20 @@ DMTZ_HD float next_up_f32(float x) {
     return r;
 }
 
+DMTZ_HD double next_up_f64(double x) {
+    if (x == 0.0) return 4.9406564584124654e-324;  // smallest positive subnormal
+    uint64_t b;
+    memcpy(&b, &x, 8);
+    b = (x > 0.0) ? b + 1u : b - 1u;
+    double r;
+    memcpy(&r, &b, 8);
+    return r;
+}
+
+// The next representable value above x (x finite).
+DMTZ_HD float next_up(float x) { return next_up_f32(x); }
+DMTZ_HD double next_up(double x) { return next_up_f64(x); }
+
 template <class T>
 struct EditArith;
 

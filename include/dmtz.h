@@ -11,7 +11,7 @@
  * Conventions
  *   - Fields are contiguous, x fastest: index = x + y*nx + z*nx*ny (main_3d.cu:1945-1947).
  *   - Grids need nx >= 3, ny >= 2, nz >= 2 and nx*ny*nz < 2^31; values must be finite.
- *   - Return codes: 0 = converged / OK, 1 = not converged within max_sweeps,
+ *   - Return codes: 0 = converged / OK, 1 = not converged within max_sweeps, 2 = stalled,
  *     negative = error (see DMTZ_E_*); dmtz_last_error() gives a message. Nothing calls exit().
  *   - A context owns one HIP stream and its scratch memory; calls on one context are
  *     host-synchronous and a context must not be used from two threads at once.
@@ -27,14 +27,25 @@ extern "C" {
 
 #define DMTZ_OK 0
 #define DMTZ_NOT_CONVERGED 1
+/* The loop reached a fixed point with mismatches left: a sweep changed no value and no counter
+ * and its counts repeat, so every later sweep repeats it (the reference loops forever there:
+ * main_3d.cu:6784 has no exit). The returned field is the state the reference keeps. */
+#define DMTZ_STALLED 2
 #define DMTZ_E_ARG (-1)
 #define DMTZ_E_HIP (-2)
 #define DMTZ_E_CAPACITY (-3)
 #define DMTZ_E_UNSUPPORTED (-4)
 #define DMTZ_E_WALK (-5)
 
-/* The reference's positional flags (main_3d.cu:6264-6271) plus a sweep cap.
- * max_sweeps <= 0 selects the default cap (100000). mode: 0 = plain loop. */
+/* The reference's positional flags (main_3d.cu:6264-6271) plus a sweep cap and a finisher.
+ * max_sweeps <= 0 selects the default cap (100000). mode: 0 = plain loop.
+ * finish = 1 (no counterpart in the reference): on the k-th stall, snap every vertex within
+ * Chebyshev distance k of the edit sets of the stalled sweeps (g = the snap of f - eps, counter 6:
+ * the terminal edit of main_3d.cu:1140-1141, which the reference never undoes). When those boxes
+ * would cover the grid, or the edit sets are empty, restore g = f on every vertex instead
+ * (counter 7, never edited again). Edits only lower g or raise counters and the radius grows
+ * with every stall, so the loop then ends with zero mismatches and |g - f| <= eps. Runs that
+ * never stall are unchanged. */
 typedef struct dmtz_flags {
     int32_t preserve_saddles;    /* argv[4]: fix 1- and 2-saddle mismatches            */
     int32_t preserve_vpath;      /* argv[5]: also match separatrices (V-paths)          */
@@ -45,7 +56,7 @@ typedef struct dmtz_flags {
     int32_t edit_type;           /* argv[10]: 0 = halve the gap, 1 = -2^-k eps steps     */
     int32_t max_sweeps;          /* cap on correction sweeps (the reference has none)    */
     int32_t mode;                /* 0 = plain                                           */
-    int32_t reserved;
+    int32_t finish;              /* 1 = snap boxes around stalled edit sets (see above)  */
 } dmtz_flags;
 
 /* Mismatch counters count_f_* (main_3d.cu:68-75) in the order min, 1-saddle, 2-saddle, max,
@@ -69,7 +80,8 @@ int dmtz_eps_from_rel_f64(const double* f, int64_t n, double rel, double* eps_ou
 /* Full correction of g (host pointers): main_3d.cu:6384-7156 (H2D copy, gradient of f and g,
  * mismatch detection, the sweep loop of main_3d.cu:6784-7149, D2H copy of the corrected field).
  * g_inout is corrected in place. counter_out (nullable, nx*ny*nz bytes) receives delta_counter
- * (main_3d.cu:84, values 1..6). sweeps_out (nullable) receives the number of sweeps. */
+ * (main_3d.cu:84, values 1..6; 7 = restored to f by the finisher). sweeps_out (nullable) receives the
+ * number of sweeps. */
 int dmtz_correct_f32(dmtz_ctx* ctx, const float* f, float* g_inout, int64_t nx, int64_t ny, int64_t nz,
                      double eps, double range, const dmtz_flags* flags, dmtz_counts* out,
                      uint8_t* counter_out, int32_t* sweeps_out);
@@ -102,6 +114,11 @@ int dmtz_phase_times(const dmtz_ctx* ctx, double* out6);
  * the context stream: out4 = {launches, total milliseconds, algorithmic bytes (f and g read
  * once + 8 bytes per emitted mismatch record), voxels classified}. reset != 0 zeroes them. */
 int dmtz_kernel_stats(dmtz_ctx* ctx, double* out4, int reset);
+
+/* Outcome of the last correction on ctx beyond the counts: out4 = {stalls, vertices changed by
+ * the finisher, finisher radius reached, lower stars re-evaluated incrementally}. No reference
+ * counterpart (the reference neither detects stalls nor finishes them). */
+int dmtz_run_info(const dmtz_ctx* ctx, int64_t* out4);
 
 /* The reference's pairing arrays of one field (main_3d.cu:1930-2316, the five passes of
  * main_3d.cu:6550-6564): host arrays of N, E, F, T int64 entries (reference simplex IDs,

@@ -20,7 +20,17 @@
 //   * descending/ascending path walks are capped at N+1 steps (the reference has no cap);
 //   * ascending connectivity labels are indexed by face id (main_3d.cu:4750/4823 shadowing bug);
 //   * walls use a private per-start DFS with a separate index array (main_3d.cu:6706, 5088);
-//   * the correction loop stops after max_sweeps sweeps and reports non-convergence.
+//   * the correction loop stops after max_sweeps sweeps and reports non-convergence;
+//   * the correction loop stops when it has reached a fixed point with mismatches left (a sweep
+//     that changes no value or counter and repeats an earlier control configuration), where the
+//     reference loops forever, and reports it as stalled.
+// Extension without a counterpart in the reference (flag finish, off by default): on the k-th
+// stall every vertex within Chebyshev distance k of the zone (the edit sets of the stalled sweeps
+// and the vertices of the mismatched critical cells) is snapped (the reference's terminal edit,
+// main_3d.cu:1140-1141; a snap of the 14 neighbours sits commented out at main_3d.cu:1144-1151),
+// ties the float snap creates are repaired by one-ulp raises, and the box is frozen (counter 7,
+// never edited again). Once the boxes would cover the grid, or when the zone is empty, g = f is
+// restored everywhere. The run then ends at zero mismatches.
 
 #include <cmath>
 #include <cstdint>
@@ -28,6 +38,7 @@
 #include <cstring>
 #include <cfloat>
 #include <climits>
+#include <limits>
 #include <vector>
 #include <algorithm>
 #include <atomic>
@@ -524,8 +535,10 @@ template <> struct Arith<float> {
 // ----------------------------------------------------------------------------------------------
 struct Flags {
     int32_t preserve_saddles, preserve_vpath, preserve_geometry, filtration, preserve_connectors,
-        preserve_type, edit_type, max_sweeps;
+        preserve_type, edit_type, max_sweeps, finish;
 };
+
+constexpr int32_t FROZEN = 7;  // counter value of a vertex restored to f by the finisher
 
 enum { F_FIELD = 0, G_FIELD = 1 };
 
@@ -1500,13 +1513,22 @@ template <class T> struct State {
     // ------------------------------------------------------------------------------------------
     // Edit buffer apply (main_3d.cu:1122-1164) and filtration (main_3d.cu:2405-2490)
     // ------------------------------------------------------------------------------------------
+    // Returns the number of edit-set vertices whose value or counter changed; the edit set is kept
+    // in last_edits. Frozen vertices are never edited.
+    std::vector<i64> last_edits;
     i64 apply() {
-        i64 n = 0;
-#pragma omp parallel for schedule(static) reduction(+ : n)
-        for (i64 v = 0; v < g.N; v++) {
-            if (!target[v]) continue;
+        last_edits.clear();
+        for (i64 v = 0; v < g.N; v++)
+            if (target[v]) last_edits.push_back(v);
+        const i64 n = (i64)last_edits.size();
+        i64 moved = 0;
+#pragma omp parallel for schedule(static) reduction(+ : moved)
+        for (i64 k = 0; k < n; k++) {
+            const i64 v = last_edits[k];
             target[v] = 0;
-            n++;
+            if (counter[v] == FROZEN) continue;
+            const T g0 = d[v];
+            const int32_t k0 = counter[v];
             double dl = delta(v);
             if (fl.preserve_type == 0) {
                 T gv = d[v];
@@ -1521,8 +1543,89 @@ template <class T> struct State {
                 counter[v] = 6;
                 d[v] = Arith<T>::snap(f[v], eps);
             }
+            if (!(d[v] == g0) || counter[v] != k0) moved++;
         }
-        return n;
+        return moved;
+    }
+
+    // Vertices of the first c[k] cells of the four mismatch lists.
+    void mismatch_vertices(const i64* c, std::vector<i64>& out) const {
+        for (i64 i = 0; i < c[0]; i++) out.push_back(l_min[i]);
+        for (i64 i = 0; i < c[1]; i++) {
+            i64 a, b;
+            edge_verts(g, l_s1[i], a, b);
+            out.push_back(a);
+            out.push_back(b);
+        }
+        for (i64 i = 0; i < c[2]; i++) {
+            i64 a, b, e;
+            tri_verts(g, l_s2[i], a, b, e);
+            out.push_back(a);
+            out.push_back(b);
+            out.push_back(e);
+        }
+        for (i64 i = 0; i < c[3]; i++) {
+            i64 a, b, e, h;
+            tet_verts(g, l_max[i], a, b, e, h);
+            out.push_back(a);
+            out.push_back(b);
+            out.push_back(e);
+            out.push_back(h);
+        }
+    }
+
+    // Finisher step (no counterpart in the reference). For r below the longest grid side and a
+    // non-empty zone, on the box B of vertices within Chebyshev distance r of a zone vertex:
+    //   1. every vertex of B not yet frozen is snapped (g = snap(f)) and B is frozen (counter 7);
+    //   2. ties the float snap creates are repaired in Jacobi rounds: a vertex u of B with a
+    //      neighbour w such that g(w) == g(u), f(w) < f(u) and w > u (so the index tie-break
+    //      orders them against f) is raised to the next float; rounds repeat until none is.
+    // Otherwise g = f is restored on every vertex, all frozen. Returns the number of vertices
+    // in B (every vertex for a restore).
+    i64 finish_step(const std::vector<i64>& zone, int r) {
+        const i64 span = std::max(g.W, std::max(g.H, g.D));
+        if (zone.empty() || r >= span) {
+            for (i64 u = 0; u < g.N; u++) {
+                d[u] = f[u];
+                counter[u] = FROZEN;
+            }
+            return g.N;
+        }
+        std::vector<uint8_t> mark(g.N, 0);
+        for (i64 v : zone) {
+            const i64 x = v % g.W, y = (v / g.W) % g.H, z = v / (g.W * g.H);
+            const i64 x0 = std::max<i64>(x - r, 0), x1 = std::min<i64>(x + r, g.W - 1);
+            const i64 y0 = std::max<i64>(y - r, 0), y1 = std::min<i64>(y + r, g.H - 1);
+            const i64 z0 = std::max<i64>(z - r, 0), z1 = std::min<i64>(z + r, g.D - 1);
+            for (i64 zz = z0; zz <= z1; zz++)
+                for (i64 yy = y0; yy <= y1; yy++)
+                    for (i64 xx = x0; xx <= x1; xx++) mark[(zz * g.H + yy) * g.W + xx] = 1;
+        }
+        std::vector<i64> box;
+        for (i64 u = 0; u < g.N; u++)
+            if (mark[u]) {
+                box.push_back(u);
+                if (counter[u] != FROZEN) d[u] = Arith<T>::snap(f[u], eps);
+                counter[u] = FROZEN;
+            }
+        for (;;) {
+            std::vector<i64> raise;
+            for (i64 u : box) {
+                const i64 x = u % g.W, y = (u / g.W) % g.H, z = u / (g.W * g.H);
+                for (int k = 0; k < 14; k++) {
+                    const i64 xx = x + DIRS[k][0], yy = y + DIRS[k][1], zz = z + DIRS[k][2];
+                    if (xx < 0 || yy < 0 || zz < 0 || xx >= g.W || yy >= g.H || zz >= g.D) continue;
+                    const i64 w = (zz * g.H + yy) * g.W + xx;
+                    if (d[w] == d[u] && f[w] < f[u] && w > u) {
+                        raise.push_back(u);
+                        break;
+                    }
+                }
+            }
+            if (raise.empty()) break;
+            for (i64 u : raise) d[u] = std::nextafter(d[u], std::numeric_limits<T>::infinity());
+        }
+        return (i64)box.size();
     }
 
     void pin(i64 v) {
@@ -1588,7 +1691,7 @@ using namespace orc;
 // ================================================================================================
 typedef struct {
     int32_t preserve_saddles, preserve_vpath, preserve_geometry, filtration, preserve_connectors,
-        preserve_type, edit_type, max_sweeps;
+        preserve_type, edit_type, max_sweeps, finish;
 } orc_flags;
 
 extern "C" int orc_set_threads(int n) {
@@ -1762,11 +1865,17 @@ extern "C" int orc_paths_f64(const double* f, const double* g, i64 W, i64 H, i64
 // Full correction loop, main_3d.cu:6521-7149.
 // counts8 = {min, saddle1, saddle2, max, ds_vpath, as_vpath, ds_wall, as_wall} at exit.
 // traj (nullable) receives 8 counts per sweep, up to traj_cap sweeps.
-// Returns 0 converged, 1 not converged within max_sweeps.
+// info4 (nullable) receives {stalls, vertices changed by the finisher, finisher radius, 0}.
+// Returns 0 converged, 1 not converged within max_sweeps, 2 stalled (a fixed point with
+// mismatches left; only without the finisher), -2 a capped walk.
+//
+// Stall rule: a sweep is a function of (g, counter, counts, sweep kind). When a sweep changes no
+// value or counter and its (counts, kind) repeats one seen since the last change, every later
+// sweep repeats the same cycle: the reference never leaves it.
 template <class T>
 static int correct_impl(const T* f, T* gfield, i64 W, i64 H, i64 D, double eps, double range, const orc_flags* fl,
                         i64* counts8, int32_t* counter_out, i64* sweeps_out, i64* traj, i64 traj_cap,
-                        double* phase_s) {
+                        double* phase_s, i64* info4) {
     init_dirs();
     State<T> s(W, H, D);
     s.f = f;
@@ -1804,6 +1913,87 @@ static int correct_impl(const T* f, T* gfield, i64 W, i64 H, i64 D, double eps, 
         if (traj && sweeps - 1 < traj_cap)
             for (int k = 0; k < 8; k++) traj[(sweeps - 1) * 8 + k] = c[k];
     };
+    // Re-evaluation after the apply of a critical-point sweep (main_3d.cu:6864-6921).
+    auto eval_cp = [&]() {
+        double tb = now();
+        s.gradient(G_FIELD);
+        double tc = now();
+        ph[0] += tc - tb;
+        s.detect_min();
+        if (fl->preserve_saddles == 1) {
+            s.detect_s1();
+            s.detect_s2();
+        } else {
+            s.l_s1.clear();
+            s.l_s2.clear();
+        }
+        s.detect_max();
+        ph[1] += now() - tc;
+        c[0] = (i64)s.l_min.size();
+        c[1] = (i64)s.l_s1.size();
+        c[2] = (i64)s.l_s2.size();
+        c[3] = (i64)s.l_max.size();
+    };
+    // Re-evaluation after the apply of a separatrix sweep (main_3d.cu:7069-7143).
+    auto eval_path = [&]() {
+        double tc = now();
+        s.gradient(G_FIELD);
+        double td = now();
+        ph[0] += td - tc;
+        s.detect_min();
+        s.detect_s1();
+        s.detect_s2();
+        s.detect_max();
+        c[0] = (i64)s.l_min.size();
+        c[1] = (i64)s.l_s1.size();
+        c[2] = (i64)s.l_s2.size();
+        c[3] = (i64)s.l_max.size();
+        double te = now();
+        ph[1] += te - td;
+        c[4] = c[5] = c[6] = c[7] = 0;
+        if (c[0] == 0 && c[1] == 0 && c[2] == 0 && c[3] == 0) {
+            s.detect_paths();
+            c[4] = (i64)s.l_ds.size();
+            c[5] = (i64)s.l_as.size();
+            c[6] = (i64)s.l_dsw.size();
+            ph[2] += now() - te;
+        }
+    };
+    std::vector<std::vector<i64>> seen;  // (counts, kind) since the last change
+    std::vector<i64> zone;               // edit sets of the unchanged sweeps since the last change
+    i64 stalls = 0, finished = 0;
+    int radius = 0;
+    // After the re-evaluation of a sweep of the given kind (0 critical points, 1 separatrices):
+    // returns true when the run stalls without the finisher.
+    auto after_sweep = [&](i64 moved, int kind) -> bool {
+        std::vector<i64> key(c, c + 8);
+        key.push_back(kind);
+        if (moved > 0) {
+            seen.assign(1, key);
+            zone.clear();
+            return false;
+        }
+        zone.insert(zone.end(), s.last_edits.begin(), s.last_edits.end());
+        bool any = false;
+        for (int k = 0; k < 8; k++) any = any || c[k] != 0;
+        if (!any) return false;
+        if (std::find(seen.begin(), seen.end(), key) == seen.end()) {
+            seen.push_back(key);
+            return false;
+        }
+        stalls++;
+        if (fl->finish != 1) return true;
+        radius++;
+        s.mismatch_vertices(c, zone);  // the zone: stalled edit sets + vertices of mismatched cells
+        finished += s.finish_step(zone, radius);
+        zone.clear();
+        if (kind == 0) eval_cp();
+        else eval_path();
+        key.assign(c, c + 8);
+        key.push_back(kind);
+        seen.assign(1, key);
+        return false;
+    };
     while (c[5] > 0 || c[4] > 0 || c[6] > 0 || c[2] > 0 || c[3] > 0 || c[0] > 0 || c[1] > 0) {
         while (c[3] > 0 || c[2] > 0 || c[0] > 0 || c[1] > 0) {
             if (sweeps >= fl->max_sweeps) {
@@ -1817,27 +2007,15 @@ static int correct_impl(const T* f, T* gfield, i64 W, i64 H, i64 D, double eps, 
             if (c[1] > 0 && fl->preserve_saddles == 1) s.for_list(s.l_s1, [&](i64 id) { s.fix_saddle(id); });
             if (c[2] > 0 && fl->preserve_saddles == 1) s.for_list(s.l_s2, [&](i64 id) { s.fix_2saddle(id); });
             if (c[3] > 0) s.for_list(s.l_max, [&](i64 id) { s.fix_maximum(id); });
-            s.apply();
-            double tb = now();
-            ph[3] += tb - ta;
-            s.gradient(G_FIELD);
-            double tc = now();
-            ph[0] += tc - tb;
-            s.detect_min();
-            if (fl->preserve_saddles == 1) {
-                s.detect_s1();
-                s.detect_s2();
-            } else {
-                s.l_s1.clear();
-                s.l_s2.clear();
-            }
-            s.detect_max();
-            ph[1] += now() - tc;
-            c[0] = (i64)s.l_min.size();
-            c[1] = (i64)s.l_s1.size();
-            c[2] = (i64)s.l_s2.size();
-            c[3] = (i64)s.l_max.size();
+            i64 moved = s.apply();
+            ph[3] += now() - ta;
+            eval_cp();
+            bool stall = after_sweep(moved, 0);
             record();
+            if (stall) {
+                status = 2;
+                goto done;
+            }
         }
         if (fl->preserve_vpath == 0) break;
         if (sweeps >= fl->max_sweeps) {
@@ -1855,31 +2033,15 @@ static int correct_impl(const T* f, T* gfield, i64 W, i64 H, i64 D, double eps, 
             if (nas > 0) s.for_list(s.l_as, [&](i64 id) { s.fix_as_vpath(id); });
             if (nds > 0) s.for_list(s.l_ds, [&](i64 id) { s.fix_ds_vpath(id); });
             if (ndsw > 0 && fl->preserve_connectors == 1) s.for_list(s.l_dsw, [&](i64 id) { s.fix_ds_wall(id); });
-            s.apply();
-            double tc = now();
-            ph[4] += tc - tb;
-            s.gradient(G_FIELD);
-            double td = now();
-            ph[0] += td - tc;
-            s.detect_min();
-            s.detect_s1();
-            s.detect_s2();
-            s.detect_max();
-            c[0] = (i64)s.l_min.size();
-            c[1] = (i64)s.l_s1.size();
-            c[2] = (i64)s.l_s2.size();
-            c[3] = (i64)s.l_max.size();
-            double te = now();
-            ph[1] += te - td;
-            c[4] = c[5] = c[6] = c[7] = 0;
-            if (c[0] == 0 && c[1] == 0 && c[2] == 0 && c[3] == 0) {
-                s.detect_paths();
-                c[4] = (i64)s.l_ds.size();
-                c[5] = (i64)s.l_as.size();
-                c[6] = (i64)s.l_dsw.size();
-                ph[2] += now() - te;
-            }
+            i64 moved = s.apply();
+            ph[4] += now() - tb;
+            eval_path();
+            bool stall = after_sweep(moved, 1);
             record();
+            if (stall) {
+                status = 2;
+                goto done;
+            }
         }
     }
 done:
@@ -1890,16 +2052,26 @@ done:
     if (sweeps_out) *sweeps_out = sweeps;
     if (phase_s)
         for (int k = 0; k < 5; k++) phase_s[k] = ph[k];
+    if (info4) {
+        info4[0] = stalls;
+        info4[1] = finished;
+        info4[2] = radius;
+        info4[3] = 0;
+    }
     if (s.walk_overflow > 0) return -2;
     return status;
 }
 extern "C" int orc_correct_f32(const float* f, float* g, i64 W, i64 H, i64 D, double eps, double range, const orc_flags* fl,
-                    i64* counts8, int32_t* counter_out, i64* sweeps_out, i64* traj, i64 traj_cap, double* phase_s) {
-    return correct_impl<float>(f, g, W, H, D, eps, range, fl, counts8, counter_out, sweeps_out, traj, traj_cap, phase_s);
+                    i64* counts8, int32_t* counter_out, i64* sweeps_out, i64* traj, i64 traj_cap, double* phase_s,
+                    i64* info4) {
+    return correct_impl<float>(f, g, W, H, D, eps, range, fl, counts8, counter_out, sweeps_out, traj, traj_cap, phase_s,
+                               info4);
 }
 extern "C" int orc_correct_f64(const double* f, double* g, i64 W, i64 H, i64 D, double eps, double range, const orc_flags* fl,
-                    i64* counts8, int32_t* counter_out, i64* sweeps_out, i64* traj, i64 traj_cap, double* phase_s) {
-    return correct_impl<double>(f, g, W, H, D, eps, range, fl, counts8, counter_out, sweeps_out, traj, traj_cap, phase_s);
+                    i64* counts8, int32_t* counter_out, i64* sweeps_out, i64* traj, i64 traj_cap, double* phase_s,
+                    i64* info4) {
+    return correct_impl<double>(f, g, W, H, D, eps, range, fl, counts8, counter_out, sweeps_out, traj, traj_cap,
+                                phase_s, info4);
 }
 
 

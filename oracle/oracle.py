@@ -21,7 +21,8 @@ P = ctypes.c_void_p
 
 
 class OrcFlags(ctypes.Structure):
-    # Field order of the reference's argv flags, main_3d.cu:6264-6271, plus a sweep cap.
+    # Field order of the reference's argv flags, main_3d.cu:6264-6271, plus a sweep cap and the
+    # finisher switch (restore f around a stalled edit set; no counterpart in the reference).
     _fields_ = [
         ("preserve_saddles", ctypes.c_int32),
         ("preserve_vpath", ctypes.c_int32),
@@ -31,6 +32,7 @@ class OrcFlags(ctypes.Structure):
         ("preserve_type", ctypes.c_int32),
         ("edit_type", ctypes.c_int32),
         ("max_sweeps", ctypes.c_int32),
+        ("finish", ctypes.c_int32),
     ]
 
 
@@ -67,7 +69,7 @@ def lib():
                                                       ctypes.c_double, P, P, P]
             getattr(L, "orc_paths_" + t).argtypes = [P, P, I64, I64, I64, P, P, P, P, P, P, P, P]
             getattr(L, "orc_correct_" + t).argtypes = [P, P, I64, I64, I64, ctypes.c_double,
-                                                      ctypes.c_double, P, P, P, P, P, I64, P]
+                                                      ctypes.c_double, P, P, P, P, P, I64, P, P]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         _LIB = L
     return _LIB
@@ -132,9 +134,9 @@ def cell_tetra(dims, t, literal=True):
 
 
 def flags(preserve_saddles=1, preserve_vpath=0, preserve_geometry=0, filtration=0,
-          preserve_connectors=0, preserve_type=0, edit_type=0, max_sweeps=100000):
+          preserve_connectors=0, preserve_type=0, edit_type=0, max_sweeps=100000, finish=0):
     return OrcFlags(preserve_saddles, preserve_vpath, preserve_geometry, filtration,
-                    preserve_connectors, preserve_type, edit_type, max_sweeps)
+                    preserve_connectors, preserve_type, edit_type, max_sweeps, finish)
 
 
 def gradient(field, literal=False):
@@ -188,7 +190,10 @@ def paths(f, g, fl):
 
 
 def correct(f, g, eps, rng, fl=None, traj_cap=4096):
-    """Run the reference correction loop. Returns dict(g, counts, counter, sweeps, status, traj)."""
+    """Run the reference correction loop.
+
+    Returns dict(g, counts, counter, sweeps, status, traj, phases, stalls, finished, radius); status
+    0 converged, 1 max_sweeps reached, 2 stalled (fixed point with mismatches left), -2 capped walk."""
     f = np.ascontiguousarray(f)
     g = np.array(g, dtype=f.dtype, copy=True, order="C")
     D, H, W = f.shape
@@ -198,11 +203,13 @@ def correct(f, g, eps, rng, fl=None, traj_cap=4096):
     sweeps = np.zeros(1, np.int64)
     traj = np.zeros(traj_cap * 8, np.int64)
     phases = np.zeros(8, np.float64)
+    info = np.zeros(4, np.int64)
     rc = getattr(lib(), "orc_correct_" + _tag(f))(_p(f), _p(g), W, H, D, float(eps), float(rng),
                                                    ctypes.byref(fl), _p(c8), _p(counter),
-                                                   _p(sweeps), _p(traj), traj_cap, _p(phases))
+                                                   _p(sweeps), _p(traj), traj_cap, _p(phases), _p(info))
     ns = int(sweeps[0])
     return dict(g=g, counts=tuple(int(x) for x in c8), counter=counter.reshape(f.shape),
                 sweeps=ns, status=rc, traj=traj[:min(ns, traj_cap) * 8].reshape(-1, 8),
                 phases=dict(zip(("gradient_pairing", "get_f_cp", "get_f_path", "fix_f_cp",
-                                 "fix_f_vpath"), phases[:5].tolist())))
+                                 "fix_f_vpath"), phases[:5].tolist())),
+                stalls=int(info[0]), finished=int(info[1]), radius=int(info[2]))

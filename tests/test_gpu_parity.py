@@ -58,10 +58,16 @@ FLAG_CASES = [
     ("edit_type1", dict(edit_type=1, preserve_vpath=1, preserve_geometry=1)),
     ("filtration", dict(filtration=1)),
     ("preserve_type", dict(preserve_type=1)),
-    # without saddle repairs these inputs do not converge (the reference has no sweep cap and
-    # would loop forever); both sides must stop at the cap in the same state
+    # without saddle repairs these inputs reach a fixed point with maxima left (the reference
+    # loops forever there); both sides must report the stall in the same state
     ("no_saddles", dict(preserve_saddles=0, max_sweeps=300)),
+    # the same stalls resolved by the finisher: restored boxes, frozen counters, zero mismatches
+    ("no_saddles_finish", dict(preserve_saddles=0, max_sweeps=300, finish=1)),
+    # runs that never stall are unchanged by the finisher
+    ("geometry_finish", dict(preserve_vpath=1, preserve_geometry=1, finish=1)),
 ]
+# expected status per case (0 converged, 2 stalled)
+EXPECT = {"no_saddles": 2}
 
 
 def _compare_run(oracle, dev, f, g, eps, rng, fl):
@@ -85,8 +91,10 @@ def test_correction_bit_exact(oracle, dev, name, fl):
         g = fields.quantize(f, eps)
         print("correct", name, shape, seed, flush=True)
         got = _compare_run(oracle, dev, f, g, eps, rng, fl)
-        if "max_sweeps" not in fl:
+        assert got.status == EXPECT.get(name, 0)
+        if got.status == 0:
             assert got.counts == (0,) * 8
+            assert np.abs(got.g.astype(np.float64) - f).max() <= eps
 
 
 def test_correction_bit_exact_f64(oracle, dev):

@@ -219,3 +219,123 @@ def test_snap_is_tightest_float_within_eps(oracle):
     assert (fs - x <= eps).all()
     below = np.nextafter(r["g"][snapped], np.float32(-np.inf)).astype(np.float64)
     assert (fs - below > eps).all()
+
+
+def _snap_f32(f, eps):
+    """Smallest float32 x with f - x <= eps, computed with numpy."""
+    x = (f.astype(np.float64) - eps).astype(np.float32)
+    up = f.astype(np.float64) - x.astype(np.float64) > eps
+    x[up] = np.nextafter(x[up], np.float32(np.inf))
+    return x
+
+
+# (seed, shape, rel, bumps, noise) whose runs without saddle repairs stall; the first two need
+# one finisher step, the third three.
+STALLS = [(2, (10, 10, 10), 5e-3, 3, 0.02), (4, (14, 10, 12), 2e-2, 5, 0.01), (2, (14, 10, 12), 2e-2, 5, 0.01)]
+
+
+def _stall_input(seed, shape, rel, bumps, noise):
+    f = fields.gaussian_mixture(shape, bumps, seed=seed, noise=noise)
+    eps, rng = fields.eps_from_rel(f, rel)
+    return f, fields.quantize(f, eps), eps, rng
+
+
+@pytest.mark.parametrize("case", STALLS)
+def test_stall_is_a_fixed_point(oracle, case):
+    """A stalled run is a true fixed point of the reference loop: the next sweep's edit set holds
+    only vertices with counter 6 already at the float32 snap of f, which the edit rule of
+    main_3d.cu:1122-1164 snaps again to the same value, so every later sweep repeats."""
+    f, g, eps, rng = _stall_input(*case)
+    fl = oracle.flags(preserve_saddles=0, max_sweeps=300)
+    r = oracle.correct(f, g, eps, rng, fl)
+    assert r["status"] == 2 and r["stalls"] == 1 and r["finished"] == 0
+    assert any(r["counts"])
+    tg, c4 = oracle.targets(f, r["g"], eps, rng, fl)
+    assert (c4[0], c4[3]) == (r["counts"][0], r["counts"][3])
+    assert (r["counter"][tg] == 6).all()
+    assert (r["g"][tg] == _snap_f32(f, eps)[tg]).all()
+    # the sweep cap never triggers first: the uncapped reference would run forever from here
+    r2 = oracle.correct(f, g, eps, rng, oracle.flags(preserve_saddles=0, max_sweeps=r["sweeps"] + 50))
+    assert r2["status"] == 2 and r2["sweeps"] == r["sweeps"]
+
+
+def _mismatch_vertices(oracle, f, g):
+    """Vertices of the cells critical in exactly one of f, g (minima and maxima only)."""
+    A, B = oracle.gradient(f), oracle.gradient(g)
+    dims = (f.shape[2], f.shape[1], f.shape[0])
+    out = set(np.nonzero((A[0] == -1) != (B[0] == -1))[0].tolist())
+    for q in np.nonzero((A[3] == -1) != (B[3] == -1))[0]:
+        out.update(oracle.tet_verts(dims, int(q)))
+    return out
+
+
+def _tie_flips(f, g, where):
+    """Vertices u in where with a 14-neighbour w such that g(w) == g(u), f(w) < f(u) and w > u:
+    pairs whose float tie the index tie-break orders against f."""
+    D, H, W = f.shape
+    ff, gg = f.reshape(-1), g.reshape(-1)
+    out = []
+    for u in np.nonzero(where.reshape(-1))[0]:
+        z, y, x = u // (W * H), (u // W) % H, u % W
+        for dx, dy, dz in DIRS:
+            xx, yy, zz = x + dx, y + dy, z + dz
+            if 0 <= xx < W and 0 <= yy < H and 0 <= zz < D:
+                w = (zz * H + yy) * W + xx
+                if gg[w] == gg[u] and ff[w] < ff[u] and w > u:
+                    out.append(int(u))
+    return out
+
+
+@pytest.mark.parametrize("case", STALLS[:2])
+def test_finisher_snaps_the_box_around_the_stall(oracle, case):
+    """One finisher step snaps and freezes the Chebyshev radius-1 boxes around the zone (the
+    stalled edit set and the vertices of the mismatched cells, an independent scipy dilation),
+    leaves no float tie ordered against f inside them, and the run then converges."""
+    from scipy.ndimage import binary_dilation
+
+    f, g, eps, rng = _stall_input(*case)
+    r0 = oracle.correct(f, g, eps, rng, oracle.flags(preserve_saddles=0, max_sweeps=300))
+    zone, _ = oracle.targets(f, r0["g"], eps, rng, oracle.flags(preserve_saddles=0))
+    zone = zone.reshape(-1).copy()
+    zone[list(_mismatch_vertices(oracle, f, r0["g"]))] = True
+    box = binary_dilation(zone.reshape(f.shape), structure=np.ones((3, 3, 3), bool))
+    snap = _snap_f32(f, eps)
+    r = oracle.correct(f, g, eps, rng, oracle.flags(preserve_saddles=0, max_sweeps=300, finish=1))
+    assert r["status"] == 0 and r["counts"] == (0,) * 8
+    assert r["stalls"] == 1 and r["radius"] == 1
+    assert r["finished"] == box.sum()
+    assert ((r["counter"] == 7) == box).all()
+    gb, sb = r["g"][box], snap[box]
+    assert (gb >= sb).all() and (np.abs(gb.astype(np.float64) - f[box]) <= eps).all()
+    raised = gb != sb  # only one-ulp steps above the snap, and only to break ties
+    assert (np.nextafter(sb[raised], np.float32(np.inf)) <= gb[raised]).all()
+    assert _tie_flips(f, r["g"], box) == []
+    n0 = r0["sweeps"]
+    assert (r["traj"][: n0 - 1] == r0["traj"][: n0 - 1]).all()  # the same run up to the stall
+
+
+@pytest.mark.parametrize("case", STALLS)
+def test_finisher_end_state(oracle, case):
+    f, g, eps, rng = _stall_input(*case)
+    r = oracle.correct(f, g, eps, rng, oracle.flags(preserve_saddles=0, max_sweeps=300, finish=1))
+    assert r["status"] == 0 and r["counts"] == (0,) * 8
+    assert r["stalls"] == r["radius"] >= 1
+    frozen = r["counter"] == 7
+    assert 0 < frozen.sum() <= r["finished"] < f.size  # local boxes; nothing restored to f
+    assert (r["g"][~frozen] <= g[~frozen]).all()  # outside the boxes every edit lowers g
+    assert (np.abs(r["g"].astype(np.float64) - f) <= eps).all()
+    assert _tie_flips(f, r["g"], frozen) == []
+    crit = _crit_sets(oracle.gradient(r["g"]))
+    want = _crit_sets(oracle.gradient(f))
+    assert (crit[0], crit[3]) == (want[0], want[3])  # minima and maxima (saddles are not repaired)
+
+
+def test_finisher_leaves_converging_runs_unchanged(oracle):
+    f = fields.gaussian_mixture((12, 10, 9), 5, seed=11, noise=0.02)
+    eps, rng = fields.eps_from_rel(f, 2e-2)
+    g = fields.quantize(f, eps)
+    fl = dict(preserve_vpath=1, preserve_geometry=1)
+    a = oracle.correct(f, g, eps, rng, oracle.flags(**fl))
+    b = oracle.correct(f, g, eps, rng, oracle.flags(**fl, finish=1))
+    assert a["status"] == b["status"] == 0 and b["stalls"] == 0
+    assert a["g"].tobytes() == b["g"].tobytes() and (a["counter"] == b["counter"]).all()
