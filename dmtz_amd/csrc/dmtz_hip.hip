@@ -83,6 +83,12 @@ __device__ __forceinline__ void wave_add(ull* ctr, bool x) {
     if (__lane_id() == 0 && b) atomicAdd(ctr, (ull)__popcll(b));
 }
 
+// f-critical status of the 75 local cells of a vertex (mismatch.h: local_crit).
+struct __align__(16) CritMask {
+    uint64_t lo;
+    uint32_t hi, pad;
+};
+
 struct Lists4 {
     uint64_t* l[4];
     ull cap;
@@ -146,16 +152,28 @@ __global__ __launch_bounds__(BLOCK) void k_code_list(const T* fld, Dims d, Count
     encode_star(ls, code[v]);
 }
 
-// Mismatch detection from the star codes of f and g.
+// f-critical status of the local cells of every vertex (once per run).
 template <class T>
-__global__ __launch_bounds__(BLOCK) void k_detect(const T* f, const StarCode* cf, const T* g, const StarCode* cg,
-                                                  Dims d, Counts c, Lists4 out, ull* ctr) {
+__global__ __launch_bounds__(BLOCK) void k_critf(const T* f, const StarCode* cf, Dims d, Counts c, CritMask* out) {
+    const i64 v = (i64)blockIdx.x * BLOCK + threadIdx.x;
+    if (v < c.N) {
+        FieldGrad<T> F(f, cf, d, c);
+        CritMask m;
+        local_crit(F, v, m.lo, m.hi);
+        m.pad = 0;
+        out[v] = m;
+    }
+}
+
+// Dense mismatch detection: every vertex lists the mismatched cells it owns in g.
+__global__ __launch_bounds__(BLOCK) void k_detect(const StarCode* cg, const CritMask* crit_f, i64 n, Lists4 out,
+                                                  ull* ctr) {
     const i64 v = (i64)blockIdx.x * BLOCK + threadIdx.x;
     uint64_t lo = 0;
     uint32_t hi = 0;
-    if (v < c.N) {
-        FieldGrad<T> F(f, cf, d, c), G(g, cg, d, c);
-        vertex_mismatch(F, G, v, lo, hi);
+    if (v < n) {
+        const CritMask m = crit_f[v];
+        owner_mismatch(cg[v], m.lo, m.hi, lo, hi);
     }
     emit_records(v, lo, hi, out, ctr);
 }
@@ -630,6 +648,7 @@ struct dmtz_ctx {
     uint32_t* d_dirty = nullptr;    // vertices whose g star code must be re-evaluated
     StarCode* d_code_f = nullptr;   // star codes of f and g
     StarCode* d_code_g = nullptr;
+    CritMask* d_critf = nullptr;    // f-critical status of every vertex's local cells
     uint64_t* d_rec = nullptr;    // 4 mismatch record lists of rec_cap entries
     uint64_t* d_fc = nullptr;     // 4 f-critical lists of fc_cap entries
     int64_t* d_path = nullptr;    // ds id/rec, as id/rec (fc_cap each)
@@ -663,10 +682,12 @@ void free_bufs(dmtz_ctx* ctx) {
     (void)hipFree(ctx->d_dirty);
     (void)hipFree(ctx->d_code_f);
     (void)hipFree(ctx->d_code_g);
+    (void)hipFree(ctx->d_critf);
     ctx->d_f = ctx->d_g = nullptr;
     ctx->d_counter = nullptr;
     ctx->d_stamp = ctx->d_dstamp = ctx->d_targets = ctx->d_changed = ctx->d_dirty = nullptr;
     ctx->d_code_f = ctx->d_code_g = nullptr;
+    ctx->d_critf = nullptr;
     ctx->field_bytes = 0;
     ctx->cap_n = 0;
 }
@@ -710,6 +731,7 @@ int ensure(dmtz_ctx* ctx, i64 N, size_t elem, bool want_fields) {
         CK(hipMalloc(&ctx->d_dirty, sizeof(uint32_t) * n));
         CK(hipMalloc(&ctx->d_code_f, sizeof(StarCode) * n));
         CK(hipMalloc(&ctx->d_code_g, sizeof(StarCode) * n));
+        CK(hipMalloc(&ctx->d_critf, sizeof(CritMask) * n));
         ctx->cap_n = N;
         ctx->epoch = 0;
     }
@@ -835,8 +857,8 @@ struct Driver {
     int detect_once() {
         CK(hipMemsetAsync(ctx->d_ctr + K_MIN, 0, sizeof(ull) * 4, ctx->stream));
         CK(hipEventRecord(ctx->ev0, ctx->stream));
-        k_detect<T><<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(f, ctx->d_code_f, g, ctx->d_code_g, d, c, rec_lists(ctx),
-                                                              ctx->d_ctr + K_MIN);
+        k_detect<<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(ctx->d_code_g, ctx->d_critf, c.N, rec_lists(ctx),
+                                                           ctx->d_ctr + K_MIN);
         CK(hipGetLastError());
         CK(hipEventRecord(ctx->ev1, ctx->stream));
         int rc = sync_ctr(ctx);
@@ -1007,6 +1029,16 @@ struct Driver {
         return 0;
     }
 
+    // Star codes of f and the f-critical status of every vertex's local cells (the gradient of
+    // f, main_3d.cu:6539-6564, done once per run).
+    int f_codes() {
+        int rc = codes(f, ctx->d_code_f);
+        if (rc) return rc;
+        k_critf<T><<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(f, ctx->d_code_f, d, c, ctx->d_critf);
+        CK(hipGetLastError());
+        return 0;
+    }
+
     // Star codes of every vertex of a field.
     int codes(const T* fld, StarCode* out) {
         k_code<T><<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(fld, d, c, out);
@@ -1136,7 +1168,7 @@ struct Driver {
         CK(hipMemsetAsync(ctx->d_ctr, 0, sizeof(ull) * K_NCTR, ctx->stream));
         ctx->dirty_total = 0;
         auto t0 = Clock::now();
-        int rc = codes(f, ctx->d_code_f);  // gradient of f, main_3d.cu:6539-6564
+        int rc = f_codes();  // gradient of f, main_3d.cu:6539-6564
         if (rc) return rc;
         if (fl.filtration == 1) {
             rc = filtration();
@@ -1421,7 +1453,7 @@ int targets_host(dmtz_ctx* ctx, const T* f, const T* g, i64 nx, i64 ny, i64 nz, 
     CK(hipMemsetAsync(ctx->d_counter, 1, (size_t)c.N, ctx->stream));
     CK(hipMemsetAsync(ctx->d_ctr, 0, sizeof(ull) * K_NCTR, ctx->stream));
     Driver<T> drv{ctx, (const T*)ctx->d_f, (T*)ctx->d_g, d, c, eps, range, *flags};
-    rc = drv.codes((const T*)ctx->d_f, ctx->d_code_f);
+    rc = drv.f_codes();
     if (rc) return rc;
     rc = drv.codes((const T*)ctx->d_g, ctx->d_code_g);
     if (rc) return rc;
@@ -1473,7 +1505,7 @@ int paths_host(dmtz_ctx* ctx, const T* f, const T* g, i64 nx, i64 ny, i64 nz, co
     dmtz_flags fl = *flags;
     fl.preserve_vpath = 1;
     Driver<T> drv{ctx, (const T*)ctx->d_f, (T*)ctx->d_g, d, c, 0.0, 0.0, fl};
-    rc = drv.codes((const T*)ctx->d_f, ctx->d_code_f);
+    rc = drv.f_codes();
     if (rc) return rc;
     rc = drv.codes((const T*)ctx->d_g, ctx->d_code_g);
     if (rc) return rc;

@@ -2,14 +2,10 @@
 // from per-vertex lower stars.
 //
 // The reference tests crit_f(cell) XOR crit_g(cell) for every simplex. A cell's status in a field
-// is decided in the lower star of its highest vertex in that field, and that owner can differ
-// between f and g. Per vertex v:
-//   * cells owned by v in both fields: XOR of the two local critical bits;
-//   * cells owned by v in f only and critical in f: mismatched unless g's pairing lookup
-//     (evaluated at the g-owner) says critical;
-//   * cells owned by v in g only and critical in g: mismatched unless f's lookup says critical.
-// A mismatched cell is thus reported exactly once: by its common owner, or by the owner in the
-// field where it is critical.
+// is decided in the lower star of its highest vertex in that field (its owner), and the owners in
+// f and g can differ. f does not change during the correction, so crit_f is evaluated once for
+// every local cell of every vertex (local_crit); each sweep then judges every cell at its owner in
+// g from the g star code alone (owner_mismatch), with no lookups.
 #pragma once
 #include "gradient.h"
 
@@ -52,37 +48,59 @@ DMTZ_HD bool is_crit_lookup(const FieldGrad<T>& X, int type, i64 id) {
     }
 }
 
-// Mismatched local cells of vertex v: (lo, hi) bit masks over the 75 local cells, from the
-// star codes of v in both fields.
+// Existing local cells of vertex v (every cell whose vertices lie in the grid), as (lo, hi).
+DMTZ_HD void local_cells_exist(const Dims& d, i64 v, uint64_t& lo, uint32_t& hi) {
+    i64 x, y, z;
+    vxyz(d, v, x, y, z);
+    if (x > 0 && y > 0 && z > 0 && x < d.W - 1 && y < d.H - 1 && z < d.D - 1) {
+        lo = ~0ull >> (64 - LC_TET);
+        hi = (1u << (LC_COUNT - LC_TET)) - 1u;
+        return;
+    }
+    uint32_t s = 0;  // neighbour slots inside the grid
+    for (int k = 0; k < 14; k++)
+        if (inb(d, x + DIR[k][0], y + DIR[k][1], z + DIR[k][2])) s |= 1u << k;
+    lo = 1ull | ((uint64_t)s << LC_EDGE);
+    for (int t = 0; t < 36; t++)
+        if ((s >> ST.tri_i[t] & 1u) && (s >> ST.tri_j[t] & 1u)) lo |= 1ull << (LC_TRI + t);
+    hi = 0;
+    for (int q = 0; q < 24; q++)
+        if ((s >> ST.tet_s[q][0] & 1u) && (s >> ST.tet_s[q][1] & 1u) && (s >> ST.tet_s[q][2] & 1u)) hi |= 1u << q;
+}
+
+// Critical status in field F of every existing local cell of v: the bits of the cells v owns in
+// F come from v's star code, the others from a pairing lookup at their owner. F is fixed during
+// the correction, so this is evaluated once per run for f.
 template <class T>
-DMTZ_HD void vertex_mismatch(const FieldGrad<T>& F, const FieldGrad<T>& G, i64 v, uint64_t& mlo, uint32_t& mhi) {
-    uint64_t cf_lo, cg_lo, if_lo, ig_lo;
-    uint32_t cf_hi, cg_hi, if_hi, ig_hi;
-    code_masks(F.code[v], cf_lo, cf_hi, if_lo, if_hi);
-    code_masks(G.code[v], cg_lo, cg_hi, ig_lo, ig_hi);
-    mlo = (cf_lo ^ cg_lo) & if_lo & ig_lo;
-    mhi = (cf_hi ^ cg_hi) & if_hi & ig_hi;
-    uint64_t xf_lo = cf_lo & ~ig_lo, xg_lo = cg_lo & ~if_lo;
-    uint32_t xf_hi = cf_hi & ~ig_hi, xg_hi = cg_hi & ~if_hi;
-    // cross-owner cells, resolved by a lookup in the other field
-    while (xf_lo | xf_hi) {
-        int lc = xf_lo ? __builtin_ctzll(xf_lo) : 51 + __builtin_ctz(xf_hi);
-        if (xf_lo) xf_lo &= xf_lo - 1;
-        else xf_hi &= xf_hi - 1;
-        if (!is_crit_lookup(G, lc_type(lc), local_cell_id(F.d, F.c, v, lc))) {
-            if (lc < 51) mlo |= 1ull << lc;
-            else mhi |= 1u << (lc - 51);
+DMTZ_HD void local_crit(const FieldGrad<T>& F, i64 v, uint64_t& lo, uint32_t& hi) {
+    uint64_t c_lo, i_lo, e_lo;
+    uint32_t c_hi, i_hi, e_hi;
+    code_masks(F.code[v], c_lo, c_hi, i_lo, i_hi);
+    local_cells_exist(F.d, v, e_lo, e_hi);
+    lo = c_lo & i_lo;
+    hi = c_hi & i_hi;
+    uint64_t x_lo = e_lo & ~i_lo;
+    uint32_t x_hi = e_hi & ~i_hi;
+    while (x_lo | x_hi) {
+        int lc = x_lo ? __builtin_ctzll(x_lo) : 51 + __builtin_ctz(x_hi);
+        if (x_lo) x_lo &= x_lo - 1;
+        else x_hi &= x_hi - 1;
+        if (is_crit_lookup(F, lc_type(lc), local_cell_id(F.d, F.c, v, lc))) {
+            if (lc < 51) lo |= 1ull << lc;
+            else hi |= 1u << (lc - 51);
         }
     }
-    while (xg_lo | xg_hi) {
-        int lc = xg_lo ? __builtin_ctzll(xg_lo) : 51 + __builtin_ctz(xg_hi);
-        if (xg_lo) xg_lo &= xg_lo - 1;
-        else xg_hi &= xg_hi - 1;
-        if (!is_crit_lookup(F, lc_type(lc), local_cell_id(F.d, F.c, v, lc))) {
-            if (lc < 51) mlo |= 1ull << lc;
-            else mhi |= 1u << (lc - 51);
-        }
-    }
+}
+
+// Mismatched cells owned by v in g (main_3d.cu:2492-2550: crit_f XOR crit_g per cell), from v's
+// g star code and the f-critical mask of v's local cells. Every cell has exactly one owner in g,
+// so the union over all vertices lists each mismatched cell once.
+DMTZ_HD void owner_mismatch(const StarCode& cg, uint64_t cf_lo, uint32_t cf_hi, uint64_t& mlo, uint32_t& mhi) {
+    uint64_t c_lo, i_lo;
+    uint32_t c_hi, i_hi;
+    code_masks(cg, c_lo, c_hi, i_lo, i_hi);
+    mlo = (c_lo ^ cf_lo) & i_lo;
+    mhi = (c_hi ^ cf_hi) & i_hi;
 }
 
 }  // namespace dmtz

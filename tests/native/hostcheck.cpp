@@ -75,11 +75,12 @@ static int targets_impl(const T* f, const T* g, i64 W, i64 H, i64 D, double eps,
     Repair<T, VecSink> R(f, cf.data(), g, cg.data(), d, c, counter.data(), eps, range, edit_type, VecSink{&tg});
     for (int k = 0; k < 4; k++) counts4[k] = 0;
     std::vector<std::pair<int, i64>> recs;
-    FieldGrad<T> Fg(f, cf.data(), d, c), Gg(g, cg.data(), d, c);
+    FieldGrad<T> Fg(f, cf.data(), d, c);
     for (i64 v = 0; v < c.N; v++) {
-        uint64_t lo;
-        uint32_t hi;
-        vertex_mismatch(Fg, Gg, v, lo, hi);
+        uint64_t flo, lo;
+        uint32_t fhi, hi;
+        local_crit(Fg, v, flo, fhi);
+        owner_mismatch(cg[v], flo, fhi, lo, hi);
         for (int lc = 0; lc < LC_COUNT; lc++) {
             if (!lc_bit(lo, hi, lc)) continue;
             int ty = lc_type(lc);
