@@ -27,6 +27,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(workload, seed):
+    """HBM bytes per k_detect launch from the committed rocprofv3 --pmc summary of this same
+    workload (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the counter's reading on a plain
+    copy of known size), or None when no summary for this workload exists."""
+    path = os.path.join(ROOT, "profiles", "k_detect_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        t = json.load(fh)
+    if t.get("workload") != workload or t.get("seed") != seed:
+        return None
+    return int(t["traffic_bytes_per_launch"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -42,6 +56,9 @@ def main():
     ap.add_argument("--dtype", default="f32", choices=("f32", "f64"),
                     help="arithmetic of the correction (f64 = the reference's double precision)")
     ap.add_argument("--max-sweeps", type=int, default=5000)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB four times (the known-size "
+                         "reference for scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--finish", type=int, default=1,
                     help="1: resolve stalls (fixed points where the reference loops forever) by snapping "
                          "growing boxes around the stalled edit set")
@@ -53,6 +70,17 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 4, "copies": 4}))
+        return
 
     import dmtz_amd
     from dmtz_amd import fields
@@ -103,10 +131,13 @@ def main():
     avg_ms = ks["ms"] / launches
     bytes_per_launch = ks["bytes"] / launches
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    workload = "gaussian-mixture %d^3 %s, %d bumps, eps=%g*range, %s" % (
+        n, "float32" if args.dtype == "f32" else "float64", args.bumps, args.rel,
+        "critical cells + separatrix geometry" if args.geometry else "critical cells")
     roofline = dict(bound="hbm", kernel="k_detect (dense mismatch detection)", achieved=round(achieved, 2),
-                    peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None,
-                    bytes_per_launch=int(bytes_per_launch), avg_launch_ms=round(avg_ms, 4),
-                    launches=int(ks["launches"]))
+                    peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 5),
+                    traffic=measured_traffic(workload, args.seed), bytes_per_launch=int(bytes_per_launch),
+                    avg_launch_ms=round(avg_ms, 4), launches=int(ks["launches"]))
 
     cpu = None
     if args.cpu_sample > 0:
@@ -140,8 +171,7 @@ def main():
         "vs_baseline": None,
         "dtype": args.dtype,
         "data": "synthetic",
-        "config": {"workload": "gaussian-mixture %d^3 float32, %d bumps, eps=%g*range, %s" % (
-            n, args.bumps, args.rel, "critical cells + separatrix geometry" if args.geometry else "critical cells"),
+        "config": {"workload": workload,
                    "size": [n, n, n], "bumps": args.bumps, "rel_eps": args.rel, "seed": args.seed,
                    "sweeps": res.sweeps, "form": "plain", "finish": args.finish,
                    "stalls": res.info.get("stalls"), "finisher_vertices": res.info.get("finished")},

@@ -1,21 +1,28 @@
 // MI355X (gfx950) implementation of the DMTz topology-correction path behind include/dmtz.h.
 //
-// Device work:
-//   k_detect      per-vertex lower stars of f and g -> mismatch records, 4 typed lists
-//                 (get_false_* main_3d.cu:2492-2550 over the gradient of main_3d.cu:1930-2316)
+// Device work (one HIP stream per context):
+//   k_code        lower star of every vertex -> 32-byte star code (the gradient of
+//                 main_3d.cu:1930-2316 restricted to each vertex's lower star)
+//   k_critf       f-critical status of the 75 local cells of every vertex (once per run)
+//   k_detect      mismatch records from the g star codes and the f-critical masks
+//                 (get_false_* main_3d.cu:2492-2550), 4 typed lists
 //   k_fix_cp      repair rules over a record list -> edit set (main_3d.cu:2554-4303)
 //   k_apply       epsilon-checked edit of the edit set (applyDeltaBuffer1 main_3d.cu:1122-1164)
+//   k_dirty, k_code_list
+//                 re-evaluation of the g star codes around the vertices whose value changed
 //   k_fcrit       f-critical cells (for separatrix checks and filtration)
 //   k_ds_geo, k_as_branch + k_as_combine, k_ds_branch + k_ds_combine, k_labels
 //                 separatrix checks (main_3d.cu:4437-5062)
 //   k_fix_paths   separatrix repairs (main_3d.cu:5135-5295)
 //   k_pin         filtration (main_3d.cu:2405-2490)
-// Host: the sweep loop of main_3d.cu:6784-7149 on one HIP stream; per-sweep counts are read back
-// like the reference's cudaMemcpyFromSymbol calls (main_3d.cu:6918-6921, 6989-6992, 7096-7143).
+//   k_zone_cells, k_zone, k_snap_boxes, k_tie_mark, k_tie_raise, k_freeze_all
+//                 the optional finisher at stalls (no counterpart in the reference)
+// Host: the sweep loop of main_3d.cu:6784-7149; per-sweep counts are read back like the
+// reference's cudaMemcpyFromSymbol calls (main_3d.cu:6918-6921, 6989-6992, 7096-7143).
 //
 // Records: a mismatched (or f-critical) cell is stored as (vertex << 7) | local cell, where the
-// vertex is the cell's owner in the emitting field and the local cell numbers the 75 cells of a
-// lower star (lower_star.h). Lists are filled with wave-aggregated atomics: the set is
+// vertex is the cell's owner in g (in f for f-critical lists) and the local cell numbers the 75
+// cells around a vertex (lower_star.h). Lists are filled with wave-aggregated atomics: the set is
 // deterministic, the order is not, and every consumer is order independent.
 #include <hip/hip_runtime.h>
 

@@ -10,6 +10,17 @@
 // searches) and DIRECT (the same candidate sets enumerated through an adjacency table); tests
 // check that both produce identical arrays.
 //
+// Pins (tests/test_oracle.py): closed-form simplex counts, ID bijections and the clique structure
+// of the implicit complex, cofacet counts, LITERAL == DIRECT gradients, Euler characteristic and
+// matching validity of every gradient, linear fields, invariance under monotone value maps,
+// float/double agreement, end-state invariants of the loop, the float32 snap, the stall fixed
+// point and the finisher's boxes and tie repair.
+// Parity unpinned: the target choices of the repair rules (fix_minimum, fix_saddle, fix_2saddle,
+// fix_maximum, fix_as_vpath, fix_ds_vpath, fix_ds_wall), the separatrix checks (is_false and its
+// cofacet enumeration order, the descending-walk comparison, the connectivity labels) and the
+// walls are pinned only by this restatement, read branch by branch against the cited lines, and
+// by end-state invariants: the reference ships no tests or vectors and cannot be built here.
+//
 // Field type T is float or double. With T = double all value arithmetic is the reference's.
 // With T = float the values are float32, comparisons are exact float32 ordering with index
 // tie-break, and edit arithmetic is done in double then rounded to float (see Arith<float>).
