@@ -493,6 +493,39 @@ struct Repair {
         }
     }
 
+    // The comparison steps of as_walk from tet id, taken on f alone: every tet cur_i of f's
+    // ascending path whose f partner face t_i is not -1, in walk order, up to the step where the
+    // chain ends. as_walk(id) finds the first step with G.pt(cur_i) != t_i (divergent = cur_i,
+    // found for the start tet id), or nothing when all steps agree: f is fixed during the
+    // correction, so these steps are computed once per run. out(i, cur_i, t_i) receives each
+    // step; returns the number of steps (overflow = true past cap steps).
+    template <class Out>
+    DMTZ_HD i64 as_fpath(i64 id, i64 cap, bool& overflow, Out out) const {
+        i64 cur = id, n = 0;
+        bool done = id < 0;
+        while (!done) {
+            const i64 t = F.pt(cur);
+            bool chain = false;
+            i64 nxt = -1;
+            if (t != -1) {
+                out(n, cur, t);
+                n++;
+                const i64 pft = F.pf(t);
+                chain = pft != -1 && F.pt(pft) == t;
+                i64 t1 = -1, t2 = -1;
+                if (chain) cofacets(t, t1, t2);
+                nxt = (t1 == cur) ? t2 : t1;
+            }
+            done = t == -1 || !chain || nxt == cur || nxt == -1;
+            cur = nxt;
+            if (!done && n > cap) {
+                overflow = true;
+                done = true;
+            }
+        }
+        return n;
+    }
+
     // get_paired_id main_3d.cu:4499-4537 on field which (0 = f, 1 = g): the tet reached by
     // following the ascending pairing from tet id.
     DMTZ_HD i64 paired_max(i64 id, int which, i64 cap, bool& overflow) const {

@@ -190,3 +190,55 @@ extern "C" int hc_path_targets_f32(const float* f, const float* g, i64 W, i64 H,
                                    int et, const i64* as_rec, i64 nas, const i64* ds_rec, i64 nds, uint8_t* targets) {
     return path_targets_impl<float>(f, g, W, H, D, eps, range, et, as_rec, nas, ds_rec, nds, targets);
 }
+
+// Ascending geometry check of every (f-critical face, cofacet) branch two ways: Repair::as_walk
+// directly, and as the first step of Repair::as_fpath (computed on f alone) whose g partner face
+// differs. out_walk / out_steps receive (found tet, divergent tet) per branch; returns the
+// number of branches.
+struct StepVec {
+    std::vector<std::pair<i64, i64>>* v;
+    void operator()(i64, i64 cur, i64 t) const { v->push_back({cur, t}); }
+};
+template <class T>
+static i64 as_two_ways_impl(const T* f, const T* g, i64 W, i64 H, i64 D, i64* out_walk, i64* out_steps) {
+    Dims d{W, H, D};
+    Counts c = make_counts(d);
+    std::vector<StarCode> cf = all_codes(f, d, c), cg = all_codes(g, d, c);
+    Repair<T, NoSinkH> R(f, cf.data(), g, cg.data(), d, c, nullptr, 0, 0, 0, NoSinkH{});
+    Repair<T, NoSinkH> RF(f, cf.data(), f, cf.data(), d, c, nullptr, 0, 0, 0, NoSinkH{});
+    FieldGrad<T> G(g, cg.data(), d, c);
+    i64 nb = 0;
+    for (i64 v = 0; v < c.N; v++) {
+        uint64_t lo, ilo;
+        uint32_t hi, ihi;
+        code_masks(cf[v], lo, hi, ilo, ihi);
+        for (int lc = LC_TRI; lc < LC_TET; lc++) {
+            if (!lc_bit(lo & ilo, hi & ihi, lc)) continue;
+            i64 t = local_cell_id(d, c, v, lc), q[2];
+            R.cofacets(t, q[0], q[1]);
+            for (int k = 0; k < 2; k++) {
+                bool ovf = false, found = false;
+                i64 dv = -1;
+                R.as_walk(q[k], c.N, found, dv, ovf);
+                out_walk[2 * nb] = found ? q[k] : -1;
+                out_walk[2 * nb + 1] = dv;
+                std::vector<std::pair<i64, i64>> st;
+                RF.as_fpath(q[k], c.N, ovf, StepVec{&st});
+                i64 fq = -1, fdv = -1;
+                for (auto& p : st)
+                    if (G.pt(p.first) != p.second) {
+                        fq = st[0].first;
+                        fdv = p.first;
+                        break;
+                    }
+                out_steps[2 * nb] = fq;
+                out_steps[2 * nb + 1] = fdv;
+                nb++;
+            }
+        }
+    }
+    return nb;
+}
+extern "C" i64 hc_as_two_ways_f32(const float* f, const float* g, i64 W, i64 H, i64 D, i64* ow, i64* os) {
+    return as_two_ways_impl<float>(f, g, W, H, D, ow, os);
+}

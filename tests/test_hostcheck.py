@@ -86,3 +86,27 @@ def test_repair_edit_set_matches_oracle(hc, oracle, dtype, edit_type):
             fn(_p(f), _p(g), W, H, D, eps, rng, 1, edit_type, _p(tg), _p(c4))
             assert tuple(c4) == wc
             assert (tg.reshape(shape).astype(bool) == want).all()
+
+
+def test_precomputed_ascending_paths_equal_the_walk(hc, oracle):
+    """The ascending geometry check over steps precomputed on f alone (Repair::as_fpath, used by
+    the device's incremental path checks) finds exactly what the direct walk (Repair::as_walk,
+    is_false of main_3d.cu:4437-4497) finds, for every branch."""
+    hc.hc_as_two_ways_f32.argtypes = [P, P, I, I, I, P, P]
+    hc.hc_as_two_ways_f32.restype = I
+    found = 0
+    for seed in range(4):
+        for shape, rel in [((12, 12, 12), 1e-2), ((10, 14, 9), 3e-2), ((16, 16, 16), 5e-3)]:
+            f = fields.gaussian_mixture(shape, 5, seed=seed, noise=0.02)
+            eps, rng = fields.eps_from_rel(f, rel)
+            g0 = fields.quantize(f, eps)
+            g1 = oracle.correct(f, g0, eps, rng, oracle.flags())["g"]  # critical cells only
+            for g in (g0, g1):
+                D, H, W = shape
+                cap = 2 * 36 * W * H * D
+                ow, os_ = np.zeros(cap, np.int64), np.zeros(cap, np.int64)
+                nb = hc.hc_as_two_ways_f32(_p(f), _p(g), W, H, D, _p(ow), _p(os_))
+                assert nb > 0
+                assert (ow[: 2 * nb] == os_[: 2 * nb]).all()
+                found += int((ow[: 2 * nb: 2] >= 0).sum())
+    assert found > 0
