@@ -25,6 +25,7 @@
 // cells around a vertex (lower_star.h). Lists are filled with wave-aggregated atomics: the set is
 // deterministic, the order is not, and every consumer is order independent.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <chrono>
 #include <cstdio>
@@ -90,48 +91,81 @@ __device__ __forceinline__ void wave_add(ull* ctr, bool x) {
     if (__lane_id() == 0 && b) atomicAdd(ctr, (ull)__popcll(b));
 }
 
-// f-critical status of the 75 local cells of a vertex (mismatch.h: local_crit).
-struct __align__(16) CritMask {
-    uint64_t lo;
-    uint32_t hi, pad;
-};
-
 struct Lists4 {
     uint64_t* l[4];
     ull cap;
 };
 
-// Emit the set bits of (lo, hi) as records of vertex v into the four typed lists.
-__device__ __forceinline__ void emit_records(i64 v, uint64_t lo, uint32_t hi, const Lists4& out, ull* ctr) {
-    const uint64_t m1 = (lo >> 1) & 0x3FFFull;
-    const uint64_t m2 = (lo >> 15) & ((1ull << 36) - 1);
-    const int n0 = (int)(lo & 1ull), n1 = __popcll(m1), n2 = __popcll(m2), n3 = __popc(hi);
-    ull b0 = wave_alloc(ctr + 0, n0);
-    ull b1 = wave_alloc(ctr + 1, n1);
-    ull b2 = wave_alloc(ctr + 2, n2);
-    ull b3 = wave_alloc(ctr + 3, n3);
-    const uint64_t key = (uint64_t)v << 7;
-    if (n0 && b0 < out.cap) out.l[0][b0] = key | LC_VERT;
-    uint64_t m = m1;
-    while (m) {
-        int b = __builtin_ctzll(m);
-        m &= m - 1;
-        if (b1 < out.cap) out.l[1][b1] = key | (uint64_t)(LC_EDGE + b);
-        b1++;
+// Dense emitters handle EMIT_V vertices per thread: thread t of a block takes vertices
+// base + k * BLOCK + t (k < EMIT_V), so each load of a wave is contiguous.
+constexpr int EMIT_V = 8;
+constexpr i64 EMIT_TILE = (i64)BLOCK * EMIT_V;
+
+// Append the set bits of the masks of a block's EMIT_TILE vertices to the four typed lists. Space
+// is claimed with one atomic per type per block after an LDS scan of the per-thread counts;
+// blocks with nothing to append return at once. Every thread of the block must call this.
+__device__ __forceinline__ void block_emit(i64 base, const uint64_t (&lo)[EMIT_V], const uint32_t (&hi)[EMIT_V],
+                                           const Lists4& out, ull* ctr) {
+    __shared__ uint32_t wtot[4][BLOCK / 64];
+    __shared__ ull wbase[4][BLOCK / 64];
+    int cnt[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < EMIT_V; k++) {
+        cnt[0] += (int)(lo[k] & 1ull);
+        cnt[1] += __popcll((lo[k] >> 1) & 0x3FFFull);
+        cnt[2] += __popcll((lo[k] >> 15) & ((1ull << 36) - 1));
+        cnt[3] += __popc(hi[k]);
     }
-    m = m2;
-    while (m) {
-        int b = __builtin_ctzll(m);
-        m &= m - 1;
-        if (b2 < out.cap) out.l[2][b2] = key | (uint64_t)(LC_TRI + b);
-        b2++;
+    if (__syncthreads_or(cnt[0] | cnt[1] | cnt[2] | cnt[3]) == 0) return;
+    const int wave = threadIdx.x >> 6;
+    int pre[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        int tot;
+        pre[t] = wave_excl_scan(cnt[t], tot);
+        if (__lane_id() == 0) wtot[t][wave] = (uint32_t)tot;
     }
-    uint32_t h = hi;
-    while (h) {
-        int b = __builtin_ctz(h);
-        h &= h - 1;
-        if (b3 < out.cap) out.l[3][b3] = key | (uint64_t)(LC_TET + b);
-        b3++;
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int t = threadIdx.x;
+        ull sum = 0;
+        for (int w = 0; w < BLOCK / 64; w++) sum += wtot[t][w];
+        ull b = sum ? atomicAdd(ctr + t, sum) : 0;
+        for (int w = 0; w < BLOCK / 64; w++) {
+            wbase[t][w] = b;
+            b += wtot[t][w];
+        }
+    }
+    __syncthreads();
+    ull at[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) at[t] = wbase[t][wave] + (ull)pre[t];
+#pragma unroll
+    for (int k = 0; k < EMIT_V; k++) {
+        const uint64_t key = (uint64_t)(base + (i64)k * BLOCK + threadIdx.x) << 7;
+        if ((lo[k] & 1ull) && at[0] < out.cap) out.l[0][at[0]] = key | LC_VERT;
+        at[0] += lo[k] & 1ull;
+        uint64_t m = (lo[k] >> 1) & 0x3FFFull;
+        while (m) {
+            const int b = __builtin_ctzll(m);
+            m &= m - 1;
+            if (at[1] < out.cap) out.l[1][at[1]] = key | (uint64_t)(LC_EDGE + b);
+            at[1]++;
+        }
+        m = (lo[k] >> 15) & ((1ull << 36) - 1);
+        while (m) {
+            const int b = __builtin_ctzll(m);
+            m &= m - 1;
+            if (at[2] < out.cap) out.l[2][at[2]] = key | (uint64_t)(LC_TRI + b);
+            at[2]++;
+        }
+        uint32_t h = hi[k];
+        while (h) {
+            const int b = __builtin_ctz(h);
+            h &= h - 1;
+            if (at[3] < out.cap) out.l[3][at[3]] = key | (uint64_t)(LC_TET + b);
+            at[3]++;
+        }
     }
 }
 
@@ -139,6 +173,13 @@ __device__ __forceinline__ void emit_records(i64 v, uint64_t lo, uint32_t hi, co
 // Kernels
 // ------------------------------------------------------------------------------------------------
 // Star codes of every vertex (dense) or of the listed vertices.
+// 75-bit mask over the local cells of a vertex: the f-critical status of every local cell
+// (mismatch.h: local_crit), or the mismatched cells a vertex owns in g.
+struct __align__(16) CritMask {
+    uint64_t lo;
+    uint32_t hi, pad;
+};
+
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_code(const T* fld, Dims d, Counts c, StarCode* code) {
     const i64 v = (i64)blockIdx.x * BLOCK + threadIdx.x;
@@ -148,15 +189,35 @@ __global__ __launch_bounds__(BLOCK) void k_code(const T* fld, Dims d, Counts c, 
     encode_star(ls, code[v]);
 }
 
+// g star code of vertex v and its mismatch mask: the cells v owns in g whose critical status in
+// g differs from f's (owner_mismatch, main_3d.cu:2492-2550).
 template <class T>
-__global__ __launch_bounds__(BLOCK) void k_code_list(const T* fld, Dims d, Counts c, StarCode* code,
-                                                     const uint32_t* list, ull n) {
-    const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const i64 v = list[i];
+__device__ __forceinline__ void code_g_at(const T* g, Dims d, i64 v, StarCode* code, const CritMask* crit_f,
+                                          CritMask* mm) {
     LowerStar ls;
-    star_of(fld, d, v, ls);
-    encode_star(ls, code[v]);
+    star_of(g, d, v, ls);
+    StarCode sc;
+    encode_star(ls, sc);
+    code[v] = sc;
+    const CritMask cf = crit_f[v];
+    CritMask m;
+    owner_mismatch(sc, cf.lo, cf.hi, m.lo, m.hi);
+    m.pad = 0;
+    mm[v] = m;
+}
+
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_code_g(const T* g, Dims d, Counts c, StarCode* code,
+                                                  const CritMask* crit_f, CritMask* mm) {
+    const i64 v = (i64)blockIdx.x * BLOCK + threadIdx.x;
+    if (v < c.N) code_g_at(g, d, v, code, crit_f, mm);
+}
+
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_code_list_g(const T* g, Dims d, StarCode* code, const CritMask* crit_f,
+                                                       CritMask* mm, const uint32_t* list, ull n) {
+    const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) code_g_at(g, d, (i64)list[i], code, crit_f, mm);
 }
 
 // f-critical status of the local cells of every vertex (once per run).
@@ -172,34 +233,50 @@ __global__ __launch_bounds__(BLOCK) void k_critf(const T* f, const StarCode* cf,
     }
 }
 
-// Dense mismatch detection: every vertex lists the mismatched cells it owns in g.
-__global__ __launch_bounds__(BLOCK) void k_detect(const StarCode* cg, const CritMask* crit_f, i64 n, Lists4 out,
-                                                  ull* ctr) {
-    const i64 v = (i64)blockIdx.x * BLOCK + threadIdx.x;
-    uint64_t lo = 0;
-    uint32_t hi = 0;
-    if (v < n) {
-        const CritMask m = crit_f[v];
-        owner_mismatch(cg[v], m.lo, m.hi, lo, hi);
+// Dense mismatch detection: every vertex lists the mismatched cells it owns in g, from the
+// per-vertex mismatch masks kept with the g star codes (16 bytes per vertex). Waves without a
+// mismatch skip the list appends.
+__global__ __launch_bounds__(BLOCK) void k_detect(const CritMask* mm, i64 n, Lists4 out, ull* ctr) {
+    const i64 base = (i64)blockIdx.x * EMIT_TILE;
+    uint64_t lo[EMIT_V];
+    uint32_t hi[EMIT_V];
+#pragma unroll
+    for (int k = 0; k < EMIT_V; k++) {
+        const i64 v = base + (i64)k * BLOCK + threadIdx.x;
+        lo[k] = 0;
+        hi[k] = 0;
+        if (v < n) {
+            const CritMask m = mm[v];
+            lo[k] = m.lo;
+            hi[k] = m.hi;
+        }
     }
-    emit_records(v, lo, hi, out, ctr);
+    block_emit(base, lo, hi, out, ctr);
 }
 
 // f-critical cells owned by each vertex in f; type_mask selects which types are listed.
 __global__ __launch_bounds__(BLOCK) void k_fcrit(const StarCode* cf, Counts c, Lists4 out, ull* ctr, int type_mask) {
-    const i64 v = (i64)blockIdx.x * BLOCK + threadIdx.x;
-    uint64_t lo = 0, ilo;
-    uint32_t hi = 0, ihi;
-    if (v < c.N) {
-        code_masks(cf[v], lo, hi, ilo, ihi);
-        uint64_t keep = 0;
-        if (type_mask & 1) keep |= 1ull;
-        if (type_mask & 2) keep |= 0x3FFFull << 1;
-        if (type_mask & 4) keep |= ((1ull << 36) - 1) << 15;
-        lo &= keep;
-        if (!(type_mask & 8)) hi = 0;
+    const i64 base = (i64)blockIdx.x * EMIT_TILE;
+    uint64_t keep = 0;
+    if (type_mask & 1) keep |= 1ull;
+    if (type_mask & 2) keep |= 0x3FFFull << 1;
+    if (type_mask & 4) keep |= ((1ull << 36) - 1) << 15;
+    uint64_t lo[EMIT_V];
+    uint32_t hi[EMIT_V];
+#pragma unroll
+    for (int k = 0; k < EMIT_V; k++) {
+        const i64 v = base + (i64)k * BLOCK + threadIdx.x;
+        lo[k] = 0;
+        hi[k] = 0;
+        if (v < c.N) {
+            uint64_t ilo;
+            uint32_t ihi;
+            code_masks(cf[v], lo[k], hi[k], ilo, ihi);
+            lo[k] &= keep;
+            if (!(type_mask & 8)) hi[k] = 0;
+        }
     }
-    emit_records(v, lo, hi, out, ctr);
+    block_emit(base, lo, hi, out, ctr);
 }
 
 struct TargetSink {
@@ -217,8 +294,9 @@ struct TargetSink {
     }
 };
 
-template <class T>
-__global__ __launch_bounds__(BLOCK) void k_fix_cp(int type, const uint64_t* recs, ull n, const T* f, const StarCode* cf,
+// One kernel per cell type, so each holds only its own repair rule.
+template <class T, int TYPE>
+__global__ __launch_bounds__(BLOCK) void k_fix_cp(const uint64_t* recs, ull n, const T* f, const StarCode* cf,
                                                   const T* g, const StarCode* cg, Dims d, Counts c,
                                                   const uint8_t* counter, double eps, double range, int edit_type,
                                                   TargetSink sink) {
@@ -229,12 +307,10 @@ __global__ __launch_bounds__(BLOCK) void k_fix_cp(int type, const uint64_t* recs
     const int lc = (int)(r & 127u);
     const i64 id = local_cell_id(d, c, v, lc);
     Repair<T, TargetSink> R(f, cf, g, cg, d, c, counter, eps, range, edit_type, sink);
-    switch (type) {
-        case 0: R.fix_minimum(id); break;
-        case 1: R.fix_saddle(id); break;
-        case 2: R.fix_2saddle(id); break;
-        default: R.fix_maximum(id); break;
-    }
+    if (TYPE == 0) R.fix_minimum(id);
+    else if (TYPE == 1) R.fix_saddle(id);
+    else if (TYPE == 2) R.fix_2saddle(id);
+    else R.fix_maximum(id);
 }
 
 // Frozen vertices are never edited. moved counts the edits that changed a value or a counter.
@@ -525,6 +601,97 @@ __global__ __launch_bounds__(BLOCK) void k_as_branch(const uint64_t* fcf, ull n,
     if (ovf) atomicAdd(walk_ctr, 1ull);
 }
 
+// Precomputed ascending paths (geometry mode). A branch j is (f-critical face j / 2, cofacet
+// j % 2); its comparison steps (Repair::as_fpath) are fixed for the whole run.
+struct CountSteps {
+    DMTZ_HD void operator()(i64, i64, i64) const {}
+};
+struct StoreSteps {
+    uint2* p;
+    DMTZ_HD void operator()(i64 i, i64 cur, i64 t) const { p[i] = make_uint2((uint32_t)cur, (uint32_t)t); }
+};
+
+template <class T>
+__device__ i64 branch_tet(const uint64_t* fcf, ull j, const T* f, const StarCode* cf, Dims d, Counts c) {
+    const uint64_t r = fcf[j >> 1];
+    const i64 t = local_cell_id(d, c, (i64)(r >> 7), (int)(r & 127u));
+    Repair<T, NoSink> R(f, cf, f, cf, d, c, nullptr, 0, 0, 0, NoSink{});
+    i64 q0, q1;
+    R.cofacets(t, q0, q1);
+    return (j & 1) ? q1 : q0;
+}
+
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_as_count(const uint64_t* fcf, ull nb, const T* f, const StarCode* cf,
+                                                    Dims d, Counts c, ull* cnt, ull* walk_ctr) {
+    const ull j = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= nb) return;
+    const i64 q = branch_tet(fcf, j, f, cf, d, c);
+    Repair<T, NoSink> R(f, cf, f, cf, d, c, nullptr, 0, 0, 0, NoSink{});
+    bool ovf = false;
+    cnt[j] = (ull)R.as_fpath(q, c.N, ovf, CountSteps{});
+    if (ovf) atomicAdd(walk_ctr, 1ull);
+}
+
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_as_fill(const uint64_t* fcf, ull nb, const T* f, const StarCode* cf,
+                                                   Dims d, Counts c, const ull* off, uint2* steps) {
+    const ull j = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= nb) return;
+    const i64 q = branch_tet(fcf, j, f, cf, d, c);
+    Repair<T, NoSink> R(f, cf, f, cf, d, c, nullptr, 0, 0, 0, NoSink{});
+    bool ovf = false;
+    R.as_fpath(q, c.N, ovf, StoreSteps{steps + off[j]});
+}
+
+// is_false (main_3d.cu:4437-4497) over the precomputed steps: the first step whose g partner
+// face differs from f's. A branch keeps its previous result unless full is set or one of its
+// tets lies in a cube marked dirty since the last check (a tet's g pairing can only change when
+// one of its vertices had its lower star re-evaluated).
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_as_check(ull nb, const ull* off, const uint2* steps, const T* g,
+                                                    const StarCode* cg, Dims d, Counts c, const uint32_t* cubes,
+                                                    int full, int64_t* res) {
+    const ull j = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= nb) return;
+    const ull a = off[j], b = off[j + 1];
+    bool redo = full != 0;
+    for (ull s = a; s < b && !redo; s++) {
+        const uint32_t cube = steps[s].x / 6u;
+        redo = ((cubes[cube >> 5] >> (cube & 31u)) & 1u) != 0;
+    }
+    if (!redo) return;
+    FieldGrad<T> G(g, cg, d, c);
+    i64 found = -1, dv = -1;
+    bool done = a >= b;
+    ull s = a;
+    while (!done) {
+        const uint2 st = steps[s];
+        if (G.pt((i64)st.x) != (i64)st.y) {
+            found = (i64)steps[a].x;
+            dv = (i64)st.x;
+        }
+        s++;
+        done = found >= 0 || s >= b;
+    }
+    res[2 * j] = found;
+    res[2 * j + 1] = dv;
+}
+
+// Mark the (up to 8) cubes that have a dirty vertex as a corner.
+__global__ __launch_bounds__(BLOCK) void k_cube_mark(const uint32_t* dirty, ull n, Dims d, uint32_t* cubes) {
+    const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    i64 x, y, z;
+    vxyz(d, (i64)dirty[i], x, y, z);
+    for (int k = 0; k < 8; k++) {
+        const i64 ax = x - (k & 1), ay = y - ((k >> 1) & 1), az = z - ((k >> 2) & 1);
+        if (ax < 0 || ay < 0 || az < 0 || ax > d.W - 2 || ay > d.H - 2 || az > d.D - 2) continue;
+        const ull cube = (ull)(az * (d.W - 1) * (d.H - 1) + ay * (d.W - 1) + ax);
+        atomicOr(cubes + (cube >> 5), 1u << (cube & 31u));
+    }
+}
+
 __global__ __launch_bounds__(BLOCK) void k_as_combine(const int64_t* res, ull n, PathList out) {
     const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
     bool has = false;
@@ -656,6 +823,7 @@ struct dmtz_ctx {
     StarCode* d_code_f = nullptr;   // star codes of f and g
     StarCode* d_code_g = nullptr;
     CritMask* d_critf = nullptr;    // f-critical status of every vertex's local cells
+    CritMask* d_mm = nullptr;       // mismatched cells each vertex owns in g
     uint64_t* d_rec = nullptr;    // 4 mismatch record lists of rec_cap entries
     uint64_t* d_fc = nullptr;     // 4 f-critical lists of fc_cap entries
     int64_t* d_path = nullptr;    // ds id/rec, as id/rec (fc_cap each)
@@ -674,6 +842,13 @@ struct dmtz_ctx {
     double det_launches = 0, det_ms = 0, det_bytes = 0, det_voxels = 0;
     double dirty_total = 0;  // star codes re-evaluated incrementally in the last correction
     int64_t stalls = 0, finished = 0, radius = 0;  // finisher outcome of the last correction
+    // precomputed ascending separatrix paths of f (geometry mode)
+    ull* d_as_cnt = nullptr;   // comparison steps per branch, then scanned into d_as_off
+    ull* d_as_off = nullptr;
+    uint2* d_as_steps = nullptr;  // (tet, f partner face) per step
+    uint32_t* d_cubes = nullptr;  // dirty-cube bitmap since the last path check
+    void* d_scan_tmp = nullptr;
+    size_t as_cnt_cap = 0, as_off_cap = 0, as_steps_cap = 0, cubes_cap = 0, scan_tmp_cap = 0;
 };
 
 namespace {
@@ -690,11 +865,12 @@ void free_bufs(dmtz_ctx* ctx) {
     (void)hipFree(ctx->d_code_f);
     (void)hipFree(ctx->d_code_g);
     (void)hipFree(ctx->d_critf);
+    (void)hipFree(ctx->d_mm);
     ctx->d_f = ctx->d_g = nullptr;
     ctx->d_counter = nullptr;
     ctx->d_stamp = ctx->d_dstamp = ctx->d_targets = ctx->d_changed = ctx->d_dirty = nullptr;
     ctx->d_code_f = ctx->d_code_g = nullptr;
-    ctx->d_critf = nullptr;
+    ctx->d_critf = ctx->d_mm = nullptr;
     ctx->field_bytes = 0;
     ctx->cap_n = 0;
 }
@@ -703,6 +879,19 @@ void free_rec(dmtz_ctx* ctx) {
     (void)hipFree(ctx->d_rec);
     ctx->d_rec = nullptr;
     ctx->rec_cap = 0;
+}
+
+void free_paths(dmtz_ctx* ctx) {
+    (void)hipFree(ctx->d_as_cnt);
+    (void)hipFree(ctx->d_as_off);
+    (void)hipFree(ctx->d_as_steps);
+    (void)hipFree(ctx->d_cubes);
+    (void)hipFree(ctx->d_scan_tmp);
+    ctx->d_as_cnt = ctx->d_as_off = nullptr;
+    ctx->d_as_steps = nullptr;
+    ctx->d_cubes = nullptr;
+    ctx->d_scan_tmp = nullptr;
+    ctx->as_cnt_cap = ctx->as_off_cap = ctx->as_steps_cap = ctx->cubes_cap = ctx->scan_tmp_cap = 0;
 }
 
 void free_fc(dmtz_ctx* ctx) {
@@ -739,6 +928,7 @@ int ensure(dmtz_ctx* ctx, i64 N, size_t elem, bool want_fields) {
         CK(hipMalloc(&ctx->d_code_f, sizeof(StarCode) * n));
         CK(hipMalloc(&ctx->d_code_g, sizeof(StarCode) * n));
         CK(hipMalloc(&ctx->d_critf, sizeof(CritMask) * n));
+        CK(hipMalloc(&ctx->d_mm, sizeof(CritMask) * n));
         ctx->cap_n = N;
         ctx->epoch = 0;
     }
@@ -773,6 +963,19 @@ int grow_fc(dmtz_ctx* ctx, ull need) {
     CK(hipMalloc(&ctx->d_labels, sizeof(int64_t) * 4 * cap));
     CK(hipMalloc(&ctx->d_walk, sizeof(int64_t) * 4 * cap));
     ctx->fc_cap = cap;
+    return 0;
+}
+
+// Grow a device buffer of P to at least need elements (by x1.25 + 4096).
+template <class P>
+int grow_buf(dmtz_ctx* ctx, P*& p, size_t& cap, size_t need) {
+    if (need <= cap) return 0;
+    (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t n = need + need / 4 + 4096;
+    CK(hipMalloc(&p, sizeof(P) * n));
+    cap = n;
     return 0;
 }
 
@@ -864,8 +1067,9 @@ struct Driver {
     int detect_once() {
         CK(hipMemsetAsync(ctx->d_ctr + K_MIN, 0, sizeof(ull) * 4, ctx->stream));
         CK(hipEventRecord(ctx->ev0, ctx->stream));
-        k_detect<<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(ctx->d_code_g, ctx->d_critf, c.N, rec_lists(ctx),
-                                                           ctx->d_ctr + K_MIN);
+        k_detect<<<(unsigned)((c.N + EMIT_TILE - 1) / EMIT_TILE), BLOCK, 0, ctx->stream>>>(ctx->d_mm, c.N,
+                                                                                         rec_lists(ctx),
+                                                                                         ctx->d_ctr + K_MIN);
         CK(hipGetLastError());
         CK(hipEventRecord(ctx->ev1, ctx->stream));
         int rc = sync_ctr(ctx);
@@ -914,9 +1118,10 @@ struct Driver {
         for (int ty = 0; ty < 4; ty++) {
             if (counts[ty] == 0) continue;
             if ((ty == 1 || ty == 2) && fl.preserve_saddles != 1) continue;
-            k_fix_cp<T><<<nblocks(counts[ty]), BLOCK, 0, ctx->stream>>>(ty, L.l[ty], counts[ty], f, ctx->d_code_f, g,
-                                                                         ctx->d_code_g, d, c, ctx->d_counter, eps,
-                                                                         range, fl.edit_type, sink());
+            auto kern = ty == 0 ? k_fix_cp<T, 0> : ty == 1 ? k_fix_cp<T, 1> : ty == 2 ? k_fix_cp<T, 2> : k_fix_cp<T, 3>;
+            kern<<<nblocks(counts[ty]), BLOCK, 0, ctx->stream>>>(L.l[ty], counts[ty], f, ctx->d_code_f, g,
+                                                                 ctx->d_code_g, d, c, ctx->d_counter, eps, range,
+                                                                 fl.edit_type, sink());
             CK(hipGetLastError());
         }
         return apply();
@@ -951,6 +1156,13 @@ struct Driver {
     // (the g gradient of main_3d.cu:6864-6874 restricted to where it can change).
     int refresh(ull nc) {
         if (!nc) return 0;
+        if (16 * nc >= (ull)c.N) {  // most stars change: re-evaluate all of them
+            int rc = g_codes();
+            if (rc) return rc;
+            as_full = true;
+            ctx->dirty_total += (double)c.N;
+            return 0;
+        }
         k_dirty<<<nblocks(15 * nc), BLOCK, 0, ctx->stream>>>(ctx->d_changed, nc, d, ctx->d_dstamp, ctx->epoch,
                                                             ctx->d_dirty, ctx->d_ctr + K_DIRTY);
         CK(hipGetLastError());
@@ -958,8 +1170,13 @@ struct Driver {
         if (rc) return rc;
         ull nd = ctx->h_ctr[K_DIRTY];
         if (nd) {
-            k_code_list<T><<<nblocks(nd), BLOCK, 0, ctx->stream>>>(g, d, c, ctx->d_code_g, ctx->d_dirty, nd);
+            k_code_list_g<T><<<nblocks(nd), BLOCK, 0, ctx->stream>>>(g, d, ctx->d_code_g, ctx->d_critf, ctx->d_mm,
+                                                                      ctx->d_dirty, nd);
             CK(hipGetLastError());
+            if (as_ready) {
+                k_cube_mark<<<nblocks(nd), BLOCK, 0, ctx->stream>>>(ctx->d_dirty, nd, d, ctx->d_cubes);
+                CK(hipGetLastError());
+            }
         }
         ctx->dirty_total += (double)nd;
         return 0;
@@ -993,8 +1210,9 @@ struct Driver {
         if (nz == 0 || r >= span) {
             k_freeze_all<T><<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(f, g, ctx->d_counter, c.N);
             CK(hipGetLastError());
-            rc = codes(g, ctx->d_code_g);
+            rc = g_codes();
             if (rc) return rc;
+            as_full = true;
             ctx->finished += c.N;
         } else {
             rc = begin_edits();  // fresh epoch: each vertex is claimed once
@@ -1046,6 +1264,14 @@ struct Driver {
         return 0;
     }
 
+    // g star codes and mismatch masks of every vertex (the g gradient, main_3d.cu:6624-6638);
+    // needs the f-critical masks.
+    int g_codes() {
+        k_code_g<T><<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(g, d, c, ctx->d_code_g, ctx->d_critf, ctx->d_mm);
+        CK(hipGetLastError());
+        return 0;
+    }
+
     // Star codes of every vertex of a field.
     int codes(const T* fld, StarCode* out) {
         k_code<T><<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(fld, d, c, out);
@@ -1074,7 +1300,8 @@ struct Driver {
         for (;;) {
             Lists4 FL = fcrit_lists(ctx);
             CK(hipMemsetAsync(ctx->d_ctr + K_FC0, 0, sizeof(ull) * 4, ctx->stream));
-            k_fcrit<<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(ctx->d_code_f, c, FL, ctx->d_ctr + K_FC0, type_mask);
+            k_fcrit<<<(unsigned)((c.N + EMIT_TILE - 1) / EMIT_TILE), BLOCK, 0, ctx->stream>>>(
+                ctx->d_code_f, c, FL, ctx->d_ctr + K_FC0, type_mask);
             CK(hipGetLastError());
             int rc = sync_ctr(ctx);
             if (rc) return rc;
@@ -1106,12 +1333,17 @@ struct Driver {
         CK(hipMemsetAsync(ctx->d_ctr + K_DS, 0, sizeof(ull) * 2, ctx->stream));
         Lists4 FL = fcrit_lists(ctx);
         const bool conn = fl.preserve_geometry != 1;
-        if (nfcf) {
+        if (nfcf && as_precomputable()) {
+            int rc = as_check();
+            if (rc) return rc;
+        } else if (nfcf) {
             const int64_t* lab = conn ? ctx->d_labels + 2 * ctx->fc_cap : nullptr;
             k_as_branch<T><<<nblocks(2 * nfcf), BLOCK, 0, ctx->stream>>>(FL.l[2], nfcf, lab, f, ctx->d_code_f, g,
                                                                          ctx->d_code_g, d, c, ctx->d_walk,
                                                                          ctx->d_ctr + K_WALK);
             CK(hipGetLastError());
+        }
+        if (nfcf) {
             k_as_combine<<<nblocks(nfcf), BLOCK, 0, ctx->stream>>>(ctx->d_walk, nfcf, path_list(1));
             CK(hipGetLastError());
         }
@@ -1133,6 +1365,64 @@ struct Driver {
         int rc = sync_ctr(ctx);
         if (rc) return rc;
         return overflow_check(ctx);
+    }
+
+    // Ascending geometry checks over precomputed f paths: in geometry mode, when tet and face
+    // ids fit 32 bits.
+    bool as_ready = false, as_full = true;
+
+    bool as_precomputable() const {
+        return fl.preserve_geometry == 1 && c.T < ((i64)1 << 32) && c.F < ((i64)1 << 32);
+    }
+
+    int as_prepare() {
+        const ull nb = 2 * nfcf;
+        Lists4 FL = fcrit_lists(ctx);
+        int rc = grow_buf(ctx, ctx->d_as_cnt, ctx->as_cnt_cap, nb + 1);
+        if (rc) return rc;
+        rc = grow_buf(ctx, ctx->d_as_off, ctx->as_off_cap, nb + 1);
+        if (rc) return rc;
+        CK(hipMemsetAsync(ctx->d_as_cnt + nb, 0, sizeof(ull), ctx->stream));
+        k_as_count<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(FL.l[2], nb, f, ctx->d_code_f, d, c, ctx->d_as_cnt,
+                                                               ctx->d_ctr + K_WALK);
+        CK(hipGetLastError());
+        size_t tmp = 0;
+        CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, ctx->d_as_cnt, ctx->d_as_off, (int)(nb + 1), ctx->stream));
+        uint8_t* tp = (uint8_t*)ctx->d_scan_tmp;
+        rc = grow_buf(ctx, tp, ctx->scan_tmp_cap, tmp);
+        ctx->d_scan_tmp = tp;
+        if (rc) return rc;
+        CK(hipcub::DeviceScan::ExclusiveSum(ctx->d_scan_tmp, tmp, ctx->d_as_cnt, ctx->d_as_off, (int)(nb + 1),
+                                            ctx->stream));
+        ull total = 0;
+        CK(hipMemcpyAsync(&total, ctx->d_as_off + nb, sizeof(ull), hipMemcpyDeviceToHost, ctx->stream));
+        CK(hipStreamSynchronize(ctx->stream));
+        rc = grow_buf(ctx, ctx->d_as_steps, ctx->as_steps_cap, total);
+        if (rc) return rc;
+        k_as_fill<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(FL.l[2], nb, f, ctx->d_code_f, d, c, ctx->d_as_off,
+                                                              ctx->d_as_steps);
+        CK(hipGetLastError());
+        const size_t words = (size_t)(c.N >> 5) + 1;
+        rc = grow_buf(ctx, ctx->d_cubes, ctx->cubes_cap, words);
+        if (rc) return rc;
+        CK(hipMemsetAsync(ctx->d_cubes, 0, sizeof(uint32_t) * words, ctx->stream));
+        as_ready = true;
+        as_full = true;
+        return 0;
+    }
+
+    int as_check() {
+        if (!as_ready) {
+            int rc = as_prepare();
+            if (rc) return rc;
+        }
+        const ull nb = 2 * nfcf;
+        k_as_check<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(nb, ctx->d_as_off, ctx->d_as_steps, g, ctx->d_code_g,
+                                                               d, c, ctx->d_cubes, as_full ? 1 : 0, ctx->d_walk);
+        CK(hipGetLastError());
+        CK(hipMemsetAsync(ctx->d_cubes, 0, sizeof(uint32_t) * ((size_t)(c.N >> 5) + 1), ctx->stream));
+        as_full = false;
+        return 0;
     }
 
     int fix_paths(ull nds, ull nas) {
@@ -1187,7 +1477,7 @@ struct Driver {
             rc = prepare_paths();
             if (rc) return rc;
         }
-        rc = codes(g, ctx->d_code_g);  // gradient of g, main_3d.cu:6624-6638
+        rc = g_codes();  // gradient of g, main_3d.cu:6624-6638
         if (rc) return rc;
         CK(hipStreamSynchronize(ctx->stream));
         auto t1 = Clock::now();
@@ -1462,7 +1752,7 @@ int targets_host(dmtz_ctx* ctx, const T* f, const T* g, i64 nx, i64 ny, i64 nz, 
     Driver<T> drv{ctx, (const T*)ctx->d_f, (T*)ctx->d_g, d, c, eps, range, *flags};
     rc = drv.f_codes();
     if (rc) return rc;
-    rc = drv.codes((const T*)ctx->d_g, ctx->d_code_g);
+    rc = drv.g_codes();
     if (rc) return rc;
     rc = drv.detect();
     if (rc) return rc;
@@ -1474,10 +1764,10 @@ int targets_host(dmtz_ctx* ctx, const T* f, const T* g, i64 nx, i64 ny, i64 nz, 
     for (int ty = 0; ty < 4; ty++) {
         if (cc[ty] == 0) continue;
         if ((ty == 1 || ty == 2) && flags->preserve_saddles != 1) continue;
-        k_fix_cp<T><<<nblocks(cc[ty]), BLOCK, 0, ctx->stream>>>(ty, L.l[ty], cc[ty], (const T*)ctx->d_f,
-                                                                 ctx->d_code_f, (const T*)ctx->d_g, ctx->d_code_g, d, c,
-                                                                 ctx->d_counter, eps, range, flags->edit_type,
-                                                                 drv.sink());
+        auto kern = ty == 0 ? k_fix_cp<T, 0> : ty == 1 ? k_fix_cp<T, 1> : ty == 2 ? k_fix_cp<T, 2> : k_fix_cp<T, 3>;
+        kern<<<nblocks(cc[ty]), BLOCK, 0, ctx->stream>>>(L.l[ty], cc[ty], (const T*)ctx->d_f, ctx->d_code_f,
+                                                         (const T*)ctx->d_g, ctx->d_code_g, d, c, ctx->d_counter, eps,
+                                                         range, flags->edit_type, drv.sink());
         CK(hipGetLastError());
     }
     rc = sync_ctr(ctx);
@@ -1514,7 +1804,7 @@ int paths_host(dmtz_ctx* ctx, const T* f, const T* g, i64 nx, i64 ny, i64 nz, co
     Driver<T> drv{ctx, (const T*)ctx->d_f, (T*)ctx->d_g, d, c, 0.0, 0.0, fl};
     rc = drv.f_codes();
     if (rc) return rc;
-    rc = drv.codes((const T*)ctx->d_g, ctx->d_code_g);
+    rc = drv.g_codes();
     if (rc) return rc;
     rc = drv.prepare_paths();
     if (rc) return rc;
@@ -1606,6 +1896,7 @@ void dmtz_ctx_destroy(dmtz_ctx* ctx) {
     free_bufs(ctx);
     free_rec(ctx);
     free_fc(ctx);
+    free_paths(ctx);
     (void)hipFree(ctx->d_ctr);
     (void)hipHostFree(ctx->h_ctr);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);

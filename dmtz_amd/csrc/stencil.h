@@ -25,8 +25,56 @@ namespace dmtz {
 
 typedef int64_t i64;
 
+DMTZ_HD uint64_t mulhi64(uint64_t a, uint64_t b) { return (uint64_t)(((unsigned __int128)a * b) >> 64); }
+
+// Unsigned division by a run-time invariant divisor d >= 1 through a precomputed multiplier
+// (Granlund and Montgomery, "Division by invariant integers using multiplication", PLDI 1994,
+// Figure 4.1): with l = ceil(log2 d) and m = floor(2^64 (2^l - d) / d) + 1,
+// n / d = (t + ((n - t) >> min(l, 1))) >> max(l - 1, 0) where t = mulhi(m, n), for all 64-bit n.
+struct FastDiv {
+    uint64_t m = 1;
+    uint32_t d = 1;
+    uint16_t s1 = 0, s2 = 0;
+    FastDiv() = default;
+    explicit FastDiv(uint32_t dv) : d(dv) {
+        int l = 0;
+        while (l < 32 && ((uint64_t)1 << l) < dv) l++;
+        const unsigned __int128 num = (unsigned __int128)(((uint64_t)1 << l) - dv) << 64;
+        m = (uint64_t)(num / dv) + 1;
+        s1 = (uint16_t)(l < 1 ? l : 1);
+        s2 = (uint16_t)(l > 1 ? l - 1 : 0);
+    }
+    DMTZ_HD uint64_t div(uint64_t n) const {
+        const uint64_t t = mulhi64(m, n);
+        return (t + ((n - t) >> s1)) >> s2;
+    }
+};
+
+// Negative numerators (ids of -1 passed through a decoder) take C's truncating division.
+DMTZ_HD void divmod_neg(i64 n, i64 dv, i64& q, i64& r) {
+    q = n / dv;
+    r = n % dv;
+}
+
+// q = n / fd.d and r = n % fd.d with C semantics (truncation toward zero).
+DMTZ_HD void divmod(i64 n, const FastDiv& fd, i64& q, i64& r) {
+    if (n >= 0) {
+        q = (i64)fd.div((uint64_t)n);
+        r = n - q * (i64)fd.d;
+    } else {
+        divmod_neg(n, (i64)fd.d, q, r);
+    }
+}
+
+// Grid sides plus their divisors for the ID decoders (mixed-radix digits), prepared once per
+// grid on the host.
 struct Dims {
-    i64 W, H, D;
+    i64 W = 0, H = 0, D = 0;
+    FastDiv fW, fH, fW1, fH1, fD1;
+    Dims() = default;
+    Dims(i64 w, i64 h, i64 dd)
+        : W(w), H(h), D(dd), fW((uint32_t)w), fH((uint32_t)h), fW1((uint32_t)(w - 1)), fH1((uint32_t)(h - 1)),
+          fD1((uint32_t)(dd - 1)) {}
 };
 
 // main_3d.cu:44-51
@@ -129,10 +177,9 @@ struct Cell {
 
 DMTZ_HD i64 vidx(const Dims& d, i64 x, i64 y, i64 z) { return x + y * d.W + z * d.W * d.H; }
 DMTZ_HD void vxyz(const Dims& d, i64 v, i64& x, i64& y, i64& z) {
-    x = v % d.W;
-    i64 r = v / d.W;
-    y = r % d.H;
-    z = r / d.H;
+    i64 r;
+    divmod(v, d.fW, r, x);
+    divmod(r, d.fH, z, y);
 }
 DMTZ_HD bool inb(const Dims& d, i64 x, i64 y, i64 z) {
     return x >= 0 && x < d.W && y >= 0 && y < d.H && z >= 0 && z < d.D;
@@ -196,22 +243,22 @@ DMTZ_HD void edge_verts(const Dims& d, const Counts& c, i64 e, i64& a, i64& b) {
     int blk = 0;
     while (blk < 6 && e >= c.eoff[blk + 1]) blk++;
     i64 r = e - c.eoff[blk];
-    const i64 W = d.W, H = d.H, D = d.D;
-    i64 x = 0, y = 0, z = 0, x1 = 0, y1 = 0, z1 = 0;
+    // block layouts as mixed-radix digits (innermost first)
+    i64 x = 0, y = 0, z = 0, x1 = 0, y1 = 0, z1 = 0, k = 0;
     switch (blk) {
-        case 0: z = r / ((W - 1) * H); y = (r % ((W - 1) * H)) / (W - 1); x = r % (W - 1);
+        case 0: divmod(r, d.fW1, k, x); divmod(k, d.fH, z, y);
                 x1 = x + 1; y1 = y; z1 = z; break;
-        case 1: z = r / (W * (H - 1)); x = (r % (W * (H - 1))) / (H - 1); y = r % (H - 1);
+        case 1: divmod(r, d.fH1, k, y); divmod(k, d.fW, z, x);
                 x1 = x; y1 = y + 1; z1 = z; break;
-        case 2: y = r / (W * (D - 1)); x = (r % (W * (D - 1))) / (D - 1); z = r % (D - 1);
+        case 2: divmod(r, d.fD1, k, z); divmod(k, d.fW, y, x);
                 x1 = x; y1 = y; z1 = z + 1; break;
-        case 3: z = r / ((W - 1) * (H - 1)); y = (r % ((W - 1) * (H - 1))) / (W - 1); x = r % (W - 1);
+        case 3: divmod(r, d.fW1, k, x); divmod(k, d.fH1, z, y);
                 x1 = x; y1 = y + 1; z1 = z; x = x + 1; break;
-        case 4: x = r / ((H - 1) * (D - 1)); y = (r % ((H - 1) * (D - 1))) / (D - 1); z = r % (D - 1);
+        case 4: divmod(r, d.fD1, k, z); divmod(k, d.fH1, x, y);
                 x1 = x; y1 = y + 1; z1 = z + 1; break;
-        case 5: y = r / ((W - 1) * (D - 1)); z = (r % ((W - 1) * (D - 1))) / (W - 1); x = r % (W - 1);
+        case 5: divmod(r, d.fW1, k, x); divmod(k, d.fD1, y, z);
                 x1 = x; y1 = y; z1 = z + 1; x = x + 1; break;
-        default: z = r / ((W - 1) * (H - 1)); y = (r % ((W - 1) * (H - 1))) / (W - 1); x = r % (W - 1);
+        default: divmod(r, d.fW1, k, x); divmod(k, d.fH1, z, y);
                  x1 = x; y1 = y + 1; z1 = z + 1; x = x + 1; break;
     }
     i64 p = vidx(d, x, y, z), q = vidx(d, x1, y1, z1);
@@ -300,32 +347,27 @@ DMTZ_HD void sort_desc3(i64& a, i64& b, i64& c) {
 // main_3d.cu:439-556; an id beyond the last block decodes arithmetically like the reference and
 // may give indices >= N (callers drop those).
 DMTZ_HD void tri_verts(const Dims& d, const Counts& c, i64 t, i64& v1, i64& v2, i64& v3) {
-    const i64 W = d.W, H = d.H, D = d.D;
-    i64 ax, ay, az;
+    i64 ax, ay, az, q;
     uint32_t mask;
     if (t < c.foff[1]) {
         i64 b = t / 2;
-        az = b / ((W - 1) * (H - 1));
-        ay = (b % ((W - 1) * (H - 1))) / (W - 1);
-        ax = b % (W - 1);
-        mask = TRI_MASK[0][t % 2];
+        divmod(b, d.fW1, q, ax);
+        divmod(q, d.fH1, az, ay);
+        mask = TRI_MASK[0][t % 2 != 0 ? 1 : 0];  // a negative odd id takes the odd pattern, like the reference
     } else if (t < c.foff[2]) {
         i64 u = t - c.foff[1], b = u / 2;
-        ax = b / ((H - 1) * (D - 1));
-        ay = (b % ((H - 1) * (D - 1))) / (D - 1);
-        az = b % (D - 1);
+        divmod(b, d.fD1, q, az);
+        divmod(q, d.fH1, ax, ay);
         mask = TRI_MASK[1][u % 2];
     } else if (t < c.foff[3]) {
         i64 u = t - c.foff[2], b = u / 2;
-        ay = b / ((W - 1) * (D - 1));
-        az = (b % ((W - 1) * (D - 1))) / (W - 1);
-        ax = b % (W - 1);
+        divmod(b, d.fW1, q, ax);
+        divmod(q, d.fD1, ay, az);
         mask = TRI_MASK[2][u % 2];
     } else {
         i64 u = t - c.foff[3], b = u / 6;
-        az = b / ((W - 1) * (H - 1));
-        ay = (b % ((W - 1) * (H - 1))) / (W - 1);
-        ax = b % (W - 1);
+        divmod(b, d.fW1, q, ax);
+        divmod(q, d.fH1, az, ay);
         mask = TRI_MASK[3][u % 6];
     }
     i64 out[3];
@@ -350,12 +392,12 @@ DMTZ_HD void sort_desc4(i64& a, i64& b, i64& c, i64& e) {
 
 // Vertices of a tet id, descending index order (main_3d.cu:672-740).
 DMTZ_HD void tet_verts(const Dims& d, i64 q, i64& v1, i64& v2, i64& v3, i64& v4) {
-    const i64 W = d.W, H = d.H;
     i64 b = q / 6;
-    uint32_t mask = TET_MASK[q % 6];
-    i64 az = b / ((W - 1) * (H - 1));
-    i64 ay = (b % ((W - 1) * (H - 1))) / (W - 1);
-    i64 ax = b % (W - 1);
+    const i64 type = q % 6;
+    uint32_t mask = TET_MASK[type < 0 ? 5 : type];  // a negative id takes the last type, like the reference
+    i64 az, ay, ax, r;
+    divmod(b, d.fW1, r, ax);
+    divmod(r, d.fH1, az, ay);
     i64 out[4];
     int n = 0;
     for (int k = 0; k < 8; k++)

@@ -242,3 +242,48 @@ static i64 as_two_ways_impl(const T* f, const T* g, i64 W, i64 H, i64 D, i64* ou
 extern "C" i64 hc_as_two_ways_f32(const float* f, const float* g, i64 W, i64 H, i64 D, i64* ow, i64* os) {
     return as_two_ways_impl<float>(f, g, W, H, D, ow, os);
 }
+
+// FastDiv / divmod against native 64-bit division: divisors 1..4096 and large 32-bit ones, numerators
+// at the edges (0, d-1, d, multiples, 2^63-1, 2^64-1) and pseudo-random. Returns the number of
+// disagreements.
+extern "C" i64 hc_divmod_check(i64 samples) {
+    i64 bad = 0;
+    uint64_t s = 0x9E3779B97F4A7C15ull;
+    auto rnd = [&]() {
+        s ^= s << 13;
+        s ^= s >> 7;
+        s ^= s << 17;
+        return s;
+    };
+    std::vector<uint64_t> ds;
+    for (uint64_t d = 1; d <= 4096; d++) ds.push_back(d);
+    for (int k = 0; k < 2000; k++) ds.push_back(1 + (rnd() >> (33 + rnd() % 31)));
+    ds.push_back(511ull * 512);
+    ds.push_back(1023ull * 1023);
+    ds.push_back((1ull << 31) - 1);
+    ds.push_back((1ull << 32) - 1);
+    for (uint64_t d : ds) {
+        FastDiv fd((uint32_t)d);
+        uint64_t edge[] = {0, 1, d - 1, d, d + 1, 2 * d - 1, 2 * d, 1000 * d + 7, (1ull << 63) - 1,
+                           ~0ull, ~0ull - d, (1ull << 34) + 12345};
+        for (uint64_t n : edge) bad += fd.div(n) != n / d;
+        for (i64 k = 0; k < samples; k++) {
+            uint64_t n = rnd() >> (rnd() % 64);
+            bad += fd.div(n) != n / d;
+            i64 sn = (i64)(rnd() >> 1) * ((rnd() & 1) ? -1 : 1) >> (rnd() % 62), q, r;
+            divmod(sn, fd, q, r);
+            bad += (q != sn / (i64)d) + (r != sn % (i64)d);
+        }
+    }
+    return bad;
+}
+
+// The ID decoders of stencil.h for a list of ids (negative and past-the-end ones included):
+// out receives 3 vertices per triangle id, then 4 per tet id.
+extern "C" void hc_decode(i64 W, i64 H, i64 D, const i64* tris, i64 nt, const i64* tets, i64 nq, i64* out) {
+    Dims d{W, H, D};
+    Counts c = make_counts(d);
+    for (i64 k = 0; k < nt; k++) tri_verts(d, c, tris[k], out[3 * k], out[3 * k + 1], out[3 * k + 2]);
+    i64* o = out + 3 * nt;
+    for (i64 k = 0; k < nq; k++) tet_verts(d, tets[k], o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+}

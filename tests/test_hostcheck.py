@@ -110,3 +110,57 @@ def test_precomputed_ascending_paths_equal_the_walk(hc, oracle):
                 assert (ow[: 2 * nb] == os_[: 2 * nb]).all()
                 found += int((ow[: 2 * nb: 2] >= 0).sum())
     assert found > 0
+
+
+def test_fast_division_matches_native(hc):
+    """The multiply-high division the ID decoders use (stencil.h FastDiv / divmod) equals native
+    64-bit division and C remainder semantics, negative numerators included."""
+    hc.hc_divmod_check.argtypes = [I]
+    hc.hc_divmod_check.restype = I
+    assert hc.hc_divmod_check(200) == 0
+
+
+@pytest.mark.parametrize("geometry", [1, 0])
+def test_separatrix_lists_match_oracle(hc, oracle, geometry):
+    """Separatrix detection lists (geometry and connectivity mode) from the host build of the
+    device headers against the oracle, on g whose critical cells already match f's."""
+    hc.hc_paths_f32.argtypes = [P, P, I, I, I, ctypes.c_int, P, P, P, P, P]
+    for seed in range(3):
+        for shape, rel in [((12, 12, 12), 1e-2), ((10, 14, 9), 3e-2), ((13, 17, 11), 5e-2)]:
+            f = fields.gaussian_mixture(shape, 5, seed=seed, noise=0.02)
+            eps, rng = fields.eps_from_rel(f, rel)
+            g = oracle.correct(f, fields.quantize(f, eps), eps, rng, oracle.flags())["g"]
+            want = oracle.paths(f, g, oracle.flags(preserve_vpath=1, preserve_geometry=geometry))
+            D, H, W = shape
+            N, E, F, T = oracle.counts(W, H, D)
+            c2 = np.zeros(2, np.int64)
+            ds_id, ds_rec = np.zeros(E, np.int64), np.zeros(E, np.int64)
+            as_id, as_rec = np.zeros(2 * F, np.int64), np.zeros(2 * F, np.int64)
+            rc = hc.hc_paths_f32(_p(f), _p(g), W, H, D, geometry, _p(c2), _p(ds_id), _p(ds_rec), _p(as_id),
+                                 _p(as_rec))
+            assert rc == 0
+            nds, nas = int(c2[0]), int(c2[1])
+            assert sorted(zip(ds_id[:nds].tolist(), ds_rec[:nds].tolist())) == \
+                sorted(zip(want["ds"].tolist(), want["ds_rec"].tolist()))
+            assert sorted(zip(as_id[:nas].tolist(), as_rec[:nas].tolist())) == \
+                sorted(zip(want["as_"].tolist(), want["as_rec"].tolist()))
+
+
+def test_decoders_match_oracle_on_every_id(hc, oracle):
+    """Triangle and tet decoders on in-range, negative and past-the-end ids: the device decoders
+    (stencil.h) decode exactly as the oracle's restatement of the reference, whose arithmetic
+    gives a negative id a definite pattern."""
+    hc.hc_decode.argtypes = [I, I, I, P, I, P, I, P]
+    for dims in [(5, 6, 7), (3, 2, 2), (11, 17, 13)]:
+        W, H, D = dims
+        N, E, F, T = oracle.counts(W, H, D)
+        tris = np.array(list(range(-13, 40)) + list(range(F - 20, F + 20)), np.int64)
+        tets = np.array(list(range(-13, 40)) + list(range(T - 20, T + 20)), np.int64)
+        out = np.zeros(3 * len(tris) + 4 * len(tets), np.int64)
+        hc.hc_decode(W, H, D, _p(tris), len(tris), _p(tets), len(tets), _p(out))
+        got_t = out[: 3 * len(tris)].reshape(-1, 3)
+        got_q = out[3 * len(tris):].reshape(-1, 4)
+        for k, t in enumerate(tris):
+            assert tuple(got_t[k]) == oracle.tri_verts(dims, int(t)), (dims, t)
+        for k, q in enumerate(tets):
+            assert tuple(got_q[k]) == oracle.tet_verts(dims, int(q)), (dims, q)
