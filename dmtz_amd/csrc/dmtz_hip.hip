@@ -573,6 +573,50 @@ __global__ __launch_bounds__(BLOCK) void k_ds_geo(const uint64_t* fce, ull n, co
     out.push_wave(has, e, rec);
 }
 
+// Descending geometry checks over per-edge data computed once per run: endpoints and their f
+// pairings (ds_geometry compares only those with g's). pairing -1 is stored as ~0u.
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_ds_prepare(const uint64_t* fce, ull n, const T* f, const StarCode* cf,
+                                                      Dims d, Counts c, uint4* dsv, uint32_t* dse) {
+    const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t r = fce[i];
+    const i64 e = local_cell_id(d, c, (i64)(r >> 7), (int)(r & 127u));
+    FieldGrad<T> F(f, cf, d, c);
+    i64 v1, v2;
+    edge_verts(d, c, e, v1, v2);
+    const i64 p1 = F.pv(v1), p2 = F.pv(v2);
+    dsv[i] = make_uint4((uint32_t)v1, (uint32_t)v2, p1 < 0 ? ~0u : (uint32_t)p1, p2 < 0 ? ~0u : (uint32_t)p2);
+    dse[i] = (uint32_t)e;
+}
+
+// get_false_ds_vpath (main_3d.cu:4837-4959) as ds_geometry does it: an edge is re-evaluated when
+// full is set or one of its endpoints had its g lower star re-evaluated since the last check
+// (its g pairing depends on nothing else); res keeps each edge's recorded vertex (-1: none).
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_ds_check(ull n, const uint4* dsv, const uint32_t* dse, const T* g,
+                                                    const StarCode* cg, Dims d, Counts c, const uint32_t* vbits,
+                                                    int full, int64_t* res, PathList out) {
+    const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    i64 rec = -1, e = -1;
+    if (i < n) {
+        const uint4 q = dsv[i];
+        e = (i64)dse[i];
+        const bool redo = full != 0 || ((vbits[q.x >> 5] >> (q.x & 31u)) & 1u) || ((vbits[q.y >> 5] >> (q.y & 31u)) & 1u);
+        if (redo) {
+            FieldGrad<T> G(g, cg, d, c);
+            const i64 p1 = q.z == ~0u ? -1 : (i64)q.z, p2 = q.w == ~0u ? -1 : (i64)q.w;
+            const bool b1 = p1 != -1 && p1 != G.pv((i64)q.x);
+            const bool b2 = p2 != -1 && p2 != G.pv((i64)q.y);
+            rec = b1 ? (i64)q.x : (b2 ? (i64)q.y : -1);
+            res[i] = rec;
+        } else {
+            rec = res[i];
+        }
+    }
+    out.push_wave(rec >= 0, e, rec);
+}
+
 // Ascending branch walks (is_false main_3d.cu:4437-4497), thread = (f-critical face, cofacet k).
 // Connectivity mode (maxlab != null) first compares g's reached maximum with f's label
 // (main_3d.cu:4821-4826). res[2*j] = start tet if the branch fails else -1, res[2*j+1] = the
@@ -678,12 +722,15 @@ __global__ __launch_bounds__(BLOCK) void k_as_check(ull nb, const ull* off, cons
     res[2 * j + 1] = dv;
 }
 
-// Mark the (up to 8) cubes that have a dirty vertex as a corner.
-__global__ __launch_bounds__(BLOCK) void k_cube_mark(const uint32_t* dirty, ull n, Dims d, uint32_t* cubes) {
+// Mark a dirty vertex and the (up to 8) cubes that have it as a corner.
+__global__ __launch_bounds__(BLOCK) void k_cube_mark(const uint32_t* dirty, ull n, Dims d, uint32_t* cubes,
+                                                     uint32_t* vbits) {
     const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
+    const uint32_t v = dirty[i];
+    atomicOr(vbits + (v >> 5), 1u << (v & 31u));
     i64 x, y, z;
-    vxyz(d, (i64)dirty[i], x, y, z);
+    vxyz(d, (i64)v, x, y, z);
     for (int k = 0; k < 8; k++) {
         const i64 ax = x - (k & 1), ay = y - ((k >> 1) & 1), az = z - ((k >> 2) & 1);
         if (ax < 0 || ay < 0 || az < 0 || ax > d.W - 2 || ay > d.H - 2 || az > d.D - 2) continue;
@@ -847,8 +894,13 @@ struct dmtz_ctx {
     ull* d_as_off = nullptr;
     uint2* d_as_steps = nullptr;  // (tet, f partner face) per step
     uint32_t* d_cubes = nullptr;  // dirty-cube bitmap since the last path check
+    uint32_t* d_vbits = nullptr;  // dirty-vertex bitmap since the last path check
+    uint4* d_dsv = nullptr;       // per f-critical edge: endpoints and their f pairings
+    uint32_t* d_dse = nullptr;    // per f-critical edge: its id
+    int64_t* d_ds_res = nullptr;  // per f-critical edge: recorded vertex of the last check
     void* d_scan_tmp = nullptr;
     size_t as_cnt_cap = 0, as_off_cap = 0, as_steps_cap = 0, cubes_cap = 0, scan_tmp_cap = 0;
+    size_t vbits_cap = 0, dsv_cap = 0, dse_cap = 0, ds_res_cap = 0;
 };
 
 namespace {
@@ -886,12 +938,19 @@ void free_paths(dmtz_ctx* ctx) {
     (void)hipFree(ctx->d_as_off);
     (void)hipFree(ctx->d_as_steps);
     (void)hipFree(ctx->d_cubes);
+    (void)hipFree(ctx->d_vbits);
+    (void)hipFree(ctx->d_dsv);
+    (void)hipFree(ctx->d_dse);
+    (void)hipFree(ctx->d_ds_res);
     (void)hipFree(ctx->d_scan_tmp);
     ctx->d_as_cnt = ctx->d_as_off = nullptr;
     ctx->d_as_steps = nullptr;
-    ctx->d_cubes = nullptr;
+    ctx->d_cubes = ctx->d_vbits = ctx->d_dse = nullptr;
+    ctx->d_dsv = nullptr;
+    ctx->d_ds_res = nullptr;
     ctx->d_scan_tmp = nullptr;
     ctx->as_cnt_cap = ctx->as_off_cap = ctx->as_steps_cap = ctx->cubes_cap = ctx->scan_tmp_cap = 0;
+    ctx->vbits_cap = ctx->dsv_cap = ctx->dse_cap = ctx->ds_res_cap = 0;
 }
 
 void free_fc(dmtz_ctx* ctx) {
@@ -1174,7 +1233,7 @@ struct Driver {
                                                                       ctx->d_dirty, nd);
             CK(hipGetLastError());
             if (as_ready) {
-                k_cube_mark<<<nblocks(nd), BLOCK, 0, ctx->stream>>>(ctx->d_dirty, nd, d, ctx->d_cubes);
+                k_cube_mark<<<nblocks(nd), BLOCK, 0, ctx->stream>>>(ctx->d_dirty, nd, d, ctx->d_cubes, ctx->d_vbits);
                 CK(hipGetLastError());
             }
         }
@@ -1333,7 +1392,12 @@ struct Driver {
         CK(hipMemsetAsync(ctx->d_ctr + K_DS, 0, sizeof(ull) * 2, ctx->stream));
         Lists4 FL = fcrit_lists(ctx);
         const bool conn = fl.preserve_geometry != 1;
-        if (nfcf && as_precomputable()) {
+        const bool pre = as_precomputable();
+        if (pre && !as_ready) {
+            int rc = as_prepare();
+            if (rc) return rc;
+        }
+        if (nfcf && pre) {
             int rc = as_check();
             if (rc) return rc;
         } else if (nfcf) {
@@ -1347,7 +1411,10 @@ struct Driver {
             k_as_combine<<<nblocks(nfcf), BLOCK, 0, ctx->stream>>>(ctx->d_walk, nfcf, path_list(1));
             CK(hipGetLastError());
         }
-        if (nfce) {
+        if (nfce && pre) {
+            int rc = ds_check();
+            if (rc) return rc;
+        } else if (nfce) {
             if (!conn) {
                 k_ds_geo<T><<<nblocks(nfce), BLOCK, 0, ctx->stream>>>(FL.l[1], nfce, f, ctx->d_code_f, g, ctx->d_code_g,
                                                                       d, c, path_list(0));
@@ -1362,6 +1429,10 @@ struct Driver {
                 CK(hipGetLastError());
             }
         }
+        if (pre) {
+            int rc = paths_checked();
+            if (rc) return rc;
+        }
         int rc = sync_ctr(ctx);
         if (rc) return rc;
         return overflow_check(ctx);
@@ -1372,12 +1443,25 @@ struct Driver {
     bool as_ready = false, as_full = true;
 
     bool as_precomputable() const {
-        return fl.preserve_geometry == 1 && c.T < ((i64)1 << 32) && c.F < ((i64)1 << 32);
+        return fl.preserve_geometry == 1 && c.T < ((i64)1 << 32) && c.F < ((i64)1 << 32) && c.E < ((i64)1 << 32) &&
+               c.N < ((i64)1 << 32);
     }
 
+    // Per-run precomputation of the separatrix checks (geometry mode): the ascending paths of f
+    // (flat steps per branch), the descending per-edge data, and the dirty bitmaps.
     int as_prepare() {
         const ull nb = 2 * nfcf;
         Lists4 FL = fcrit_lists(ctx);
+        int rc = nb ? as_paths(FL, nb) : 0;
+        if (rc) return rc;
+        rc = as_bitmaps_and_ds(FL, (size_t)(c.N >> 5) + 1);
+        if (rc) return rc;
+        as_ready = true;
+        as_full = true;
+        return 0;
+    }
+
+    int as_paths(const Lists4& FL, ull nb) {
         int rc = grow_buf(ctx, ctx->d_as_cnt, ctx->as_cnt_cap, nb + 1);
         if (rc) return rc;
         rc = grow_buf(ctx, ctx->d_as_off, ctx->as_off_cap, nb + 1);
@@ -1402,12 +1486,27 @@ struct Driver {
         k_as_fill<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(FL.l[2], nb, f, ctx->d_code_f, d, c, ctx->d_as_off,
                                                               ctx->d_as_steps);
         CK(hipGetLastError());
-        const size_t words = (size_t)(c.N >> 5) + 1;
-        rc = grow_buf(ctx, ctx->d_cubes, ctx->cubes_cap, words);
+        return 0;
+    }
+
+    int as_bitmaps_and_ds(const Lists4& FL, size_t words) {
+        int rc = grow_buf(ctx, ctx->d_cubes, ctx->cubes_cap, words);
+        if (rc) return rc;
+        rc = grow_buf(ctx, ctx->d_vbits, ctx->vbits_cap, words);
         if (rc) return rc;
         CK(hipMemsetAsync(ctx->d_cubes, 0, sizeof(uint32_t) * words, ctx->stream));
-        as_ready = true;
-        as_full = true;
+        CK(hipMemsetAsync(ctx->d_vbits, 0, sizeof(uint32_t) * words, ctx->stream));
+        if (nfce) {
+            rc = grow_buf(ctx, ctx->d_dsv, ctx->dsv_cap, nfce);
+            if (rc) return rc;
+            rc = grow_buf(ctx, ctx->d_dse, ctx->dse_cap, nfce);
+            if (rc) return rc;
+            rc = grow_buf(ctx, ctx->d_ds_res, ctx->ds_res_cap, nfce);
+            if (rc) return rc;
+            k_ds_prepare<T><<<nblocks(nfce), BLOCK, 0, ctx->stream>>>(FL.l[1], nfce, f, ctx->d_code_f, d, c,
+                                                                       ctx->d_dsv, ctx->d_dse);
+            CK(hipGetLastError());
+        }
         return 0;
     }
 
@@ -1420,7 +1519,22 @@ struct Driver {
         k_as_check<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(nb, ctx->d_as_off, ctx->d_as_steps, g, ctx->d_code_g,
                                                                d, c, ctx->d_cubes, as_full ? 1 : 0, ctx->d_walk);
         CK(hipGetLastError());
-        CK(hipMemsetAsync(ctx->d_cubes, 0, sizeof(uint32_t) * ((size_t)(c.N >> 5) + 1), ctx->stream));
+        return 0;
+    }
+
+    int ds_check() {
+        k_ds_check<T><<<nblocks(nfce), BLOCK, 0, ctx->stream>>>(nfce, ctx->d_dsv, ctx->d_dse, g, ctx->d_code_g, d, c,
+                                                                 ctx->d_vbits, as_full ? 1 : 0, ctx->d_ds_res,
+                                                                 path_list(0));
+        CK(hipGetLastError());
+        return 0;
+    }
+
+    // After a path check: both dirty bitmaps start empty and the next check is incremental.
+    int paths_checked() {
+        const size_t words = (size_t)(c.N >> 5) + 1;
+        CK(hipMemsetAsync(ctx->d_cubes, 0, sizeof(uint32_t) * words, ctx->stream));
+        CK(hipMemsetAsync(ctx->d_vbits, 0, sizeof(uint32_t) * words, ctx->stream));
         as_full = false;
         return 0;
     }
