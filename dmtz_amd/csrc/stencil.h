@@ -226,12 +226,41 @@ DMTZ_HD i64 edge_within(const Dims& d, int blk, i64 x, i64 y, i64 z) {
     }
 }
 
+// Start offsets of edge blocks 1..6 (equal to Counts::eoff[1..6]), computed from the grid sides:
+// the values are uniform, so they stay in scalar registers, whereas a runtime index into an
+// offset array would force the array into scratch memory.
+struct EdgeBlocks {
+    i64 s1, s2, s3, s4, s5, s6;
+    DMTZ_HD explicit EdgeBlocks(const Dims& d) {
+        const i64 W = d.W, H = d.H, D = d.D;
+        s1 = (W - 1) * H * D;
+        s2 = s1 + W * (H - 1) * D;
+        s3 = s2 + W * H * (D - 1);
+        s4 = s3 + (W - 1) * (H - 1) * D;
+        s5 = s4 + W * (H - 1) * (D - 1);
+        s6 = s5 + (W - 1) * H * (D - 1);
+    }
+    DMTZ_HD i64 start(int blk) const {
+        return (blk > 0 ? s1 - 0 : 0) + (blk > 1 ? s2 - s1 : 0) + (blk > 2 ? s3 - s2 : 0) + (blk > 3 ? s4 - s3 : 0) +
+               (blk > 4 ? s5 - s4 : 0) + (blk > 5 ? s6 - s5 : 0);
+    }
+    DMTZ_HD int block_of(i64 e) const {
+        return (int)(e >= s1) + (int)(e >= s2) + (int)(e >= s3) + (int)(e >= s4) + (int)(e >= s5) + (int)(e >= s6);
+    }
+};
+
+// Vertex at cube-local corner k (bit = dx + 2 dy + 4 dz) of the cube anchored at (ax, ay, az).
+DMTZ_HD i64 corner_vidx(const Dims& d, i64 ax, i64 ay, i64 az, int k) {
+    return vidx(d, ax + (k & 1), ay + ((k >> 1) & 1), az + ((k >> 2) & 1));
+}
+
 // Edge between vertex (x,y,z) and its neighbour in direction k.
 DMTZ_HD i64 edge_id_dir(const Dims& d, const Counts& c, i64 x, i64 y, i64 z, int k) {
     i64 x1 = x + DIR[k][0], y1 = y + DIR[k][1], z1 = z + DIR[k][2];
     i64 ax = x < x1 ? x : x1, ay = y < y1 ? y : y1, az = z < z1 ? z : z1;
     int blk = k >> 1;
-    return c.eoff[blk] + edge_within(d, blk, ax, ay, az);
+    (void)c;
+    return EdgeBlocks(d).start(blk) + edge_within(d, blk, ax, ay, az);
 }
 
 // Endpoints of an edge id, larger index first (main_3d.cu:1895-1899). Out-of-range -> -1, -1.
@@ -240,9 +269,10 @@ DMTZ_HD void edge_verts(const Dims& d, const Counts& c, i64 e, i64& a, i64& b) {
         a = b = -1;
         return;
     }
-    int blk = 0;
-    while (blk < 6 && e >= c.eoff[blk + 1]) blk++;
-    i64 r = e - c.eoff[blk];
+    // block = number of block starts at or below e (offsets are non-decreasing)
+    const EdgeBlocks eb(d);
+    const int blk = eb.block_of(e);
+    i64 r = e - eb.start(blk);
     // block layouts as mixed-radix digits (innermost first)
     i64 x = 0, y = 0, z = 0, x1 = 0, y1 = 0, z1 = 0, k = 0;
     switch (blk) {
@@ -370,13 +400,15 @@ DMTZ_HD void tri_verts(const Dims& d, const Counts& c, i64 t, i64& v1, i64& v2, 
         divmod(q, d.fH1, az, ay);
         mask = TRI_MASK[3][u % 6];
     }
-    i64 out[3];
-    int n = 0;
-    for (int k = 0; k < 8; k++)
-        if (mask & (1u << k)) out[n++] = vidx(d, ax + (k & 1), ay + ((k >> 1) & 1), az + ((k >> 2) & 1));
-    v1 = out[0];
-    v2 = out[1];
-    v3 = out[2];
+    // the three set corners of the mask, in increasing corner order
+    const int k0 = __builtin_ctz(mask);
+    mask &= mask - 1;
+    const int k1 = __builtin_ctz(mask);
+    mask &= mask - 1;
+    const int k2 = __builtin_ctz(mask);
+    v1 = corner_vidx(d, ax, ay, az, k0);
+    v2 = corner_vidx(d, ax, ay, az, k1);
+    v3 = corner_vidx(d, ax, ay, az, k2);
     sort_desc3(v1, v2, v3);
 }
 
@@ -398,11 +430,17 @@ DMTZ_HD void tet_verts(const Dims& d, i64 q, i64& v1, i64& v2, i64& v3, i64& v4)
     i64 az, ay, ax, r;
     divmod(b, d.fW1, r, ax);
     divmod(r, d.fH1, az, ay);
-    i64 out[4];
-    int n = 0;
-    for (int k = 0; k < 8; k++)
-        if (mask & (1u << k)) out[n++] = vidx(d, ax + (k & 1), ay + ((k >> 1) & 1), az + ((k >> 2) & 1));
-    v1 = out[0]; v2 = out[1]; v3 = out[2]; v4 = out[3];
+    const int k0 = __builtin_ctz(mask);
+    mask &= mask - 1;
+    const int k1 = __builtin_ctz(mask);
+    mask &= mask - 1;
+    const int k2 = __builtin_ctz(mask);
+    mask &= mask - 1;
+    const int k3 = __builtin_ctz(mask);
+    v1 = corner_vidx(d, ax, ay, az, k0);
+    v2 = corner_vidx(d, ax, ay, az, k1);
+    v3 = corner_vidx(d, ax, ay, az, k2);
+    v4 = corner_vidx(d, ax, ay, az, k3);
     sort_desc4(v1, v2, v3, v4);
 }
 
