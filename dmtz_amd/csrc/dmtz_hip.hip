@@ -220,17 +220,28 @@ __global__ __launch_bounds__(BLOCK) void k_code_list_g(const T* g, Dims d, StarC
     if (i < n) code_g_at(g, d, (i64)list[i], code, crit_f, mm);
 }
 
-// f-critical status of the local cells of every vertex (once per run).
-template <class T>
-__global__ __launch_bounds__(BLOCK) void k_critf(const T* f, const StarCode* cf, Dims d, Counts c, CritMask* out) {
+// f-critical status of the local cells of every vertex (once per run), in two passes: each
+// vertex writes the critical cells it owns, then spreads them to the cells' other vertices.
+__global__ __launch_bounds__(BLOCK) void k_critf_own(const StarCode* cf, i64 n, CritMask* out) {
     const i64 v = (i64)blockIdx.x * BLOCK + threadIdx.x;
-    if (v < c.N) {
-        FieldGrad<T> F(f, cf, d, c);
-        CritMask m;
-        local_crit(F, v, m.lo, m.hi);
-        m.pad = 0;
-        out[v] = m;
-    }
+    if (v >= n) return;
+    uint64_t lo, ilo;
+    uint32_t hi, ihi;
+    code_masks(cf[v], lo, hi, ilo, ihi);
+    CritMask m;
+    m.lo = lo & ilo;
+    m.hi = hi & ihi;
+    m.pad = 0;
+    out[v] = m;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_critf_spread(const StarCode* cf, Dims d, i64 n, CritMask* out) {
+    const i64 v = (i64)blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= n) return;
+    crit_scatter_spread(d, cf[v], v, [&](i64 w, int lc) {
+        if (lc < 51) atomicOr((ull*)&out[w].lo, 1ull << lc);
+        else atomicOr(&out[w].hi, 1u << (lc - 51));
+    });
 }
 
 // Dense mismatch detection: every vertex lists the mismatched cells it owns in g, from the
@@ -1318,7 +1329,9 @@ struct Driver {
     int f_codes() {
         int rc = codes(f, ctx->d_code_f);
         if (rc) return rc;
-        k_critf<T><<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(f, ctx->d_code_f, d, c, ctx->d_critf);
+        k_critf_own<<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(ctx->d_code_f, c.N, ctx->d_critf);
+        CK(hipGetLastError());
+        k_critf_spread<<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(ctx->d_code_f, d, c.N, ctx->d_critf);
         CK(hipGetLastError());
         return 0;
     }

@@ -92,6 +92,70 @@ DMTZ_HD void local_crit(const FieldGrad<T>& F, i64 v, uint64_t& lo, uint32_t& hi
     }
 }
 
+// Slot of a stencil offset, -1 when the offset is not one of the 14 directions.
+DMTZ_HD int offset_slot(int dx, int dy, int dz) { return ST.slot_of[(dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)]; }
+
+// Slot, seen from the vertex in direction `from`, of the vertex in direction `to` (both slots of
+// the same star; the two vertices are adjacent whenever they share a cell with the centre).
+DMTZ_HD int slot_from(int from, int to) {
+    return offset_slot(DIR[to][0] - DIR[from][0], DIR[to][1] - DIR[from][1], DIR[to][2] - DIR[from][2]);
+}
+
+// Every vertex w != u of local cell lc (> 0) of vertex u, with the cell's local index at w:
+// fn(w, lc_at_w). Slot s ^ 1 is the direction opposite to slot s.
+template <class Fn>
+DMTZ_HD void cell_at_other_vertices(const Dims& d, i64 u, int lc, Fn fn) {
+    i64 x, y, z;
+    vxyz(d, u, x, y, z);
+    auto at = [&](int s) { return vidx(d, x + DIR[s][0], y + DIR[s][1], z + DIR[s][2]); };
+    if (lc < LC_TRI) {
+        const int s = lc - LC_EDGE;
+        fn(at(s), LC_EDGE + (s ^ 1));
+    } else if (lc < LC_TET) {
+        const int t = lc - LC_TRI, i = ST.tri_i[t], j = ST.tri_j[t];
+        fn(at(i), LC_TRI + ST.tri_of[i ^ 1][slot_from(i, j)]);
+        fn(at(j), LC_TRI + ST.tri_of[j ^ 1][slot_from(j, i)]);
+    } else {
+        const int q = lc - LC_TET;
+        const int s0 = ST.tet_s[q][0], s1 = ST.tet_s[q][1], s2 = ST.tet_s[q][2];
+        auto tet_at = [&](int a, int b, int c) {  // tet of the centre's neighbours in slots a, b, c
+            const uint32_t key = (1u << a) | (1u << b) | (1u << c);
+            int r = 0;
+            for (int k = 0; k < 24; k++) {
+                const uint32_t kk = (1u << ST.tet_s[k][0]) | (1u << ST.tet_s[k][1]) | (1u << ST.tet_s[k][2]);
+                r = kk == key ? k : r;
+            }
+            return LC_TET + r;
+        };
+        fn(at(s0), tet_at(s0 ^ 1, slot_from(s0, s1), slot_from(s0, s2)));
+        fn(at(s1), tet_at(s1 ^ 1, slot_from(s1, s0), slot_from(s1, s2)));
+        fn(at(s2), tet_at(s2 ^ 1, slot_from(s2, s0), slot_from(s2, s1)));
+    }
+}
+
+// The critical status in f of every local cell, written the other way round from local_crit: a
+// vertex's mask first holds the critical cells it owns (code_masks of its code); then every
+// vertex spreads its owned critical cells to their other vertices: spread(w, lc) adds cell lc
+// at w.
+template <class Spread>
+DMTZ_HD void crit_scatter_spread(const Dims& d, const StarCode& code, i64 v, Spread spread) {
+    uint64_t c_lo, i_lo;
+    uint32_t c_hi, i_hi;
+    code_masks(code, c_lo, c_hi, i_lo, i_hi);
+    uint64_t m = c_lo & i_lo & ~1ull;
+    uint32_t h = c_hi & i_hi;
+    while (m) {
+        const int lc = __builtin_ctzll(m);
+        m &= m - 1;
+        cell_at_other_vertices(d, v, lc, spread);
+    }
+    while (h) {
+        const int lc = LC_TET + __builtin_ctz(h);
+        h &= h - 1;
+        cell_at_other_vertices(d, v, lc, spread);
+    }
+}
+
 // Mismatched cells owned by v in g (main_3d.cu:2492-2550: crit_f XOR crit_g per cell), from v's
 // g star code and the f-critical mask of v's local cells. Every cell has exactly one owner in g,
 // so the union over all vertices lists each mismatched cell once.

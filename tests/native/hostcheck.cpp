@@ -287,3 +287,38 @@ extern "C" void hc_decode(i64 W, i64 H, i64 D, const i64* tris, i64 nt, const i6
     i64* o = out + 3 * nt;
     for (i64 k = 0; k < nq; k++) tet_verts(d, tets[k], o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
 }
+
+// f-critical masks of every vertex two ways: gathered (local_crit, lookups at each cell's owner)
+// and scattered (each owner spreads its critical cells to the cells' other vertices). Returns the
+// number of vertices whose masks differ.
+template <class T>
+static i64 critf_two_ways_impl(const T* f, i64 W, i64 H, i64 D) {
+    Dims d{W, H, D};
+    Counts c = make_counts(d);
+    std::vector<StarCode> cf = all_codes(f, d, c);
+    FieldGrad<T> F(f, cf.data(), d, c);
+    std::vector<uint64_t> slo(c.N);
+    std::vector<uint32_t> shi(c.N);
+    for (i64 v = 0; v < c.N; v++) {
+        uint64_t lo, ilo;
+        uint32_t hi, ihi;
+        code_masks(cf[v], lo, hi, ilo, ihi);
+        slo[v] = lo & ilo;
+        shi[v] = hi & ihi;
+    }
+    for (i64 v = 0; v < c.N; v++)
+        crit_scatter_spread(d, cf[v], v, [&](i64 w, int lc) {
+            if (lc < 51) slo[w] |= 1ull << lc;
+            else shi[w] |= 1u << (lc - 51);
+        });
+    i64 bad = 0;
+    for (i64 v = 0; v < c.N; v++) {
+        uint64_t lo;
+        uint32_t hi;
+        local_crit(F, v, lo, hi);
+        bad += (lo != slo[v]) || (hi != shi[v]);
+    }
+    return bad;
+}
+extern "C" i64 hc_critf_two_ways_f32(const float* f, i64 W, i64 H, i64 D) { return critf_two_ways_impl<float>(f, W, H, D); }
+extern "C" i64 hc_critf_two_ways_f64(const double* f, i64 W, i64 H, i64 D) { return critf_two_ways_impl<double>(f, W, H, D); }

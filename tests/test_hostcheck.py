@@ -164,3 +164,16 @@ def test_decoders_match_oracle_on_every_id(hc, oracle):
             assert tuple(got_t[k]) == oracle.tri_verts(dims, int(t)), (dims, t)
         for k, q in enumerate(tets):
             assert tuple(got_q[k]) == oracle.tet_verts(dims, int(q)), (dims, q)
+
+
+def test_critical_masks_scattered_equal_gathered(hc):
+    """The f-critical masks of all local cells, spread from each cell's owner (the device's
+    k_critf) equal the masks gathered by pairing lookups at the owners (local_crit)."""
+    for name in ("hc_critf_two_ways_f32", "hc_critf_two_ways_f64"):
+        getattr(hc, name).argtypes = [P, I, I, I]
+        getattr(hc, name).restype = I
+    cases = gradient_cases() + [fields.gaussian_mixture((17, 19, 23), 6, seed=9, noise=0.02)]
+    for f in cases:
+        D, H, W = f.shape
+        fn = hc.hc_critf_two_ways_f32 if f.dtype == np.float32 else hc.hc_critf_two_ways_f64
+        assert fn(_p(f), W, H, D) == 0, (f.shape, f.dtype)
