@@ -180,8 +180,13 @@ struct __align__(16) CritMask {
     uint32_t hi, pad;
 };
 
+// The lower-star classification is fully unrolled; left alone the compiler spends ~300 VGPRs on
+// it (one wave per SIMD). CODE_BLOCKS_PER_CU bounds the registers so that more waves hide the
+// load latency.
+constexpr int CODE_BLOCKS_PER_CU = 2;
+
 template <class T>
-__global__ __launch_bounds__(BLOCK) void k_code(const T* fld, Dims d, Counts c, StarCode* code) {
+__global__ __launch_bounds__(BLOCK, CODE_BLOCKS_PER_CU) void k_code(const T* fld, Dims d, Counts c, StarCode* code) {
     const i64 v = (i64)blockIdx.x * BLOCK + threadIdx.x;
     if (v >= c.N) return;
     LowerStar ls;
@@ -207,14 +212,14 @@ __device__ __forceinline__ void code_g_at(const T* g, Dims d, i64 v, StarCode* c
 }
 
 template <class T>
-__global__ __launch_bounds__(BLOCK) void k_code_g(const T* g, Dims d, Counts c, StarCode* code,
+__global__ __launch_bounds__(BLOCK, CODE_BLOCKS_PER_CU) void k_code_g(const T* g, Dims d, Counts c, StarCode* code,
                                                   const CritMask* crit_f, CritMask* mm) {
     const i64 v = (i64)blockIdx.x * BLOCK + threadIdx.x;
     if (v < c.N) code_g_at(g, d, v, code, crit_f, mm);
 }
 
 template <class T>
-__global__ __launch_bounds__(BLOCK) void k_code_list_g(const T* g, Dims d, StarCode* code, const CritMask* crit_f,
+__global__ __launch_bounds__(BLOCK, CODE_BLOCKS_PER_CU) void k_code_list_g(const T* g, Dims d, StarCode* code, const CritMask* crit_f,
                                                        CritMask* mm, const uint32_t* list, ull n) {
     const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
     if (i < n) code_g_at(g, d, (i64)list[i], code, crit_f, mm);

@@ -135,9 +135,12 @@ DMTZ_HD void lower_star(const T* val, LowerStar& r) {
     }
     r.e0 = e0;
 
-    // Triangles: orientation, E0 status, T0 (main_3d.cu:2269-2279).
+    // Triangles: orientation, E0 status, T0 (main_3d.cu:2269-2279). T0 can only reach a vertex
+    // below both other vertices of the triangle, so the triangle is the top face of the tet it
+    // chooses and no other face can choose that tet: the tet is claimed exactly when its top face
+    // pairs with it, and the claims are collected here.
     uint64_t tin = 0, thi = 0, tE0 = 0, tT0 = 0, tD0 = 0;
-    int d0slot[36];
+    uint32_t qT0 = 0;
 #pragma unroll
     for (int t = 0; t < 36; t++) {
         const int i = ST.tri_i[t], j = ST.tri_j[t];
@@ -152,7 +155,7 @@ DMTZ_HD void lower_star(const T* val, LowerStar& r) {
         bool c1 = ((L >> k1) & 1u) && ((blwb >> k1) & 1u);
         bool pick1 = c1 && (!c0 || ((blw[k0] >> k1) & 1u));
         bool pT0 = in && !pE0 && (c0 || c1);
-        d0slot[t] = (c0 || c1) ? (pick1 ? k1 : k0) : 15;
+        qT0 |= (uint32_t)pT0 << (pick1 ? ST.tri_q[t][1] : ST.tri_q[t][0]);
         tin |= (uint64_t)in << t;
         thi |= (uint64_t)hi << t;
         tE0 |= (uint64_t)pE0 << t;
@@ -186,28 +189,19 @@ DMTZ_HD void lower_star(const T* val, LowerStar& r) {
     }
     r.e1 = e1;
 
-    // Tets: T0 claims. A tet is claimed in T0 by its top face when that face chose it.
-    uint32_t qin = 0, qT0 = 0;
+    // Tets of the lower star.
+    uint32_t qin = 0;
 #pragma unroll
     for (int q = 0; q < 24; q++) {
         const int i = ST.tet_s[q][0], j = ST.tet_s[q][1], k = ST.tet_s[q][2];
-        bool in = ((L >> i) & 1u) && ((L >> j) & 1u) && ((L >> k) & 1u);
-        bool ji = (blw[i] >> j) & 1u;  // n_j below n_i
-        bool ki = (blw[i] >> k) & 1u;
-        bool kj = (blw[j] >> k) & 1u;
-        // lowest vertex of the three: below both others
-        bool lowk = ki && kj, lowj = ji && !kj, lowi = !ji && !ki;
-        const int fij = ST.tet_f[q][0], fik = ST.tet_f[q][1], fjk = ST.tet_f[q][2];
-        bool cl = (lowk && ((tT0 >> fij) & 1ull) && d0slot[fij] == k) ||
-                  (lowj && ((tT0 >> fik) & 1ull) && d0slot[fik] == j) ||
-                  (lowi && ((tT0 >> fjk) & 1ull) && d0slot[fjk] == i);
-        qin |= (uint32_t)in << q;
-        qT0 |= (uint32_t)(in && cl) << q;
+        qin |= (uint32_t)(((L >> i) & 1u) && ((L >> j) & 1u) && ((L >> k) & 1u)) << q;
     }
 
-    // Triangles: E1 status and T1 (main_3d.cu:2282-2299).
+    // Triangles: E1 status and T1 (main_3d.cu:2282-2299). T1 reaches the vertex strictly between
+    // the triangle's other two, so the triangle is the face of the chosen tet without its middle
+    // vertex, the only face that can claim it in T1: the claims are collected here.
     uint64_t tE1 = 0, tT1 = 0, tD1 = 0;
-    int d1slot[36];
+    uint32_t qT1 = 0;
 #pragma unroll
     for (int t = 0; t < 36; t++) {
         const int i = ST.tri_i[t], j = ST.tri_j[t];
@@ -227,30 +221,12 @@ DMTZ_HD void lower_star(const T* val, LowerStar& r) {
         const int q0 = ST.tri_q[t][0], q1 = ST.tri_q[t][1];
         bool tetFree = pick1 ? !((qT0 >> q1) & 1u) : !((qT0 >> q0) & 1u);
         bool pT1 = in && !before && !pE1 && (c0 || c1) && tetFree;
-        d1slot[t] = (c0 || c1) ? (pick1 ? k1 : k0) : 15;
+        qT1 |= (uint32_t)pT1 << (pick1 ? q1 : q0);
         tE1 |= (uint64_t)pE1 << t;
         tT1 |= (uint64_t)pT1 << t;
         tD1 |= (uint64_t)(pT1 && pick1) << t;
     }
 
-    // Tets: T1 claims. The T1 claimant of a tet is its face without the middle vertex.
-    uint32_t qT1 = 0;
-#pragma unroll
-    for (int q = 0; q < 24; q++) {
-        const int i = ST.tet_s[q][0], j = ST.tet_s[q][1], k = ST.tet_s[q][2];
-        bool ji = (blw[i] >> j) & 1u;
-        bool ki = (blw[i] >> k) & 1u;
-        bool kj = (blw[j] >> k) & 1u;
-        // middle vertex: above exactly one of the other two
-        bool midi = (ji != ki);          // n_i above exactly one of n_j, n_k
-        bool midj = (!ji) != kj;         // n_j above n_i? (!ji) ; above n_k? kj
-        bool midk = (!ki) != (!kj);      // n_k above n_i? !ki ; above n_j? !kj
-        const int fij = ST.tet_f[q][0], fik = ST.tet_f[q][1], fjk = ST.tet_f[q][2];
-        bool cl = (midi && ((tT1 >> fjk) & 1ull) && d1slot[fjk] == i) ||
-                  (midj && ((tT1 >> fik) & 1ull) && d1slot[fik] == j) ||
-                  (midk && ((tT1 >> fij) & 1ull) && d1slot[fij] == k);
-        qT1 |= (uint32_t)(((qin >> q) & 1u) && !((qT0 >> q) & 1u) && cl) << q;
-    }
 
     r.tin = tin;
     r.thi = thi;
