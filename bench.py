@@ -48,6 +48,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--bumps", type=int, default=64)
+    ap.add_argument("--field", default="gm", choices=("gm", "fbm"),
+                    help="gm: Gaussian mixture + noise; fbm: Gaussian random field with P(k) ~ k^-3 on [0, 1]")
     ap.add_argument("--rel", type=float, default=1e-2)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--geometry", type=int, default=1, help="1: separatrix geometry, 0: critical cells only")
@@ -87,8 +89,12 @@ def main():
 
     n = args.size
     shape = (n, n, n)
-    log("generating %s Gaussian mixture (%d bumps)" % (shape, args.bumps))
-    f = fields.gaussian_mixture(shape, args.bumps, seed=args.seed, noise=0.01)
+    if args.field == "fbm":
+        log("generating %s fBm field" % (shape,))
+        f = fields.fbm(shape, seed=args.seed, device="cuda:0").cpu().numpy()
+    else:
+        log("generating %s Gaussian mixture (%d bumps)" % (shape, args.bumps))
+        f = fields.gaussian_mixture(shape, args.bumps, seed=args.seed, noise=0.01)
     eps, rng = fields.eps_from_rel(f, args.rel)
     g0 = fields.quantize(f, eps)
     flags = dmtz_amd.CorrectionFlags(preserve_saddles=1, preserve_vpath=1 if args.geometry else 0,
@@ -131,8 +137,10 @@ def main():
     avg_ms = ks["ms"] / launches
     bytes_per_launch = ks["bytes"] / launches
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    workload = "gaussian-mixture %d^3 %s, %d bumps, eps=%g*range, %s" % (
-        n, "float32" if args.dtype == "f32" else "float64", args.bumps, args.rel,
+    kind = "fBm (P(k)~k^-3, [0,1])" if args.field == "fbm" else "gaussian-mixture"
+    workload = "%s %d^3 %s, %seps=%g*range, %s" % (
+        kind, n, "float32" if args.dtype == "f32" else "float64",
+        "" if args.field == "fbm" else "%d bumps, " % args.bumps, args.rel,
         "critical cells + separatrix geometry" if args.geometry else "critical cells")
     roofline = dict(bound="hbm", kernel="k_detect (dense mismatch detection)", achieved=round(achieved, 2),
                     peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 5),

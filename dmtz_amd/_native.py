@@ -37,6 +37,16 @@ class Counts(ctypes.Structure):
         return tuple(int(getattr(self, n)) for n, _ in self._fields_)
 
 
+def _field_struct(name, ctype):
+    return type(name, (ctypes.Structure,), {"_fields_": [
+        ("f", P), ("g", P), ("nx", I64), ("ny", I64), ("nz", I64), ("eps", D), ("range", D),
+        ("on_device", I32), ("reserved", I32)]})
+
+
+# dmtz_field_f32 / dmtz_field_f64: one field of a batch (host or device pointers).
+FieldF32 = _field_struct("FieldF32", ctypes.c_float)
+FieldF64 = _field_struct("FieldF64", ctypes.c_double)
+
 _LIB = None
 
 
@@ -67,6 +77,8 @@ def lib():
                                                   P, P, I64, P]
         getattr(L, "dmtz_targets_" + t).argtypes = [P, P, P, I64, I64, I64, D, D, ctypes.POINTER(Flags),
                                                     P, P]
+        getattr(L, "dmtz_classify_" + t).argtypes = [P, P, I64, I64, I64, P, P, P]
+        getattr(L, "dmtz_correct_batch_" + t).argtypes = [P, P, I32, ctypes.POINTER(Flags), P, P, P]
     L.dmtz_trajectory.argtypes = [P, P, I64]
     L.dmtz_trajectory.restype = I64
     L.dmtz_phase_times.argtypes = [P, P]
@@ -85,4 +97,6 @@ EXPORTS = (
     "dmtz_gradient_f32", "dmtz_gradient_f64",
     "dmtz_targets_f32", "dmtz_targets_f64",
     "dmtz_paths_f32", "dmtz_paths_f64",
+    "dmtz_classify_f32", "dmtz_classify_f64",
+    "dmtz_correct_batch_f32", "dmtz_correct_batch_f64",
 )

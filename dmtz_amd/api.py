@@ -261,3 +261,52 @@ def paths(f, g, flags=None, ctx=None, with_targets=False):
     nds, nas = int(c2[0]), int(c2[1])
     return dict(ds=bufs[0][:nds], ds_rec=bufs[1][:nds], as_=bufs[2][:nas], as_rec=bufs[3][:nas],
                 targets=tg.astype(bool) if tg is not None else None)
+
+
+def classify(field_, ctx=None, masks=False):
+    """The classifier alone: critical cells of one field under the reference's gradient
+    (main_3d.cu:1930-2316). Returns (minima, 1-saddles, 2-saddles, maxima) counts and, with
+    masks=True, per-vertex masks of the owned critical local cells (lo: 51 bits, hi: 24 bits)."""
+    ctx = ctx or default_context()
+    field_ = np.ascontiguousarray(field_)
+    nz, ny, nx = field_.shape
+    counts = np.zeros(4, np.int64)
+    lo = np.zeros(field_.shape, np.uint64) if masks else None
+    hi = np.zeros(field_.shape, np.uint32) if masks else None
+    fn = getattr(ctx._lib, "dmtz_classify_" + _tag(field_.dtype))
+    rc = fn(ctx.handle, _p(field_), nx, ny, nz, _p(counts), _p(lo) if masks else None,
+            _p(hi) if masks else None)
+    ctx.check(rc, "dmtz_classify")
+    out = tuple(int(x) for x in counts)
+    return (out, lo, hi) if masks else out
+
+
+def correct_batch(items, flags=None, ctx=None):
+    """Correct several fields with the same flags in one call. items: list of (f, g, eps, rng)
+    numpy arrays (g is copied, not modified). Returns a list of CorrectionResult (without
+    trajectories)."""
+    ctx = ctx or default_context()
+    flags = flags or CorrectionFlags()
+    if not items:
+        return []
+    dt = np.ascontiguousarray(items[0][0]).dtype
+    tag = _tag(dt)
+    FieldT = N.FieldF32 if tag == "f32" else N.FieldF64
+    keep, arr = [], (FieldT * len(items))()
+    for i, (f, g, eps, rng) in enumerate(items):
+        f = np.ascontiguousarray(f, dtype=dt)
+        g = np.array(g, dtype=dt, copy=True, order="C")
+        keep.append((f, g))
+        nz, ny, nx = f.shape
+        arr[i] = FieldT(_p(f).value, _p(g).value, nx, ny, nz, float(eps), float(rng), 0, 0)
+    n = len(items)
+    counts = (N.Counts * n)()
+    status = np.zeros(n, np.int32)
+    sweeps = np.zeros(n, np.int32)
+    fl = flags.c()
+    rc = getattr(ctx._lib, "dmtz_correct_batch_" + tag)(ctx.handle, arr, n, ctypes.byref(fl),
+                                                         ctypes.cast(counts, ctypes.c_void_p),
+                                                         _p(status), _p(sweeps))
+    ctx.check(rc, "dmtz_correct_batch")
+    return [CorrectionResult(keep[i][1], counts[i].as_tuple(), int(sweeps[i]), int(status[i]))
+            for i in range(n)]

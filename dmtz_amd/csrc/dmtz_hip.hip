@@ -240,6 +240,29 @@ __global__ __launch_bounds__(BLOCK) void k_critf_own(const StarCode* cf, i64 n, 
     out[v] = m;
 }
 
+// Critical cells per type (vertex, edge, triangle, tet) over all owners: one atomic per wave and type.
+__global__ __launch_bounds__(BLOCK) void k_crit_count(const StarCode* code, i64 n, ull* ctr) {
+    const i64 v = (i64)blockIdx.x * BLOCK + threadIdx.x;
+    int cnt[4] = {0, 0, 0, 0};
+    if (v < n) {
+        uint64_t lo, ilo;
+        uint32_t hi, ihi;
+        code_masks(code[v], lo, hi, ilo, ihi);
+        lo &= ilo;
+        hi &= ihi;
+        cnt[0] = (int)(lo & 1ull);
+        cnt[1] = __popcll((lo >> 1) & 0x3FFFull);
+        cnt[2] = __popcll(lo >> 15);
+        cnt[3] = __popc(hi);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        int tot;
+        wave_excl_scan(cnt[t], tot);
+        if (__lane_id() == 0 && tot) atomicAdd(ctr + t, (ull)tot);
+    }
+}
+
 __global__ __launch_bounds__(BLOCK) void k_critf_spread(const StarCode* cf, Dims d, i64 n, CritMask* out) {
     const i64 v = (i64)blockIdx.x * BLOCK + threadIdx.x;
     if (v >= n) return;
@@ -1832,6 +1855,73 @@ int correct_host(dmtz_ctx* ctx, const T* f, T* g, i64 nx, i64 ny, i64 nz, double
     return st;
 }
 
+// The classifier alone (dmtz_classify_*): star codes of the field, then counts and masks of the
+// cells left unpaired.
+template <class T>
+int classify_host(dmtz_ctx* ctx, const T* fld, i64 nx, i64 ny, i64 nz, int64_t* counts4, uint64_t* crit_lo,
+                  uint32_t* crit_hi) {
+    int rc = check_dims(ctx, nx, ny, nz);
+    if (rc) return rc;
+    if (!fld || !counts4) {
+        ctx->err = "null argument";
+        return DMTZ_E_ARG;
+    }
+    const i64 N = nx * ny * nz;
+    rc = ensure(ctx, N, sizeof(T), true);
+    if (rc) return rc;
+    Dims d{nx, ny, nz};
+    Counts c = make_counts(d);
+    CK(hipMemcpyAsync(ctx->d_f, fld, sizeof(T) * (size_t)N, hipMemcpyHostToDevice, ctx->stream));
+    k_code<T><<<nblocks(N), BLOCK, 0, ctx->stream>>>((const T*)ctx->d_f, d, c, ctx->d_code_f);
+    CK(hipGetLastError());
+    CK(hipMemsetAsync(ctx->d_ctr, 0, sizeof(ull) * 4, ctx->stream));
+    k_crit_count<<<nblocks(N), BLOCK, 0, ctx->stream>>>(ctx->d_code_f, N, ctx->d_ctr);
+    CK(hipGetLastError());
+    rc = sync_ctr(ctx);
+    if (rc) return rc;
+    for (int k = 0; k < 4; k++) counts4[k] = (int64_t)ctx->h_ctr[k];
+    if (crit_lo || crit_hi) {
+        k_critf_own<<<nblocks(N), BLOCK, 0, ctx->stream>>>(ctx->d_code_f, N, ctx->d_critf);
+        CK(hipGetLastError());
+        std::vector<CritMask> hm((size_t)N);
+        CK(hipMemcpyAsync(hm.data(), ctx->d_critf, sizeof(CritMask) * (size_t)N, hipMemcpyDeviceToHost,
+                          ctx->stream));
+        CK(hipStreamSynchronize(ctx->stream));
+        for (i64 v = 0; v < N; v++) {
+            if (crit_lo) crit_lo[v] = hm[(size_t)v].lo;
+            if (crit_hi) crit_hi[v] = hm[(size_t)v].hi;
+        }
+    }
+    return 0;
+}
+
+// Batch correction (dmtz_correct_batch_*): the fields one after another on the context.
+template <class T, class Field>
+int correct_batch(dmtz_ctx* ctx, const Field* fields, int32_t n, const dmtz_flags* flags, dmtz_counts* counts_out,
+                  int32_t* status_out, int32_t* sweeps_out) {
+    if (!ctx || !flags || (n > 0 && !fields) || n < 0) {
+        if (ctx) ctx->err = "null or invalid argument";
+        return DMTZ_E_ARG;
+    }
+    int worst = 0;
+    for (int32_t i = 0; i < n; i++) {
+        const Field& fd = fields[i];
+        dmtz_counts cnt;
+        int32_t sw = 0;
+        const int rc = fd.on_device
+                           ? correct_dev<T>(ctx, fd.f, fd.g, fd.nx, fd.ny, fd.nz, fd.eps, fd.range, flags, &cnt,
+                                            nullptr, &sw)
+                           : correct_host<T>(ctx, fd.f, fd.g, fd.nx, fd.ny, fd.nz, fd.eps, fd.range, flags, &cnt,
+                                             nullptr, &sw);
+        if (status_out) status_out[i] = rc;
+        if (rc < 0) return rc;
+        if (counts_out) counts_out[i] = cnt;
+        if (sweeps_out) sweeps_out[i] = sw;
+        if (rc != 0) worst = 1;
+    }
+    return worst;
+}
+
 template <class T>
 int gradient_host(dmtz_ctx* ctx, const T* fld, i64 nx, i64 ny, i64 nz, int64_t* pv, int64_t* pe, int64_t* pf,
                   int64_t* pt) {
@@ -2094,6 +2184,32 @@ int dmtz_kernel_stats(dmtz_ctx* ctx, double* out4, int reset) {
     out4[3] = ctx->det_voxels;
     if (reset) ctx->det_launches = ctx->det_ms = ctx->det_bytes = ctx->det_voxels = 0;
     return 0;
+}
+
+int dmtz_correct_batch_f32(dmtz_ctx* ctx, const dmtz_field_f32* fields, int32_t nfields, const dmtz_flags* flags,
+                           dmtz_counts* counts_out, int32_t* status_out, int32_t* sweeps_out) {
+    if (!ctx) return DMTZ_E_ARG;
+    (void)hipSetDevice(ctx->device);
+    return correct_batch<float>(ctx, fields, nfields, flags, counts_out, status_out, sweeps_out);
+}
+int dmtz_correct_batch_f64(dmtz_ctx* ctx, const dmtz_field_f64* fields, int32_t nfields, const dmtz_flags* flags,
+                           dmtz_counts* counts_out, int32_t* status_out, int32_t* sweeps_out) {
+    if (!ctx) return DMTZ_E_ARG;
+    (void)hipSetDevice(ctx->device);
+    return correct_batch<double>(ctx, fields, nfields, flags, counts_out, status_out, sweeps_out);
+}
+
+int dmtz_classify_f32(dmtz_ctx* ctx, const float* field, int64_t nx, int64_t ny, int64_t nz, int64_t* counts4,
+                      uint64_t* crit_lo, uint32_t* crit_hi) {
+    if (!ctx) return DMTZ_E_ARG;
+    (void)hipSetDevice(ctx->device);
+    return classify_host<float>(ctx, field, nx, ny, nz, counts4, crit_lo, crit_hi);
+}
+int dmtz_classify_f64(dmtz_ctx* ctx, const double* field, int64_t nx, int64_t ny, int64_t nz, int64_t* counts4,
+                      uint64_t* crit_lo, uint32_t* crit_hi) {
+    if (!ctx) return DMTZ_E_ARG;
+    (void)hipSetDevice(ctx->device);
+    return classify_host<double>(ctx, field, nx, ny, nz, counts4, crit_lo, crit_hi);
 }
 
 int dmtz_run_info(const dmtz_ctx* ctx, int64_t* out4) {

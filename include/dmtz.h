@@ -99,6 +99,48 @@ int dmtz_correct_dev_f64(dmtz_ctx* ctx, const double* d_f, double* d_g, int64_t 
                          double eps, double range, const dmtz_flags* flags, dmtz_counts* out,
                          uint8_t* d_counter_out, int32_t* sweeps_out);
 
+/* One field of a batch: f and g (corrected in place) in host memory, or in device memory on the
+ * context's device when on_device != 0; eps and range as for dmtz_correct_*. */
+typedef struct dmtz_field_f32 {
+    const float* f;
+    float* g;
+    int64_t nx, ny, nz;
+    double eps, range;
+    int32_t on_device;
+    int32_t reserved;
+} dmtz_field_f32;
+typedef struct dmtz_field_f64 {
+    const double* f;
+    double* g;
+    int64_t nx, ny, nz;
+    double eps, range;
+    int32_t on_device;
+    int32_t reserved;
+} dmtz_field_f64;
+
+/* Batch correction: every field of the array is corrected with the same flags, one after another
+ * on the context's stream and reusing its memory (the reference runs one field per process,
+ * main_3d.cu:6255). counts_out, status_out and sweeps_out (each nullable) receive one entry per
+ * field; status_out[i] is field i's return code. Returns 0 when every field converged, 1 when one
+ * did not (not converged or stalled), or the first negative error (the fields after it are left
+ * untouched). */
+int dmtz_correct_batch_f32(dmtz_ctx* ctx, const dmtz_field_f32* fields, int32_t nfields, const dmtz_flags* flags,
+                           dmtz_counts* counts_out, int32_t* status_out, int32_t* sweeps_out);
+int dmtz_correct_batch_f64(dmtz_ctx* ctx, const dmtz_field_f64* fields, int32_t nfields, const dmtz_flags* flags,
+                           dmtz_counts* counts_out, int32_t* status_out, int32_t* sweeps_out);
+
+/* The classifier alone: critical cells of one field (host pointer) under the reference's
+ * five-pass gradient (main_3d.cu:1930-2316), i.e. the cells left unpaired (-1) in paired_vertices
+ * / paired_edges / paired_faces / paired_tetrahedras. counts4 = {minima, 1-saddles, 2-saddles,
+ * maxima}. crit_lo / crit_hi (nullable, N entries each) receive per vertex the critical cells it
+ * owns as a 75-bit mask over its local cells: bit 0 = the vertex, bits 1..14 = the edge to
+ * neighbour slot a (1 + a), bits 15..50 = star triangle t (15 + t) in crit_lo, and star tet q in
+ * bit q of crit_hi; the slots follow the stencil order of main_3d.cu:44-51. */
+int dmtz_classify_f32(dmtz_ctx* ctx, const float* field, int64_t nx, int64_t ny, int64_t nz, int64_t* counts4,
+                      uint64_t* crit_lo, uint32_t* crit_hi);
+int dmtz_classify_f64(dmtz_ctx* ctx, const double* field, int64_t nx, int64_t ny, int64_t nz, int64_t* counts4,
+                      uint64_t* crit_lo, uint32_t* crit_hi);
+
 /* Per-sweep counts of the last correction on ctx: 8 int64 per sweep (dmtz_counts order), up to
  * cap sweeps; returns the number of sweeps recorded. Mirrors the per-sweep prints of
  * main_3d.cu:6787-6791, 7146-7147. */

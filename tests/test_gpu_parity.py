@@ -161,3 +161,48 @@ def test_separatrix_lists_bit_exact(oracle, dev, geometry):
             L.orc_path_targets_f32(f.ctypes.data_as(P), g.ctypes.data_as(P), W, H, D, eps, rng,
                                    ctypes.byref(ofl), tg.ctypes.data_as(P), c2.ctypes.data_as(P))
             assert (got["targets"].ravel() == tg.astype(bool)).all()
+
+
+def test_classify_matches_oracle(oracle, dev):
+    """dmtz_classify: critical-cell counts per type equal the -1 entries of the oracle's pairing
+    arrays, the per-vertex masks list exactly the critical minima, and the critical edges the
+    masks name are the oracle's."""
+    for f in _gradient_cases():
+        print("classify", f.shape, f.dtype, flush=True)
+        pv, pe, pf, pt = oracle.gradient(f)
+        counts, lo, hi = dmtz_amd.classify(f, ctx=dev, masks=True)
+        assert counts == (int((pv == -1).sum()), int((pe == -1).sum()), int((pf == -1).sum()),
+                          int((pt == -1).sum()))
+        lo, hi = lo.reshape(-1), hi.reshape(-1)
+        assert ((lo & 1).astype(bool) == (pv == -1)).all()
+        bits = np.array([bin(int(x)).count("1") for x in lo]) + np.array([bin(int(x)).count("1") for x in hi])
+        assert bits.sum() == sum(counts)
+        D, H, W = f.shape
+        dirs = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (-1, 1, 0), (1, -1, 0),
+                (0, 1, 1), (0, -1, -1), (-1, 0, 1), (1, 0, -1), (-1, 1, 1), (1, -1, -1)]
+        edges = set()
+        for v in np.nonzero((lo >> 1) & 0x3FFF)[0]:
+            x, y, z = v % W, (v // W) % H, v // (W * H)
+            for a in range(14):
+                if (int(lo[v]) >> (1 + a)) & 1:
+                    dx, dy, dz = dirs[a]
+                    u = (x + dx) + (y + dy) * W + (z + dz) * W * H
+                    edges.add(oracle.edge_id((W, H, D), int(v), int(u)))
+        assert edges == set(np.nonzero(pe == -1)[0].tolist())
+
+
+def test_batch_equals_single_corrections(dev):
+    """dmtz_correct_batch over fields of different shapes gives each field exactly what a single
+    correction gives."""
+    items = []
+    for seed, shape, rel in [(0, (16, 16, 16), 1e-2), (1, (24, 20, 12), 2e-2), (2, (13, 17, 11), 5e-2),
+                             (3, (9, 30, 7), 3e-2)]:
+        f = fields.gaussian_mixture(shape, 6, seed=seed, noise=0.02)
+        eps, rng = fields.eps_from_rel(f, rel)
+        items.append((f, fields.quantize(f, eps), eps, rng))
+    fl = dmtz_amd.CorrectionFlags(preserve_vpath=1, preserve_geometry=1, finish=1)
+    batch = dmtz_amd.correct_batch(items, fl, ctx=dev)
+    for (f, g, eps, rng), b in zip(items, batch):
+        one = dmtz_amd.correct(f, g, eps, rng, fl, ctx=dev)
+        assert b.status == one.status and b.sweeps == one.sweeps and b.counts == one.counts
+        assert b.g.tobytes() == one.g.tobytes()
