@@ -206,3 +206,84 @@ def test_batch_equals_single_corrections(dev):
         one = dmtz_amd.correct(f, g, eps, rng, fl, ctx=dev)
         assert b.status == one.status and b.sweeps == one.sweeps and b.counts == one.counts
         assert b.g.tobytes() == one.g.tobytes()
+
+
+DIRS14 = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (-1, 1, 0), (1, -1, 0),
+          (0, 1, 1), (0, -1, -1), (-1, 0, 1), (1, 0, -1), (-1, 1, 1), (1, -1, -1)]
+
+
+def _local_cells():
+    """Star triangles and tets of the local-cell numbering (slot pairs / triples, adjacent slots),
+    in the order of the classifier's masks."""
+    dirs = set(DIRS14)
+
+    def adj(i, j):
+        return tuple(a - b for a, b in zip(DIRS14[i], DIRS14[j])) in dirs
+
+    tris = [(i, j) for i in range(14) for j in range(i + 1, 14) if adj(i, j)]
+    tets = [(i, j, k) for i in range(14) for j in range(i + 1, 14) for k in range(j + 1, 14)
+            if adj(i, j) and adj(i, k) and adj(j, k)]
+    return tris, tets
+
+
+def _critical_cells(lo, hi, shape):
+    """Critical cells named by per-owner masks, as sorted rows of vertex ids per type."""
+    D, H, W = shape
+    lo, hi = lo.reshape(-1), hi.reshape(-1)
+    off = [dx + dy * W + dz * W * H for dx, dy, dz in DIRS14]
+    tris, tets = _local_cells()
+    assert len(tris) == 36 and len(tets) == 24
+
+    def rows(groups):
+        if not groups:
+            return np.zeros((0, 1), np.int64)
+        r = np.sort(np.concatenate(groups, axis=0), axis=1)
+        return r[np.lexsort(r.T[::-1])]
+
+    out = [np.sort(np.nonzero(lo & 1)[0])]
+    out.append(rows([np.stack([v, v + off[a]], 1) for a in range(14)
+                     for v in [np.nonzero((lo >> np.uint64(1 + a)) & np.uint64(1))[0]]]))
+    out.append(rows([np.stack([v, v + off[i], v + off[j]], 1) for t, (i, j) in enumerate(tris)
+                     for v in [np.nonzero((lo >> np.uint64(15 + t)) & np.uint64(1))[0]]]))
+    out.append(rows([np.stack([v, v + off[i], v + off[j], v + off[k]], 1) for q, (i, j, k) in enumerate(tets)
+                     for v in [np.nonzero((hi >> np.uint32(q)) & np.uint32(1))[0]]]))
+    return out
+
+
+@pytest.mark.timeout(400)
+def test_fbm_512_converges_with_f_critical_cells(dev):
+    """The 512^3 fBm workload (P(k) ~ k^-3 on [0, 1], eps = 1e-3 range, separatrix geometry,
+    finisher), too large for the oracle: the run ends at zero mismatches, every value stays within
+    eps, and the classifier finds exactly f's critical cells in the corrected field (compared as
+    cells: a cell's owner, its highest vertex, may differ between f and g)."""
+    f = fields.fbm((512, 512, 512), seed=0)
+    eps, rng = fields.eps_from_rel(f, 1e-3)
+    g = fields.quantize(f, eps)
+    fl = dmtz_amd.CorrectionFlags(preserve_vpath=1, preserve_geometry=1, finish=1, max_sweeps=2000)
+    print("fbm 512^3 correction", flush=True)
+    r = dmtz_amd.correct(f, g, eps, rng, fl, ctx=dev)
+    assert r.status == 0 and r.counts == (0,) * 8
+    assert np.abs(r.g.astype(np.float64) - f).max() <= eps
+    cf, lof, hif = dmtz_amd.classify(f, ctx=dev, masks=True)
+    cg, log_, hig = dmtz_amd.classify(r.g, ctx=dev, masks=True)
+    assert cf == cg
+    for a, b in zip(_critical_cells(lof, hif, f.shape), _critical_cells(log_, hig, f.shape)):
+        assert np.array_equal(a, b)
+
+
+def test_critical_cell_sets_small(oracle, dev):
+    """The cell-set form of the classifier's masks (used at 512^3) against the oracle's critical
+    sets on small fields, so the comparison above is itself checked."""
+    for f in _gradient_cases()[:6]:
+        D, H, W = f.shape
+        pv, pe, pf, pt = oracle.gradient(f)
+        _, lo, hi = dmtz_amd.classify(f, ctx=dev, masks=True)
+        cells = _critical_cells(lo, hi, f.shape)
+        dims = (W, H, D)
+        assert cells[0].tolist() == np.nonzero(pv == -1)[0].tolist()
+        want_e = sorted(tuple(sorted(oracle.edge_verts(dims, int(e)))) for e in np.nonzero(pe == -1)[0])
+        assert [tuple(r) for r in cells[1].tolist()] == want_e
+        want_t = sorted(tuple(sorted(oracle.tri_verts(dims, int(t)))) for t in np.nonzero(pf == -1)[0])
+        assert [tuple(r) for r in cells[2].tolist()] == want_t
+        want_q = sorted(tuple(sorted(oracle.tet_verts(dims, int(q)))) for q in np.nonzero(pt == -1)[0])
+        assert [tuple(r) for r in cells[3].tolist()] == want_q
