@@ -7,9 +7,9 @@ the C ABI of include/dmtz.h; this package is the Python host side.
 """
 from .api import (CONVERGED, NOT_CONVERGED, STALLED, Context, CorrectionFlags, CorrectionResult, DmtzError,
                   classify, correct, correct_batch, correct_device, default_context, eps_from_rel, gradient,
-                  paths, targets)
+                  paths, targets, walls)
 from . import fields
 
 __all__ = ["CONVERGED", "NOT_CONVERGED", "STALLED", "Context", "CorrectionFlags", "CorrectionResult",
            "DmtzError", "classify", "correct", "correct_batch", "correct_device", "default_context",
-           "eps_from_rel", "gradient", "paths", "targets", "fields"]
+           "eps_from_rel", "gradient", "paths", "targets", "walls", "fields"]

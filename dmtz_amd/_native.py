@@ -78,6 +78,8 @@ def lib():
         getattr(L, "dmtz_targets_" + t).argtypes = [P, P, P, I64, I64, I64, D, D, ctypes.POINTER(Flags),
                                                     P, P]
         getattr(L, "dmtz_classify_" + t).argtypes = [P, P, I64, I64, I64, P, P, P]
+        getattr(L, "dmtz_walls_" + t).argtypes = [P, P, P, I64, I64, I64, D, D, ctypes.POINTER(Flags), P, P, P,
+                                                  I64, P]
         getattr(L, "dmtz_correct_batch_" + t).argtypes = [P, P, I32, ctypes.POINTER(Flags), P, P, P]
     L.dmtz_trajectory.argtypes = [P, P, I64]
     L.dmtz_trajectory.restype = I64
@@ -99,4 +101,5 @@ EXPORTS = (
     "dmtz_paths_f32", "dmtz_paths_f64",
     "dmtz_classify_f32", "dmtz_classify_f64",
     "dmtz_correct_batch_f32", "dmtz_correct_batch_f64",
+    "dmtz_walls_f32", "dmtz_walls_f64",
 )

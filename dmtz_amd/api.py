@@ -263,6 +263,28 @@ def paths(f, g, flags=None, ctx=None, with_targets=False):
                 targets=tg.astype(bool) if tg is not None else None)
 
 
+def walls(f, g, eps=0.0, rng=0.0, flags=None, ctx=None, with_targets=False):
+    """2-saddle walls on (f, g) (get_false_ds_wall main_3d.cu:5064-5133, made deterministic):
+    dict(face=(f-critical face ids), edge=(first edge whose f and g partners differ),
+    targets=edit set of fix_ds_wall main_3d.cu:5297-5413 or None). eps and range enter the
+    repair's threshold only."""
+    ctx = ctx or default_context()
+    f = np.ascontiguousarray(f)
+    g = np.ascontiguousarray(g, dtype=f.dtype)
+    nz, ny, nx = f.shape
+    cap = 2 * f.size + 16
+    face = np.zeros(cap, np.int64)
+    edge = np.zeros(cap, np.int64)
+    n = np.zeros(1, np.int64)
+    tg = np.zeros(f.shape, np.uint8) if with_targets else None
+    fl = (flags or CorrectionFlags()).c()
+    fn = getattr(ctx._lib, "dmtz_walls_" + _tag(f.dtype))
+    ctx.check(fn(ctx.handle, _p(f), _p(g), nx, ny, nz, float(eps), float(rng), ctypes.byref(fl), _p(n), _p(face),
+                 _p(edge), cap, _p(tg) if tg is not None else None), "dmtz_walls")
+    k = int(n[0])
+    return dict(face=face[:k], edge=edge[:k], targets=tg.astype(bool) if tg is not None else None)
+
+
 def classify(field_, ctx=None, masks=False):
     """The classifier alone: critical cells of one field under the reference's gradient
     (main_3d.cu:1930-2316). Returns (minima, 1-saddles, 2-saddles, maxima) counts and, with

@@ -52,7 +52,10 @@ enum : int {
     K_WALK = 12,                                // capped walks
     K_CHG = 13, K_DIRTY = 14,                   // changed vertices, dirty stars
     K_FINISH = 15,                              // finisher tie-repair marks
-    K_NCTR = 16
+    K_STARS = 16,                               // star codes re-evaluated from dirty lists
+    K_DSW = 17,                                 // 2-saddle wall mismatch list
+    K_WOVF = 18,                                // wall walks that outgrew their visited set
+    K_NCTR = 19
 };
 
 constexpr uint8_t FROZEN = 7;  // counter value of a vertex restored to f by the finisher
@@ -218,11 +221,18 @@ __global__ __launch_bounds__(BLOCK, CODE_BLOCKS_PER_CU) void k_code_g(const T* g
     if (v < c.N) code_g_at(g, d, v, code, crit_f, mm);
 }
 
+// The list length is read on the device (*n_ptr, written by the kernel that filled the list), so
+// the host launches it without waiting; the grid strides over the list. Block 0 adds the length
+// to *stars, the count of star codes re-evaluated in the run.
 template <class T>
-__global__ __launch_bounds__(BLOCK, CODE_BLOCKS_PER_CU) void k_code_list_g(const T* g, Dims d, StarCode* code, const CritMask* crit_f,
-                                                       CritMask* mm, const uint32_t* list, ull n) {
-    const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
-    if (i < n) code_g_at(g, d, (i64)list[i], code, crit_f, mm);
+__global__ __launch_bounds__(BLOCK, CODE_BLOCKS_PER_CU) void k_code_list_g(const T* g, Dims d, StarCode* code,
+                                                                         const CritMask* crit_f, CritMask* mm,
+                                                                         const uint32_t* list, const ull* n_ptr,
+                                                                         ull* stars) {
+    const ull n = *n_ptr;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(stars, n);
+    for (ull i = (ull)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (ull)gridDim.x * BLOCK)
+        code_g_at(g, d, (i64)list[i], code, crit_f, mm);
 }
 
 // f-critical status of the local cells of every vertex (once per run), in two passes: each
@@ -353,42 +363,47 @@ __global__ __launch_bounds__(BLOCK) void k_fix_cp(const uint64_t* recs, ull n, c
 }
 
 // Frozen vertices are never edited. moved counts the edits that changed a value or a counter.
+// The edit set's size is read on the device (*n_ptr, at most cap); the grid strides over the set
+// in block-uniform steps, so every lane takes part in the wave-wide appends.
 template <class T>
-__global__ __launch_bounds__(BLOCK) void k_apply(const uint32_t* list, ull n, const T* f, T* g, uint8_t* counter,
-                                                 double eps, double range, int preserve_type, int edit_type,
-                                                 uint32_t* changed, ull* chg_ctr, ull* moved) {
-    const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
-    bool ch = false, mv = false;
-    i64 v = 0;
-    if (i < n) {
-        v = list[i];
-        const int k0 = counter[v];
-        if (k0 != FROZEN) {
-            const T fv = f[v];
-            const T g0 = g[v];
-            T gv = g0;
-            int k = k0;
-            const double dl = edit_type == 1 ? halving_delta(eps, k) : EditArith<T>::delta0(fv, gv, eps);
-            if (preserve_type == 0) {
-                if (fabs_d(dl) > 1e-15 * range && k < 5 && EditArith<T>::step(fv, gv, dl, eps)) {
-                    k += 1;
+__global__ __launch_bounds__(BLOCK) void k_apply(const uint32_t* list, const ull* n_ptr, ull cap, const T* f, T* g,
+                                                 uint8_t* counter, double eps, double range, int preserve_type,
+                                                 int edit_type, uint32_t* changed, ull* chg_ctr, ull* moved) {
+    const ull n = *n_ptr < cap ? *n_ptr : cap;
+    for (ull base = (ull)blockIdx.x * BLOCK; base < n; base += (ull)gridDim.x * BLOCK) {
+        const ull i = base + threadIdx.x;
+        bool ch = false, mv = false;
+        i64 v = 0;
+        if (i < n) {
+            v = list[i];
+            const int k0 = counter[v];
+            if (k0 != FROZEN) {
+                const T fv = f[v];
+                const T g0 = g[v];
+                T gv = g0;
+                int k = k0;
+                const double dl = edit_type == 1 ? halving_delta(eps, k) : EditArith<T>::delta0(fv, gv, eps);
+                if (preserve_type == 0) {
+                    if (fabs_d(dl) > 1e-15 * range && k < 5 && EditArith<T>::step(fv, gv, dl, eps)) {
+                        k += 1;
+                    } else {
+                        k = 6;
+                        gv = EditArith<T>::snap(fv, eps);
+                    }
                 } else {
                     k = 6;
                     gv = EditArith<T>::snap(fv, eps);
                 }
-            } else {
-                k = 6;
-                gv = EditArith<T>::snap(fv, eps);
+                g[v] = gv;
+                counter[v] = (uint8_t)k;
+                ch = !(gv == g0);  // the order (hence every lower star) only sees value comparisons
+                mv = ch || k != k0;
             }
-            g[v] = gv;
-            counter[v] = (uint8_t)k;
-            ch = !(gv == g0);  // the order (hence every lower star) only sees value comparisons
-            mv = ch || k != k0;
         }
+        ull at = wave_alloc(chg_ctr, ch ? 1 : 0);
+        if (ch) changed[at] = (uint32_t)v;
+        wave_add(moved, mv);
     }
-    ull at = wave_alloc(chg_ctr, ch ? 1 : 0);
-    if (ch) changed[at] = (uint32_t)v;
-    wave_add(moved, mv);
 }
 
 // Finisher zone, part 1: the vertices of the first n[k] records of the four mismatch lists, each
@@ -523,27 +538,31 @@ __global__ __launch_bounds__(BLOCK) void k_freeze_all(const T* f, T* g, uint8_t*
 
 // Closed stars of the changed vertices: their lower stars must be re-evaluated. One thread per
 // (changed vertex, star slot); each vertex enters the dirty list once per epoch.
-__global__ __launch_bounds__(BLOCK) void k_dirty(const uint32_t* changed, ull n, Dims d, uint32_t* stamp,
+// The changed count is read on the device (*n_ptr); the grid strides in block-uniform steps.
+__global__ __launch_bounds__(BLOCK) void k_dirty(const uint32_t* changed, const ull* n_ptr, Dims d, uint32_t* stamp,
                                                  uint32_t epoch, uint32_t* dirty, ull* dirty_ctr) {
-    const ull j = (ull)blockIdx.x * BLOCK + threadIdx.x;
-    bool fresh = false;
-    i64 w = 0;
-    if (j < 15 * n) {
-        const i64 v = changed[j / 15];
-        const int slot = (int)(j % 15);
-        i64 x, y, z;
-        vxyz(d, v, x, y, z);
-        bool ok = true;
-        w = v;
-        if (slot < 14) {
-            i64 nx = x + DIR[slot][0], ny = y + DIR[slot][1], nz = z + DIR[slot][2];
-            ok = inb(d, nx, ny, nz);
-            w = ok ? vidx(d, nx, ny, nz) : 0;
+    const ull n = *n_ptr;
+    for (ull base = (ull)blockIdx.x * BLOCK; base < 15 * n; base += (ull)gridDim.x * BLOCK) {
+        const ull j = base + threadIdx.x;
+        bool fresh = false;
+        i64 w = 0;
+        if (j < 15 * n) {
+            const i64 v = changed[j / 15];
+            const int slot = (int)(j % 15);
+            i64 x, y, z;
+            vxyz(d, v, x, y, z);
+            bool ok = true;
+            w = v;
+            if (slot < 14) {
+                i64 nx = x + DIR[slot][0], ny = y + DIR[slot][1], nz = z + DIR[slot][2];
+                ok = inb(d, nx, ny, nz);
+                w = ok ? vidx(d, nx, ny, nz) : 0;
+            }
+            if (ok) fresh = atomicExch(stamp + w, epoch) != epoch;
         }
-        if (ok) fresh = atomicExch(stamp + w, epoch) != epoch;
+        ull at = wave_alloc(dirty_ctr, fresh ? 1 : 0);
+        if (fresh) dirty[at] = (uint32_t)w;
     }
-    ull at = wave_alloc(dirty_ctr, fresh ? 1 : 0);
-    if (fresh) dirty[at] = (uint32_t)w;
 }
 
 // Filtration main_3d.cu:2405-2490: every vertex of an f-critical cell is pinned to f - eps.
@@ -761,20 +780,22 @@ __global__ __launch_bounds__(BLOCK) void k_as_check(ull nb, const ull* off, cons
     res[2 * j + 1] = dv;
 }
 
-// Mark a dirty vertex and the (up to 8) cubes that have it as a corner.
-__global__ __launch_bounds__(BLOCK) void k_cube_mark(const uint32_t* dirty, ull n, Dims d, uint32_t* cubes,
-                                                     uint32_t* vbits) {
-    const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t v = dirty[i];
-    atomicOr(vbits + (v >> 5), 1u << (v & 31u));
-    i64 x, y, z;
-    vxyz(d, (i64)v, x, y, z);
-    for (int k = 0; k < 8; k++) {
-        const i64 ax = x - (k & 1), ay = y - ((k >> 1) & 1), az = z - ((k >> 2) & 1);
-        if (ax < 0 || ay < 0 || az < 0 || ax > d.W - 2 || ay > d.H - 2 || az > d.D - 2) continue;
-        const ull cube = (ull)(az * (d.W - 1) * (d.H - 1) + ay * (d.W - 1) + ax);
-        atomicOr(cubes + (cube >> 5), 1u << (cube & 31u));
+// Mark each dirty vertex and the (up to 8) cubes that have it as a corner. The list length is
+// read on the device (*n_ptr).
+__global__ __launch_bounds__(BLOCK) void k_cube_mark(const uint32_t* dirty, const ull* n_ptr, Dims d,
+                                                     uint32_t* cubes, uint32_t* vbits) {
+    const ull n = *n_ptr;
+    for (ull i = (ull)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (ull)gridDim.x * BLOCK) {
+        const uint32_t v = dirty[i];
+        atomicOr(vbits + (v >> 5), 1u << (v & 31u));
+        i64 x, y, z;
+        vxyz(d, (i64)v, x, y, z);
+        for (int k = 0; k < 8; k++) {
+            const i64 ax = x - (k & 1), ay = y - ((k >> 1) & 1), az = z - ((k >> 2) & 1);
+            if (ax < 0 || ay < 0 || az < 0 || ax > d.W - 2 || ay > d.H - 2 || az > d.D - 2) continue;
+            const ull cube = (ull)(az * (d.W - 1) * (d.H - 1) + ay * (d.W - 1) + ax);
+            atomicOr(cubes + (cube >> 5), 1u << (cube & 31u));
+        }
     }
 }
 
@@ -858,16 +879,53 @@ __global__ __launch_bounds__(BLOCK) void k_ds_combine(const uint64_t* fce, ull n
     out.push_wave(has, e, rec);
 }
 
-// fix_as_vpath / fix_ds_vpath main_3d.cu:5135-5295
+// 2-saddle walls, get_false_ds_wall main_3d.cu:5064-5133, made deterministic: from each
+// f-critical face, a depth-first walk over f's face -> edge -> paired-face steps with a stack and
+// a visited set of its own (the reference shares one stack between threads, defects D2-D5).
+// A face is marked when popped; its edges are taken in the order (v1,v2), (v1,v3), (v2,v3) and
+// pushed in that order, so the last is popped first. The first edge met whose f partner exists
+// and differs from its g partner is recorded as (face, edge), and the walk stops.
+//
+// Thread t of the launch walks start first + t (or sel[first + t]) with its own part of scratch
+// (2 * slots + 2 entries: the visited set and the stack of Repair::wall_walk). A walk that
+// outgrows its set is listed in ovf, to be walked again with more slots.
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_ds_wall(const uint64_t* starts, const uint32_t* sel, ull first, ull n,
+                                                   const T* f, const StarCode* cf, const T* g, const StarCode* cg,
+                                                   Dims d, Counts c, uint32_t* scratch, uint32_t slots, PathList out,
+                                                   uint32_t* ovf, ull* ovf_ctr) {
+    const ull t = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    if (first + t >= n) return;
+    const ull s = sel ? (ull)sel[first + t] : first + t;
+    const uint64_t r = starts[s];
+    const i64 face = local_cell_id(d, c, (i64)(r >> 7), (int)(r & 127u));
+    Repair<T, NoSink> R(f, cf, g, cg, d, c, nullptr, 0, 0, 0, NoSink{});
+    uint32_t* set = scratch + (size_t)t * (2 * (size_t)slots + 2);
+    i64 found = -1;
+    const bool overflow = R.wall_walk(face, set, set + slots, slots, found);
+    if (overflow) {
+        ovf[atomicAdd(ovf_ctr, 1ull)] = (uint32_t)s;
+    } else if (found >= 0) {
+        const ull at = atomicAdd(out.ctr, 1ull);
+        if (at < out.cap) {
+            out.id[at] = face;
+            out.rec[at] = found;
+        }
+    }
+}
+
+// fix_as_vpath / fix_ds_vpath / fix_ds_wall main_3d.cu:5135-5413
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_fix_paths(const int64_t* as_rec, ull n_as, const int64_t* ds_rec, ull n_ds,
-                                                     const T* f, const StarCode* cf, const T* g, const StarCode* cg,
-                                                     Dims d, Counts c, const uint8_t* counter, double eps, double range,
+                                                     const int64_t* wall_rec, ull n_wall, const T* f,
+                                                     const StarCode* cf, const T* g, const StarCode* cg, Dims d,
+                                                     Counts c, const uint8_t* counter, double eps, double range,
                                                      int edit_type, TargetSink sink) {
     const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
     Repair<T, TargetSink> R(f, cf, g, cg, d, c, counter, eps, range, edit_type, sink);
     if (i < n_as) R.fix_as_vpath(as_rec[i]);
     else if (i < n_as + n_ds) R.fix_ds_vpath(ds_rec[i - n_as]);
+    else if (i < n_as + n_ds + n_wall) R.fix_ds_wall(wall_rec[i - n_as - n_ds]);
 }
 
 // Dense pairing arrays for tests (dmtz_gradient_*): one thread per cell of each dimension.
@@ -886,6 +944,15 @@ __global__ __launch_bounds__(BLOCK) void k_pairing(const T* fld, const StarCode*
 }
 
 inline unsigned nblocks(ull n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
+
+// Grid of a kernel that reads its element count on the device and strides over it: sized for
+// the host's estimate of the count, at least one block and at most STRIDE_BLOCKS (eight blocks
+// per CU of a 256-CU device).
+constexpr ull STRIDE_BLOCKS = 2048;
+inline unsigned stride_blocks(ull estimate) {
+    const ull b = nblocks(estimate);
+    return (unsigned)(b < 1 ? 1 : (b > STRIDE_BLOCKS ? STRIDE_BLOCKS : b));
+}
 
 }  // namespace
 
@@ -912,7 +979,11 @@ struct dmtz_ctx {
     CritMask* d_mm = nullptr;       // mismatched cells each vertex owns in g
     uint64_t* d_rec = nullptr;    // 4 mismatch record lists of rec_cap entries
     uint64_t* d_fc = nullptr;     // 4 f-critical lists of fc_cap entries
-    int64_t* d_path = nullptr;    // ds id/rec, as id/rec (fc_cap each)
+    int64_t* d_path = nullptr;    // ds id/rec, as id/rec, wall id/rec (fc_cap each)
+    uint32_t* d_wall_scr = nullptr;  // wall walks: per-thread visited sets and stacks
+    size_t wall_scr_cap = 0;
+    uint32_t* d_wall_ovf = nullptr;  // wall walks to repeat with more slots (two lists)
+    size_t wall_ovf_cap = 0;
     int64_t* d_labels = nullptr;  // min labels (2 per f-critical edge), max labels (2 per face)
     int64_t* d_walk = nullptr;    // per-branch walk results (4 per start cell)
     ull* d_ctr = nullptr;
@@ -982,6 +1053,10 @@ void free_paths(dmtz_ctx* ctx) {
     (void)hipFree(ctx->d_dse);
     (void)hipFree(ctx->d_ds_res);
     (void)hipFree(ctx->d_scan_tmp);
+    (void)hipFree(ctx->d_wall_scr);
+    (void)hipFree(ctx->d_wall_ovf);
+    ctx->d_wall_scr = ctx->d_wall_ovf = nullptr;
+    ctx->wall_scr_cap = ctx->wall_ovf_cap = 0;
     ctx->d_as_cnt = ctx->d_as_off = nullptr;
     ctx->d_as_steps = nullptr;
     ctx->d_cubes = ctx->d_vbits = ctx->d_dse = nullptr;
@@ -1057,7 +1132,7 @@ int grow_fc(dmtz_ctx* ctx, ull need) {
     free_fc(ctx);
     ull cap = need + need / 4 + 4096;
     CK(hipMalloc(&ctx->d_fc, sizeof(uint64_t) * 4 * cap));
-    CK(hipMalloc(&ctx->d_path, sizeof(int64_t) * 4 * cap));
+    CK(hipMalloc(&ctx->d_path, sizeof(int64_t) * 6 * cap));
     CK(hipMalloc(&ctx->d_labels, sizeof(int64_t) * 4 * cap));
     CK(hipMalloc(&ctx->d_walk, sizeof(int64_t) * 4 * cap));
     ctx->fc_cap = cap;
@@ -1121,7 +1196,7 @@ int overflow_check(dmtz_ctx* ctx) {
             ctx->err = "mismatch list capacity exceeded";
             return DMTZ_E_CAPACITY;
         }
-    if (ctx->h_ctr[K_DS] > ctx->fc_cap || ctx->h_ctr[K_AS] > ctx->fc_cap) {
+    if (ctx->h_ctr[K_DS] > ctx->fc_cap || ctx->h_ctr[K_AS] > ctx->fc_cap || ctx->h_ctr[K_DSW] > ctx->fc_cap) {
         ctx->err = "separatrix list capacity exceeded";
         return DMTZ_E_CAPACITY;
     }
@@ -1213,70 +1288,64 @@ struct Driver {
         int rc = begin_edits();
         if (rc) return rc;
         Lists4 L = rec_lists(ctx);
+        ull nrec = 0;
         for (int ty = 0; ty < 4; ty++) {
             if (counts[ty] == 0) continue;
             if ((ty == 1 || ty == 2) && fl.preserve_saddles != 1) continue;
+            nrec += counts[ty];
             auto kern = ty == 0 ? k_fix_cp<T, 0> : ty == 1 ? k_fix_cp<T, 1> : ty == 2 ? k_fix_cp<T, 2> : k_fix_cp<T, 3>;
             kern<<<nblocks(counts[ty]), BLOCK, 0, ctx->stream>>>(L.l[ty], counts[ty], f, ctx->d_code_f, g,
                                                                  ctx->d_code_g, d, c, ctx->d_counter, eps, range,
                                                                  fl.edit_type, sink());
             CK(hipGetLastError());
         }
-        return apply();
+        return apply(nrec);
     }
 
     // applyDeltaBuffer1 (main_3d.cu:1122-1164) over the edit set, then re-evaluation of the g star
     // codes of the closed stars of the vertices whose value changed (the g gradient of
-    // main_3d.cu:6864-6874 restricted to where it can change).
-    ull moved = 0;  // edits of the last apply that changed a value or a counter
-
-    int apply() {
-        moved = 0;
-        int rc = sync_ctr(ctx);
-        if (rc) return rc;
-        rc = overflow_check(ctx);
-        if (rc) return rc;
-        ull nt = ctx->h_ctr[K_TGT];
-        if (!nt) return 0;
+    // main_3d.cu:6864-6874 restricted to where it can change). Nothing here waits for the
+    // device: the edit set's size stays on the device, nrec (the records repaired) sizes the
+    // launches, and the count of edits that changed a value or a counter (K_MOVED) is read with
+    // the next detection's counters. Each vertex enters the edit set at most once per epoch, so
+    // the set never exceeds the N entries of its buffer.
+    int apply(ull nrec) {
         CK(hipMemsetAsync(ctx->d_ctr + K_CHG, 0, sizeof(ull) * 2, ctx->stream));
         CK(hipMemsetAsync(ctx->d_ctr + K_MOVED, 0, sizeof(ull), ctx->stream));
-        k_apply<T><<<nblocks(nt), BLOCK, 0, ctx->stream>>>(ctx->d_targets, nt, f, g, ctx->d_counter, eps, range,
-                                                           fl.preserve_type, fl.edit_type, ctx->d_changed,
-                                                           ctx->d_ctr + K_CHG, ctx->d_ctr + K_MOVED);
+        if (!nrec) return 0;  // no record, no edit
+        const ull est = nrec < (ull)c.N ? nrec : (ull)c.N;
+        k_apply<T><<<stride_blocks(est), BLOCK, 0, ctx->stream>>>(
+            ctx->d_targets, ctx->d_ctr + K_TGT, (ull)ctx->cap_n, f, g, ctx->d_counter, eps, range, fl.preserve_type,
+            fl.edit_type, ctx->d_changed, ctx->d_ctr + K_CHG, ctx->d_ctr + K_MOVED);
         CK(hipGetLastError());
-        rc = sync_ctr(ctx);
-        if (rc) return rc;
-        moved = ctx->h_ctr[K_MOVED];
-        return refresh(ctx->h_ctr[K_CHG]);
+        return refresh(est);
     }
 
-    // Re-evaluate the g star codes of the closed stars of the nc vertices in the changed list
-    // (the g gradient of main_3d.cu:6864-6874 restricted to where it can change).
-    int refresh(ull nc) {
-        if (!nc) return 0;
-        if (16 * nc >= (ull)c.N) {  // most stars change: re-evaluate all of them
+    // Re-evaluate the g star codes of the closed stars of the vertices in the changed list (the g
+    // gradient of main_3d.cu:6864-6874 restricted to where it can change); the list's length
+    // stays on the device and est, the host's estimate of it, picks between re-evaluating every
+    // star and the closed stars of the list (both give the same codes).
+    int refresh(ull est) {
+        if (!est) return 0;
+        if (16 * est >= (ull)c.N) {  // most stars change: re-evaluate all of them
             int rc = g_codes();
             if (rc) return rc;
             as_full = true;
             ctx->dirty_total += (double)c.N;
             return 0;
         }
-        k_dirty<<<nblocks(15 * nc), BLOCK, 0, ctx->stream>>>(ctx->d_changed, nc, d, ctx->d_dstamp, ctx->epoch,
-                                                            ctx->d_dirty, ctx->d_ctr + K_DIRTY);
+        k_dirty<<<stride_blocks(15 * est), BLOCK, 0, ctx->stream>>>(ctx->d_changed, ctx->d_ctr + K_CHG, d,
+                                                                   ctx->d_dstamp, ctx->epoch, ctx->d_dirty,
+                                                                   ctx->d_ctr + K_DIRTY);
         CK(hipGetLastError());
-        int rc = sync_ctr(ctx);
-        if (rc) return rc;
-        ull nd = ctx->h_ctr[K_DIRTY];
-        if (nd) {
-            k_code_list_g<T><<<nblocks(nd), BLOCK, 0, ctx->stream>>>(g, d, ctx->d_code_g, ctx->d_critf, ctx->d_mm,
-                                                                      ctx->d_dirty, nd);
+        k_code_list_g<T><<<stride_blocks(15 * est), BLOCK, 0, ctx->stream>>>(
+            g, d, ctx->d_code_g, ctx->d_critf, ctx->d_mm, ctx->d_dirty, ctx->d_ctr + K_DIRTY, ctx->d_ctr + K_STARS);
+        CK(hipGetLastError());
+        if (as_ready) {
+            k_cube_mark<<<stride_blocks(15 * est), BLOCK, 0, ctx->stream>>>(ctx->d_dirty, ctx->d_ctr + K_DIRTY, d,
+                                                                          ctx->d_cubes, ctx->d_vbits);
             CK(hipGetLastError());
-            if (as_ready) {
-                k_cube_mark<<<nblocks(nd), BLOCK, 0, ctx->stream>>>(ctx->d_dirty, nd, d, ctx->d_cubes, ctx->d_vbits);
-                CK(hipGetLastError());
-            }
         }
-        ctx->dirty_total += (double)nd;
         return 0;
     }
 
@@ -1384,7 +1453,7 @@ struct Driver {
         ull cap = ctx->fc_cap;
         p.id = ctx->d_path + (size_t)(2 * which) * cap;
         p.rec = ctx->d_path + (size_t)(2 * which + 1) * cap;
-        p.ctr = ctx->d_ctr + (which == 0 ? K_DS : K_AS);
+        p.ctr = ctx->d_ctr + (which == 0 ? K_DS : (which == 1 ? K_AS : K_DSW));
         p.cap = cap;
         return p;
     }
@@ -1474,9 +1543,58 @@ struct Driver {
             int rc = paths_checked();
             if (rc) return rc;
         }
+        if (fl.preserve_connectors == 1) {
+            int rc = walls();
+            if (rc) return rc;
+        }
         int rc = sync_ctr(ctx);
         if (rc) return rc;
         return overflow_check(ctx);
+    }
+
+    // 2-saddle walls (preserve_connectors, main_3d.cu:5064-5133) from every f-critical face
+    // into the wall list. The walks run in tiers of visited-set size: every start with 256
+    // slots, the walks that outgrow them with 8192, then with 2^18; a wall of more than 2^17
+    // faces is a capacity error. Launches are chunked to keep the per-thread sets near 1 GiB.
+    int walls() {
+        CK(hipMemsetAsync(ctx->d_ctr + K_DSW, 0, sizeof(ull), ctx->stream));
+        if (!nfcf) return 0;
+        if (c.F >= (i64)0xFFFFFFFFll) {
+            ctx->err = "2-saddle walls need fewer than 2^32 - 1 faces";
+            return DMTZ_E_UNSUPPORTED;
+        }
+        static const uint32_t tiers[3] = {256u, 8192u, 1u << 18};
+        const size_t budget = (size_t)1 << 28;  // uint32 entries of per-thread sets in flight
+        int rc = grow_buf(ctx, ctx->d_wall_ovf, ctx->wall_ovf_cap, 2 * (size_t)nfcf);
+        if (rc) return rc;
+        Lists4 FL = fcrit_lists(ctx);
+        const uint32_t* sel = nullptr;
+        ull n = nfcf;
+        for (int tier = 0; tier < 3 && n > 0; tier++) {
+            const uint32_t slots = tiers[tier];
+            const size_t per = 2 * (size_t)slots + 2;
+            const ull chunk = (ull)(budget / per) < n ? (ull)(budget / per) : n;
+            rc = grow_buf(ctx, ctx->d_wall_scr, ctx->wall_scr_cap, chunk * per);
+            if (rc) return rc;
+            uint32_t* ovf = ctx->d_wall_ovf + (tier % 2 == 0 ? 0 : nfcf);
+            CK(hipMemsetAsync(ctx->d_ctr + K_WOVF, 0, sizeof(ull), ctx->stream));
+            for (ull first = 0; first < n; first += chunk) {
+                const ull last = first + chunk < n ? first + chunk : n;
+                k_ds_wall<T><<<nblocks(last - first), BLOCK, 0, ctx->stream>>>(
+                    FL.l[2], sel, first, last, f, ctx->d_code_f, g, ctx->d_code_g, d, c, ctx->d_wall_scr, slots,
+                    path_list(2), ovf, ctx->d_ctr + K_WOVF);
+                CK(hipGetLastError());
+            }
+            rc = sync_ctr(ctx);
+            if (rc) return rc;
+            n = ctx->h_ctr[K_WOVF];
+            sel = ovf;
+        }
+        if (n > 0) {
+            ctx->err = "a 2-saddle wall exceeds 2^17 faces";
+            return DMTZ_E_CAPACITY;
+        }
+        return 0;
     }
 
     // Ascending geometry checks over precomputed f paths: in geometry mode, when tet and face
@@ -1580,16 +1698,17 @@ struct Driver {
         return 0;
     }
 
-    int fix_paths(ull nds, ull nas) {
+    int fix_paths(ull nds, ull nas, ull nw) {
         int rc = begin_edits();
         if (rc) return rc;
-        if (nds + nas) {
-            k_fix_paths<T><<<nblocks(nds + nas), BLOCK, 0, ctx->stream>>>(
-                path_list(1).rec, nas, path_list(0).rec, nds, f, ctx->d_code_f, g, ctx->d_code_g, d, c, ctx->d_counter,
-                eps, range, fl.edit_type, sink());
+        if (fl.preserve_connectors != 1) nw = 0;
+        if (nds + nas + nw) {
+            k_fix_paths<T><<<nblocks(nds + nas + nw), BLOCK, 0, ctx->stream>>>(
+                path_list(1).rec, nas, path_list(0).rec, nds, path_list(2).rec, nw, f, ctx->d_code_f, g,
+                ctx->d_code_g, d, c, ctx->d_counter, eps, range, fl.edit_type, sink());
             CK(hipGetLastError());
         }
-        return apply();
+        return apply(nds + nas + nw);
     }
 
     int filtration() {
@@ -1651,7 +1770,7 @@ struct Driver {
                 std::fprintf(stderr, "dmtz sweep %lld: %lld %lld %lld %lld | %lld %lld  (%.3f s, %lld edits, %.0f stars)\n",
                              (long long)sweeps, (long long)cnt[0], (long long)cnt[1], (long long)cnt[2],
                              (long long)cnt[3], (long long)cnt[4], (long long)cnt[5], secs(t_start, Clock::now()),
-                             (long long)ctx->h_ctr[K_TGT], ctx->dirty_total);
+                             (long long)ctx->h_ctr[K_TGT], ctx->dirty_total + (double)ctx->h_ctr[K_STARS]);
         };
         // Re-evaluation after the apply of a critical-point sweep (main_3d.cu:6864-6921).
         auto eval_cp = [&]() -> int {
@@ -1682,6 +1801,7 @@ struct Driver {
                 if (r) return r;
                 cnt[4] = (i64)ctx->h_ctr[K_DS];
                 cnt[5] = (i64)ctx->h_ctr[K_AS];
+                cnt[6] = fl.preserve_connectors == 1 ? (i64)ctx->h_ctr[K_DSW] : 0;
                 ctx->phase[2] += secs(td, Clock::now());
             }
             return 0;
@@ -1693,7 +1813,7 @@ struct Driver {
         auto after_sweep = [&](int kind) -> int {
             std::vector<i64> key(cnt, cnt + 8);
             key.push_back(kind);
-            if (moved > 0) {
+            if (ctx->h_ctr[K_MOVED] > 0) {  // edits of the sweep that changed a value or a counter
                 seen.assign(1, key);
                 zone_from = 0;
                 return 0;
@@ -1756,8 +1876,8 @@ struct Driver {
                 if (rc) return rc;
                 auto tb = Clock::now();
                 ctx->phase[2] += secs(ta, tb);
-                ull nds = ctx->h_ctr[K_DS], nas = ctx->h_ctr[K_AS];
-                rc = fix_paths(nds, nas);
+                ull nds = ctx->h_ctr[K_DS], nas = ctx->h_ctr[K_AS], nw = ctx->h_ctr[K_DSW];
+                rc = fix_paths(nds, nas, nw);
                 if (rc) return rc;
                 ctx->phase[4] += secs(tb, Clock::now());
                 rc = eval_path();
@@ -1788,6 +1908,7 @@ struct Driver {
         if (sweeps_out) *sweeps_out = (int32_t)sweeps;
         CK(hipMemcpyAsync(ctx->h_ctr, ctx->d_ctr, sizeof(ull) * K_NCTR, hipMemcpyDeviceToHost, ctx->stream));
         CK(hipStreamSynchronize(ctx->stream));
+        ctx->dirty_total += (double)ctx->h_ctr[K_STARS];
         if (ctx->h_ctr[K_WALK] > 0) {
             ctx->err = "a separatrix walk exceeded N steps (cyclic gradient path)";
             return DMTZ_E_WALK;
@@ -1804,10 +1925,6 @@ int correct_dev(dmtz_ctx* ctx, const T* d_f, T* d_g, i64 nx, i64 ny, i64 nz, dou
     if (!flags || !d_f || !d_g) {
         ctx->err = "null argument";
         return DMTZ_E_ARG;
-    }
-    if (flags->preserve_connectors == 1) {
-        ctx->err = "preserve_connectors (2-saddle walls) is not supported on the device path";
-        return DMTZ_E_UNSUPPORTED;
     }
     if (flags->mode != 0) {
         ctx->err = "unknown mode";
@@ -2023,6 +2140,7 @@ int paths_host(dmtz_ctx* ctx, const T* f, const T* g, i64 nx, i64 ny, i64 nz, co
     CK(hipMemsetAsync(ctx->d_ctr, 0, sizeof(ull) * K_NCTR, ctx->stream));
     dmtz_flags fl = *flags;
     fl.preserve_vpath = 1;
+    fl.preserve_connectors = 0;
     Driver<T> drv{ctx, (const T*)ctx->d_f, (T*)ctx->d_g, d, c, 0.0, 0.0, fl};
     rc = drv.f_codes();
     if (rc) return rc;
@@ -2047,8 +2165,8 @@ int paths_host(dmtz_ctx* ctx, const T* f, const T* g, i64 nx, i64 ny, i64 nz, co
         if (rc) return rc;
         if (nds + nas) {
             k_fix_paths<T><<<nblocks(nds + nas), BLOCK, 0, ctx->stream>>>(
-                pa.rec, nas, pd.rec, nds, (const T*)ctx->d_f, ctx->d_code_f, (const T*)ctx->d_g, ctx->d_code_g, d, c,
-                ctx->d_counter, 0.0, 0.0, fl.edit_type, drv.sink());
+                pa.rec, nas, pd.rec, nds, nullptr, 0, (const T*)ctx->d_f, ctx->d_code_f, (const T*)ctx->d_g,
+                ctx->d_code_g, d, c, ctx->d_counter, 0.0, 0.0, fl.edit_type, drv.sink());
             CK(hipGetLastError());
         }
         rc = sync_ctr(ctx);
@@ -2065,6 +2183,70 @@ int paths_host(dmtz_ctx* ctx, const T* f, const T* g, i64 nx, i64 ny, i64 nz, co
     if (ctx->h_ctr[K_WALK] > 0) {
         ctx->err = "a separatrix walk exceeded N steps (cyclic gradient path)";
         return DMTZ_E_WALK;
+    }
+    return 0;
+}
+
+// The 2-saddle wall list on (f, g) as they stand, and optionally the edit set of fix_ds_wall over
+// it (eps and range enter its threshold, main_3d.cu:5331).
+template <class T>
+int walls_host(dmtz_ctx* ctx, const T* f, const T* g, i64 nx, i64 ny, i64 nz, double eps, double range,
+               const dmtz_flags* flags, int64_t* count, int64_t* face, int64_t* edge, int64_t cap, uint8_t* targets) {
+    int rc = check_dims(ctx, nx, ny, nz);
+    if (rc) return rc;
+    if (!count || cap < 0) {
+        ctx->err = "null argument";
+        return DMTZ_E_ARG;
+    }
+    Dims d{nx, ny, nz};
+    Counts c = make_counts(d);
+    rc = ensure(ctx, c.N, sizeof(T), true);
+    if (rc) return rc;
+    CK(hipMemcpyAsync(ctx->d_f, f, sizeof(T) * (size_t)c.N, hipMemcpyHostToDevice, ctx->stream));
+    CK(hipMemcpyAsync(ctx->d_g, g, sizeof(T) * (size_t)c.N, hipMemcpyHostToDevice, ctx->stream));
+    CK(hipMemsetAsync(ctx->d_counter, 1, (size_t)c.N, ctx->stream));
+    CK(hipMemsetAsync(ctx->d_ctr, 0, sizeof(ull) * K_NCTR, ctx->stream));
+    dmtz_flags fl{};
+    if (flags) fl = *flags;
+    fl.preserve_vpath = 1;
+    fl.preserve_connectors = 1;
+    Driver<T> drv{ctx, (const T*)ctx->d_f, (T*)ctx->d_g, d, c, eps, range, fl};
+    rc = drv.f_codes();
+    if (rc) return rc;
+    rc = drv.g_codes();
+    if (rc) return rc;
+    rc = drv.prepare_paths();
+    if (rc) return rc;
+    rc = drv.walls();
+    if (rc) return rc;
+    rc = sync_ctr(ctx);
+    if (rc) return rc;
+    rc = overflow_check(ctx);
+    if (rc) return rc;
+    const ull nw = ctx->h_ctr[K_DSW];
+    *count = (int64_t)nw;
+    PathList pw = drv.path_list(2);
+    const ull k = nw < (ull)cap ? nw : (ull)cap;
+    if (k && face) CK(hipMemcpyAsync(face, pw.id, sizeof(int64_t) * k, hipMemcpyDeviceToHost, ctx->stream));
+    if (k && edge) CK(hipMemcpyAsync(edge, pw.rec, sizeof(int64_t) * k, hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    if (targets) {
+        rc = drv.begin_edits();
+        if (rc) return rc;
+        if (nw) {
+            k_fix_paths<T><<<nblocks(nw), BLOCK, 0, ctx->stream>>>(
+                nullptr, 0, nullptr, 0, pw.rec, nw, (const T*)ctx->d_f, ctx->d_code_f, (const T*)ctx->d_g,
+                ctx->d_code_g, d, c, ctx->d_counter, eps, range, fl.edit_type, drv.sink());
+            CK(hipGetLastError());
+        }
+        rc = sync_ctr(ctx);
+        if (rc) return rc;
+        const ull nt = ctx->h_ctr[K_TGT];
+        std::vector<uint32_t> tl(nt);
+        if (nt) CK(hipMemcpyAsync(tl.data(), ctx->d_targets, sizeof(uint32_t) * nt, hipMemcpyDeviceToHost, ctx->stream));
+        CK(hipStreamSynchronize(ctx->stream));
+        std::memset(targets, 0, (size_t)c.N);
+        for (ull q = 0; q < nt; q++) targets[tl[q]] = 1;
     }
     return 0;
 }
@@ -2260,6 +2442,21 @@ int dmtz_paths_f64(dmtz_ctx* ctx, const double* f, const double* g, int64_t nx, 
     if (!ctx) return DMTZ_E_ARG;
     (void)hipSetDevice(ctx->device);
     return paths_host<double>(ctx, f, g, nx, ny, nz, fl, counts2, ds_id, ds_rec, as_id, as_rec, cap, targets);
+}
+
+int dmtz_walls_f32(dmtz_ctx* ctx, const float* f, const float* g, int64_t nx, int64_t ny, int64_t nz, double eps,
+                   double range, const dmtz_flags* flags, int64_t* count, int64_t* face, int64_t* edge, int64_t cap,
+                   uint8_t* targets) {
+    if (!ctx) return DMTZ_E_ARG;
+    (void)hipSetDevice(ctx->device);
+    return walls_host<float>(ctx, f, g, nx, ny, nz, eps, range, flags, count, face, edge, cap, targets);
+}
+int dmtz_walls_f64(dmtz_ctx* ctx, const double* f, const double* g, int64_t nx, int64_t ny, int64_t nz, double eps,
+                   double range, const dmtz_flags* flags, int64_t* count, int64_t* face, int64_t* edge, int64_t cap,
+                   uint8_t* targets) {
+    if (!ctx) return DMTZ_E_ARG;
+    (void)hipSetDevice(ctx->device);
+    return walls_host<double>(ctx, f, g, nx, ny, nz, eps, range, flags, count, face, edge, cap, targets);
 }
 
 }  // extern "C"

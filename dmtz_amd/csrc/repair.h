@@ -432,6 +432,123 @@ struct Repair {
         tgt(v_idx == a ? b : a);
     }
 
+    // fix_ds_wall main_3d.cu:5297-5413; e = the first edge of the wall whose f and g partners
+    // differ. The partner of an edge is a face id or, for an edge paired with its vertex, that
+    // vertex id; the range checks below compare both kinds against the face count, as the
+    // reference does. The first branch's third-vertex pick is two independent ifs (:5331-5334),
+    // the last branch's an else-if.
+    DMTZ_HD void fix_ds_wall(i64 e) {
+        const i64 NF = F.c.F;
+        const i64 tf = F.pe(e), ff = G.pe(e);
+        if (tf == ff) return;
+        i64 v1, v2;
+        edge_verts(F.d, F.c, e, v1, v2);
+        if (tf >= 0 && ff >= 0 && tf < NF && F.pf(tf) == e && ff < NF && G.pf(ff) == e) {
+            i64 a, b, v3;
+            tri_verts(F.d, F.c, tf, a, b, v3);
+            if (a != v1 && a != v2) v3 = a;
+            if (b != v1 && b != v2) v3 = b;
+            if (fabs_d(delta(v3)) > 1e-16 * range) {
+                tgt(v3);
+            } else {
+                tgt(v1);
+                tgt(v2);
+            }
+        } else if (tf == -1 || tf >= NF || (tf >= 0 && F.pf(tf) != e)) {
+            tgt(v1);
+            tgt(v2);
+        } else if (ff == -1 || ff >= NF || (ff >= 0 && G.pf(ff) != e)) {
+            i64 a, b, v3;
+            tri_verts(F.d, F.c, tf, a, b, v3);
+            if (a != v1 && a != v2) v3 = a;
+            else if (b != v1 && b != v2) v3 = b;
+            tgt(v3);
+        }
+    }
+
+    // Face partners of the three edges of face t in f's pairing, for the wall walk
+    // (get_false_ds_wall main_3d.cu:5064-5133): edges (v1,v2), (v1,v3), (v2,v3) of the
+    // descending vertex triple. mismatch = the first edge whose f partner exists and differs
+    // from its g partner (-1 if none before the stop); next[k] = the face paired with edge k in
+    // f when that face is not t and pairs back to edge k, else -1. Edges after the mismatch are
+    // not examined (next = -1).
+    DMTZ_HD void wall_step(i64 t, i64& mismatch, i64 next[3]) const {
+        i64 v1, v2, v3;
+        tri_verts(F.d, F.c, t, v1, v2, v3);
+        auto edge = [&](i64 a, i64 b) {
+            const int sl = slot_between(F.d, a, b);
+            return sl < 0 ? (i64)-1 : F.edge_of(a, sl);
+        };
+        const i64 es[3] = {edge(v1, v2), edge(v1, v3), edge(v2, v3)};
+        mismatch = -1;
+        for (int k = 0; k < 3; k++) {
+            next[k] = -1;
+            if (mismatch >= 0) continue;
+            const i64 j = es[k];
+            const i64 pj = F.pe(j);
+            if (pj == -1) continue;
+            if (pj != G.pe(j)) {
+                mismatch = j;
+                continue;
+            }
+            if (pj != t && pj >= 0 && pj < F.c.F && F.pf(pj) == j) next[k] = pj;
+        }
+    }
+
+    // The wall walk from face `face` (get_false_ds_wall main_3d.cu:5064-5133 with a stack and
+    // visited set of its own): depth first, a face is marked when popped, the faces of
+    // wall_step are pushed in edge order so the last is popped first; found = the first edge
+    // met whose f and g partners differ, -1 if the wall has none. The visited set is open
+    // addressing over `slots` entries (a power of two; face id + 1, 0 = empty) in set[], the
+    // stack is stack[0 .. slots + 2). Returns true (overflow, found undefined) when the walk
+    // would mark more than slots / 2 faces; each marked face pushes at most 3, so the stack
+    // then holds at most slots + 1. f's gradient has no closed V-path, so every walk ends.
+    // Face ids must be below 2^32 - 1.
+    DMTZ_HD bool wall_walk(i64 face, uint32_t* set, uint32_t* stack, uint32_t slots, i64& found) const {
+        for (uint32_t k = 0; k < slots; k++) set[k] = 0;
+        const uint32_t mask = slots - 1, limit = slots / 2, shift = 32 - (uint32_t)__builtin_ctz(slots);
+        uint32_t sp = 0, used = 0;
+        stack[sp++] = (uint32_t)face;
+        found = -1;
+        bool overflow = false;
+        while (sp > 0 && found < 0 && !overflow) {
+            const uint32_t tri = stack[--sp];
+            uint32_t h = (uint32_t)(tri * 2654435761u) >> shift;  // multiplicative hash, top bits
+            bool seen = false;
+            for (uint32_t probe = 0; probe < slots; probe++) {  // the set is at most half full
+                const uint32_t key = set[h];
+                if (key == tri + 1u) {
+                    seen = true;
+                    break;
+                }
+                if (key == 0) break;
+                h = (h + 1) & mask;
+            }
+            if (seen) continue;
+            if (used == limit) {
+                overflow = true;
+                continue;
+            }
+            set[h] = tri + 1u;
+            used++;
+            i64 mismatch, next[3];
+            wall_step((i64)tri, mismatch, next);
+            if (mismatch >= 0) {
+                found = mismatch;
+                continue;
+            }
+            for (int k = 0; k < 3; k++) {
+                if (next[k] < 0) continue;
+                if (sp >= slots + 2) {
+                    overflow = true;
+                    break;
+                }
+                stack[sp++] = (uint32_t)next[k];
+            }
+        }
+        return overflow;
+    }
+
     // ---------------------------------------------------------------------------------------
     // Separatrix checks
     // ---------------------------------------------------------------------------------------

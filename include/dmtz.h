@@ -45,13 +45,16 @@ extern "C" {
  * would cover the grid, or the edit sets are empty, restore g = f on every vertex instead
  * (counter 7, never edited again). Edits only lower g or raise counters and the radius grows
  * with every stall, so the loop then ends with zero mismatches and |g - f| <= eps. Runs that
- * never stall are unchanged. */
+ * never stall are unchanged.
+ * preserve_connectors = 1 (with preserve_vpath = 1): the walls check of main_3d.cu:5064-5133
+ * with a stack and visited set per start face instead of the reference's shared stack (its
+ * defects make the reference's result depend on thread timing); see dmtz_walls_f32. */
 typedef struct dmtz_flags {
     int32_t preserve_saddles;    /* argv[4]: fix 1- and 2-saddle mismatches            */
     int32_t preserve_vpath;      /* argv[5]: also match separatrices (V-paths)          */
     int32_t preserve_geometry;   /* argv[6]: 1 = path geometry, 0 = endpoint labels      */
     int32_t filtration;          /* argv[7]: pin f-critical cells to f - eps first       */
-    int32_t preserve_connectors; /* argv[8]: 2-saddle walls (not supported on device)   */
+    int32_t preserve_connectors; /* argv[8]: also match 2-saddle walls (deterministic)   */
     int32_t preserve_type;       /* argv[9]: nonzero = every edit snaps to f - eps       */
     int32_t edit_type;           /* argv[10]: 0 = halve the gap, 1 = -2^-k eps steps     */
     int32_t max_sweeps;          /* cap on correction sweeps (the reference has none)    */
@@ -190,6 +193,22 @@ int dmtz_paths_f32(dmtz_ctx* ctx, const float* f, const float* g, int64_t nx, in
 int dmtz_paths_f64(dmtz_ctx* ctx, const double* f, const double* g, int64_t nx, int64_t ny, int64_t nz,
                    const dmtz_flags* flags, int64_t* counts2, int64_t* ds_id, int64_t* ds_rec, int64_t* as_id,
                    int64_t* as_rec, int64_t cap, uint8_t* targets);
+
+/* 2-saddle walls on (f, g) as they stand: get_false_ds_wall (main_3d.cu:5064-5133) made
+ * deterministic. From every f-critical face, a depth-first walk over f's pairs (face -> its edges
+ * (v1,v2), (v1,v3), (v2,v3) of the descending vertex triple -> the face paired with each edge in
+ * f when it pairs back), with its own stack and visited set, records the first edge whose f and
+ * g partners differ. count receives the number of such faces; face/edge receive (face id, edge
+ * id) pairs, as many as cap allows (in no particular order). When targets is non-null it
+ * receives the edit set of fix_ds_wall (main_3d.cu:5297-5413) over the list; eps and range enter
+ * its threshold, and flags (nullable) its edit_type. DMTZ_E_CAPACITY if one wall walk visits more
+ * than 2^17 faces. */
+int dmtz_walls_f32(dmtz_ctx* ctx, const float* f, const float* g, int64_t nx, int64_t ny, int64_t nz, double eps,
+                   double range, const dmtz_flags* flags, int64_t* count, int64_t* face, int64_t* edge, int64_t cap,
+                   uint8_t* targets);
+int dmtz_walls_f64(dmtz_ctx* ctx, const double* f, const double* g, int64_t nx, int64_t ny, int64_t nz, double eps,
+                   double range, const dmtz_flags* flags, int64_t* count, int64_t* face, int64_t* edge, int64_t cap,
+                   uint8_t* targets);
 
 #ifdef __cplusplus
 }

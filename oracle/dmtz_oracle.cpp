@@ -2116,3 +2116,36 @@ extern "C" int orc_path_targets_f32(const float* f, const float* g, i64 W, i64 H
                                     const orc_flags* fl, uint8_t* targets, i64* counts2) {
     return path_targets_impl<float>(f, g, W, H, D, eps, range, fl, targets, counts2);
 }
+
+// Wall list on (f, g) as they stand and the edit set of fix_ds_wall over it (the wall step of the
+// path sweep, main_3d.cu:7010-7016 with preserve_connectors set).
+template <class T>
+static int wall_targets_impl(const T* f, const T* gfield, i64 W, i64 H, i64 D, double eps, double range,
+                             const orc_flags* fl, uint8_t* targets, i64* count) {
+    init_dirs();
+    State<T> s(W, H, D);
+    s.f = f;
+    std::vector<T> gcopy(gfield, gfield + s.g.N);
+    s.d = gcopy.data();
+    s.eps = eps;
+    s.range = range;
+    std::memcpy(&s.fl, fl, sizeof(orc_flags));
+    s.fl.preserve_connectors = 1;
+    s.literal = false;
+    s.alloc();
+    s.gradient(F_FIELD);
+    s.gradient(G_FIELD);
+    s.detect_ds_wall();
+    *count = (i64)s.l_dsw.size();
+    s.for_list(s.l_dsw, [&](i64 id) { s.fix_ds_wall(id); });
+    std::memcpy(targets, s.target.data(), s.g.N);
+    return 0;
+}
+extern "C" int orc_wall_targets_f32(const float* f, const float* g, i64 W, i64 H, i64 D, double eps, double range,
+                                    const orc_flags* fl, uint8_t* targets, i64* count) {
+    return wall_targets_impl<float>(f, g, W, H, D, eps, range, fl, targets, count);
+}
+extern "C" int orc_wall_targets_f64(const double* f, const double* g, i64 W, i64 H, i64 D, double eps, double range,
+                                    const orc_flags* fl, uint8_t* targets, i64* count) {
+    return wall_targets_impl<double>(f, g, W, H, D, eps, range, fl, targets, count);
+}

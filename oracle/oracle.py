@@ -68,6 +68,8 @@ def lib():
             getattr(L, "orc_targets_" + t).argtypes = [P, P, I64, I64, I64, ctypes.c_double,
                                                       ctypes.c_double, P, P, P]
             getattr(L, "orc_paths_" + t).argtypes = [P, P, I64, I64, I64, P, P, P, P, P, P, P, P]
+            getattr(L, "orc_wall_targets_" + t).argtypes = [P, P, I64, I64, I64, ctypes.c_double,
+                                                           ctypes.c_double, P, P, P]
             getattr(L, "orc_correct_" + t).argtypes = [P, P, I64, I64, I64, ctypes.c_double,
                                                       ctypes.c_double, P, P, P, P, P, I64, P, P]
         L.orc_set_threads.argtypes = [ctypes.c_int]
@@ -187,6 +189,19 @@ def paths(f, g, fl):
     n_ds, n_as, n_w = (int(x) for x in c3)
     return dict(rc=rc, ds=ds_l[:n_ds], ds_rec=ds_r[:n_ds], as_=as_l[:n_as], as_rec=as_r[:n_as],
                 wall=w_l[:n_w], wall_rec=w_r[:n_w])
+
+
+def wall_targets(f, g, eps, rng, fl=None):
+    """Wall list size and the edit set of fix_ds_wall over it, on (f, g) as they stand."""
+    f = np.ascontiguousarray(f)
+    g = np.ascontiguousarray(g, dtype=f.dtype)
+    D, H, W = f.shape
+    fl = fl or flags()
+    tg = np.zeros(W * H * D, np.uint8)
+    n = np.zeros(1, np.int64)
+    getattr(lib(), "orc_wall_targets_" + _tag(f))(_p(f), _p(g), W, H, D, float(eps), float(rng),
+                                                  ctypes.byref(fl), _p(tg), _p(n))
+    return tg.astype(bool).reshape(f.shape), int(n[0])
 
 
 def correct(f, g, eps, rng, fl=None, traj_cap=4096):

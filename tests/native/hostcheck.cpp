@@ -322,3 +322,42 @@ static i64 critf_two_ways_impl(const T* f, i64 W, i64 H, i64 D) {
 }
 extern "C" i64 hc_critf_two_ways_f32(const float* f, i64 W, i64 H, i64 D) { return critf_two_ways_impl<float>(f, W, H, D); }
 extern "C" i64 hc_critf_two_ways_f64(const double* f, i64 W, i64 H, i64 D) { return critf_two_ways_impl<double>(f, W, H, D); }
+
+// Walls from every f-critical face with Repair::wall_walk (the code the wall kernel runs), with
+// the kernel's tiers of visited-set size; then the fix_ds_wall edit set over the list. Returns
+// the number of walls (face/edge receive them in face order), -1 if a walk outgrows every tier.
+template <class T>
+static i64 walls_impl(const T* f, const T* g, i64 W, i64 H, i64 D, double eps, double range, int edit_type,
+                      i64* face, i64* edge, uint8_t* targets) {
+    Dims d{W, H, D};
+    Counts c = make_counts(d);
+    std::vector<StarCode> cf = all_codes(f, d, c), cg = all_codes(g, d, c);
+    std::vector<uint8_t> tg(c.N, 0);
+    std::vector<uint8_t> counter(c.N, 1);
+    Repair<T, VecSink> R(f, cf.data(), g, cg.data(), d, c, counter.data(), eps, range, edit_type, VecSink{&tg});
+    const FieldGrad<T> F(f, cf.data(), d, c);
+    i64 n = 0;
+    for (i64 t = 0; t < c.F; t++) {
+        if (F.pf(t) != -1) continue;
+        i64 found = -1;
+        bool overflow = true;
+        for (uint32_t slots : {256u, 8192u, 1u << 18}) {
+            std::vector<uint32_t> buf(2 * (size_t)slots + 2);
+            overflow = R.wall_walk(t, buf.data(), buf.data() + slots, slots, found);
+            if (!overflow) break;
+        }
+        if (overflow) return -1;
+        if (found >= 0) {
+            face[n] = t;
+            edge[n] = found;
+            n++;
+        }
+    }
+    for (i64 k = 0; k < n; k++) R.fix_ds_wall(edge[k]);
+    std::memcpy(targets, tg.data(), c.N);
+    return n;
+}
+extern "C" i64 hc_walls_f32(const float* f, const float* g, i64 W, i64 H, i64 D, double eps, double range, int et,
+                            i64* face, i64* edge, uint8_t* targets) {
+    return walls_impl<float>(f, g, W, H, D, eps, range, et, face, edge, targets);
+}

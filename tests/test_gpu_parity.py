@@ -97,6 +97,56 @@ def test_correction_bit_exact(oracle, dev, name, fl):
             assert np.abs(got.g.astype(np.float64) - f).max() <= eps
 
 
+# 2-saddle walls (preserve_connectors): expected status per input of test_correction_bit_exact;
+# without the finisher two of the inputs reach a fixed point with walls left
+WALL_CASES = [
+    ("walls_geometry", dict(preserve_vpath=1, preserve_geometry=1, preserve_connectors=1), (0, 2, 2)),
+    ("walls_connectivity", dict(preserve_vpath=1, preserve_geometry=0, preserve_connectors=1), (0, 2, 2)),
+    ("walls_finish", dict(preserve_vpath=1, preserve_geometry=1, preserve_connectors=1, finish=1), (0, 0, 0)),
+]
+
+
+@pytest.mark.parametrize("name,fl,expect", WALL_CASES, ids=[c[0] for c in WALL_CASES])
+def test_correction_with_walls_bit_exact(oracle, dev, name, fl, expect):
+    inputs = [(0, (16, 16, 16), 1e-2), (1, (24, 20, 12), 2e-2), (2, (13, 17, 11), 5e-2)]
+    for (seed, shape, rel), status in zip(inputs, expect):
+        f = fields.gaussian_mixture(shape, 6, seed=seed, noise=0.02)
+        eps, rng = fields.eps_from_rel(f, rel)
+        g = fields.quantize(f, eps)
+        print("correct", name, shape, seed, flush=True)
+        got = _compare_run(oracle, dev, f, g, eps, rng, fl)
+        assert got.status == status
+        assert (got.trajectory.reshape(-1, 8)[:, 6] > 0).sum() > 1  # walls were found and repaired
+        if status == 0:
+            assert got.counts == (0,) * 8
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_wall_lists_bit_exact(oracle, dev, dtype):
+    """Wall lists (face, first differing edge) and the fix_ds_wall edit set, on quantised g and
+    on g whose critical cells already match f's."""
+    seen = 0
+    for seed, shape, rel in [(2, (16, 16, 16), 5e-3), (1, (24, 20, 12), 2e-2), (3, (13, 17, 11), 5e-2)]:
+        f = fields.gaussian_mixture(shape, 6, seed=seed, noise=0.02).astype(dtype)
+        eps, rng = fields.eps_from_rel(f, rel)
+        g0 = fields.quantize(f, eps)
+        g1 = oracle.correct(f, g0, eps, rng, oracle.flags())["g"]
+        for g in (g0, g1):
+            for edit_type in (0, 1):
+                print("walls", shape, dtype.__name__, seed, edit_type, flush=True)
+                ofl = oracle.flags(preserve_vpath=1, preserve_geometry=1, preserve_connectors=1, edit_type=edit_type)
+                want = oracle.paths(f, g, ofl)
+                got = dmtz_amd.walls(f, g, eps, rng, dmtz_amd.CorrectionFlags(edit_type=edit_type), ctx=dev,
+                                     with_targets=True)
+                assert sorted(zip(got["face"].tolist(), got["edge"].tolist())) == \
+                    sorted(zip(want["wall"].tolist(), want["wall_rec"].tolist()))
+                want_t, n = oracle.wall_targets(f, g, eps, rng, ofl)
+                assert n == len(got["face"])
+                assert (got["targets"] == want_t).all()
+                seen += n
+    assert seen > 0
+
+
 def test_correction_bit_exact_f64(oracle, dev):
     f = fields.gaussian_mixture((20, 18, 16), 6, seed=4, noise=0.02).astype(np.float64)
     eps, rng = fields.eps_from_rel(f, 2e-2)
@@ -126,9 +176,8 @@ def test_bad_arguments_raise(dev):
     f = np.zeros((4, 4, 2), np.float32)  # nx = 2 is below the minimum grid
     with pytest.raises(dmtz_amd.DmtzError):
         dmtz_amd.correct(f, f, 0.1, 1.0, ctx=dev)
-    f = np.zeros((4, 4, 4), np.float32)
     with pytest.raises(dmtz_amd.DmtzError):
-        dmtz_amd.correct(f, f, 0.1, 1.0, dmtz_amd.CorrectionFlags(preserve_connectors=1), ctx=dev)
+        dmtz_amd.walls(f, f, ctx=dev)
 
 
 @pytest.mark.parametrize("geometry", [1, 0])

@@ -177,3 +177,34 @@ def test_critical_masks_scattered_equal_gathered(hc):
         D, H, W = f.shape
         fn = hc.hc_critf_two_ways_f32 if f.dtype == np.float32 else hc.hc_critf_two_ways_f64
         assert fn(_p(f), W, H, D) == 0, (f.shape, f.dtype)
+
+
+def test_walls_match_oracle(hc, oracle):
+    """Wall lists and the fix_ds_wall edit set from the host build of Repair::wall_walk (the
+    code the wall kernel runs) against the oracle, on quantised g and on g whose critical cells
+    already match f's."""
+    hc.hc_walls_f32.argtypes = [P, P, I, I, I, ctypes.c_double, ctypes.c_double, ctypes.c_int, P, P, P]
+    hc.hc_walls_f32.restype = I
+    seen = 0
+    for seed, shape, rel in [(2, (16, 16, 16), 5e-3), (3, (13, 17, 11), 5e-2)]:
+        f = fields.gaussian_mixture(shape, 6, seed=seed, noise=0.02)
+        eps, rng = fields.eps_from_rel(f, rel)
+        g0 = fields.quantize(f, eps)
+        g1 = oracle.correct(f, g0, eps, rng, oracle.flags())["g"]
+        D, H, W = shape
+        F = oracle.counts(W, H, D)[2]
+        for g in (g0, g1):
+            for edit_type in (0, 1):
+                ofl = oracle.flags(preserve_vpath=1, preserve_geometry=1, preserve_connectors=1, edit_type=edit_type)
+                want = oracle.paths(f, g, ofl)
+                face, edge = np.zeros(F, np.int64), np.zeros(F, np.int64)
+                tg = np.zeros(f.size, np.uint8)
+                n = hc.hc_walls_f32(_p(f), _p(g), W, H, D, eps, rng, edit_type, _p(face), _p(edge), _p(tg))
+                assert n >= 0
+                assert sorted(zip(face[:n].tolist(), edge[:n].tolist())) == \
+                    sorted(zip(want["wall"].tolist(), want["wall_rec"].tolist()))
+                want_t, nw = oracle.wall_targets(f, g, eps, rng, ofl)
+                assert nw == n
+                assert (tg.astype(bool).reshape(f.shape) == want_t).all()
+                seen += n
+    assert seen > 0

@@ -191,6 +191,7 @@ def _crit_sets(grad):
     dict(shape=(9, 11, 10), rel=2e-2, fl=dict(preserve_vpath=1, preserve_geometry=0)),
     dict(shape=(10, 10, 10), rel=2e-2, fl=dict(filtration=1)),
     dict(shape=(10, 9, 11), rel=2e-2, fl=dict(preserve_type=1)),
+    dict(shape=(11, 10, 12), rel=2e-2, fl=dict(preserve_vpath=1, preserve_geometry=1, preserve_connectors=1)),
 ])
 def test_correction_end_state(oracle, case):
     f = fields.gaussian_mixture(case["shape"], 5, seed=11, noise=0.02)
@@ -339,3 +340,53 @@ def test_finisher_leaves_converging_runs_unchanged(oracle):
     b = oracle.correct(f, g, eps, rng, oracle.flags(**fl, finish=1))
     assert a["status"] == b["status"] == 0 and b["stalls"] == 0
     assert a["g"].tobytes() == b["g"].tobytes() and (a["counter"] == b["counter"]).all()
+
+
+def _wall_closures(oracle, f, g):
+    """For every f-critical face: the faces a wall walk can reach without crossing an edge whose
+    f and g partners differ (breadth first, from the f and g pairing arrays), and the edges of
+    those faces whose f partner exists and differs from the g partner."""
+    D, H, W = f.shape
+    dims = (W, H, D)
+    _, pe_f, pf_f, _ = oracle.gradient(f)
+    _, pe_g, _, _ = oracle.gradient(g)
+    out = {}
+    for i in np.nonzero(pf_f == -1)[0].tolist():
+        seen, todo, bad = {i}, [i], set()
+        while todo:
+            t = todo.pop()
+            a, b, c = oracle.tri_verts(dims, t)
+            for u, v in ((a, b), (a, c), (b, c)):
+                j = oracle.edge_id(dims, u, v)
+                pj = int(pe_f[j])
+                if pj == -1:
+                    continue
+                if pj != int(pe_g[j]):
+                    bad.add(j)
+                elif pj != t and 0 <= pj < len(pf_f) and int(pf_f[pj]) == j and pj not in seen:
+                    seen.add(pj)
+                    todo.append(pj)
+        out[i] = bad
+    return out
+
+
+@pytest.mark.parametrize("seed", [2, 5])
+def test_wall_detection_invariants(oracle, seed):
+    """Walls (the deterministic get_false_ds_wall): a face is listed exactly when its wall,
+    explored up to the edges whose partners differ, has such an edge, and the recorded edge is
+    one of them. Which of them the depth-first order meets first is not pinned."""
+    f = fields.gaussian_mixture((16, 16, 16), 5, seed=seed, noise=0.02)
+    eps, rng = fields.eps_from_rel(f, 5e-3)
+    g0 = fields.quantize(f, eps)
+    g1 = oracle.correct(f, g0, eps, rng, oracle.flags())["g"]  # critical cells match, walls need not
+    listed_any = False
+    for g in (g0, g1):
+        got = oracle.paths(f, g, oracle.flags(preserve_vpath=1, preserve_geometry=1, preserve_connectors=1))
+        rec = dict(zip(got["wall"].tolist(), got["wall_rec"].tolist()))
+        assert len(rec) == len(got["wall"])
+        want = _wall_closures(oracle, f, g)
+        assert set(rec) == {i for i, bad in want.items() if bad}
+        for i, j in rec.items():
+            assert j in want[i]
+        listed_any = listed_any or len(rec) > 0
+    assert listed_any
