@@ -61,6 +61,10 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB four times (the known-size "
                          "reference for scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
+    ap.add_argument("--batch", type=int, default=0, metavar="K",
+                    help="time the batched form instead: K fields with sides drawn from 32..192, half "
+                         "Gaussian mixtures and half fBm, eps log-uniform in [1e-4, 1e-2]*range, corrected "
+                         "by one dmtz_correct_batch call per step")
     ap.add_argument("--finish", type=int, default=1,
                     help="1: resolve stalls (fixed points where the reference loops forever) by snapping "
                          "growing boxes around the stalled edit set")
@@ -86,6 +90,9 @@ def main():
 
     import dmtz_amd
     from dmtz_amd import fields
+
+    if args.batch:
+        return bench_batch(args)
 
     n = args.size
     shape = (n, n, n)
@@ -184,6 +191,118 @@ def main():
                    "sweeps": res.sweeps, "form": "plain", "finish": args.finish,
                    "stalls": res.info.get("stalls"), "finisher_vertices": res.info.get("finished")},
         "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def batch_fields(k, seed):
+    """k fields: sides drawn independently from 32..192, alternately a Gaussian mixture (8 bumps,
+    0.01 noise) and an fBm field generated on the next power-of-two cube and cropped; eps drawn
+    log-uniform in [1e-4, 1e-2]*range."""
+    import numpy as np
+    from dmtz_amd import fields
+
+    r = np.random.default_rng([seed, 48])
+    out = []
+    for i in range(k):
+        shape = tuple(int(x) for x in r.integers(32, 193, size=3))
+        rel = float(10.0 ** r.uniform(-4.0, -2.0))
+        if i % 2 == 0:
+            f = fields.gaussian_mixture(shape, 8, seed=seed * 1000 + i, noise=0.01)
+        else:
+            p = 1 << int(np.ceil(np.log2(max(shape))))
+            full = fields.fbm((p, p, p), seed=seed * 1000 + i, device="cuda:0").cpu().numpy()
+            f = np.ascontiguousarray(full[:shape[0], :shape[1], :shape[2]], dtype=np.float32)
+        eps, rng = fields.eps_from_rel(f, rel)
+        out.append((f, fields.quantize(f, eps), eps, rng, rel))
+    return out
+
+
+def bench_batch(args):
+    """The batched form: one dmtz_correct_batch call per step over K device-resident fields."""
+    import numpy as np
+    import torch
+
+    import dmtz_amd
+
+    log("generating %d batch fields" % args.batch)
+    items = batch_fields(args.batch, args.seed)
+    dev = torch.device("cuda:0")
+    fd = [torch.from_numpy(f).to(dev) for f, _, _, _, _ in items]
+    g0 = [torch.from_numpy(np.ascontiguousarray(g, dtype=np.float32)).to(dev) for _, g, _, _, _ in items]
+    gd = [torch.empty_like(x) for x in g0]
+    flags = dmtz_amd.CorrectionFlags(preserve_saddles=1, preserve_vpath=1 if args.geometry else 0,
+                                     preserve_geometry=1 if args.geometry else 0, max_sweeps=args.max_sweeps,
+                                     finish=args.finish)
+    ctx = dmtz_amd.Context(0)
+    batch = [(fd[i], gd[i], items[i][2], items[i][3]) for i in range(len(items))]
+
+    def step():
+        for a, b in zip(gd, g0):
+            a.copy_(b)
+        return dmtz_amd.correct_batch(batch, flags, ctx=ctx)
+
+    for w in range(args.warmup):
+        rs = step()
+        log("warmup %d: sweeps %s" % (w, [r.sweeps for r in rs]))
+    ctx.kernel_stats(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        rs = step()
+        log("step %d: total sweeps %d" % (s, sum(r.sweeps for r in rs)))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    for i, r in enumerate(rs):
+        assert r.converged, (i, r.counts)
+        err = (gd[i].double() - fd[i].double()).abs().max().item()
+        assert err <= items[i][2], (i, err)
+    voxels = sum(int(f.size) for f, _, _, _, _ in items)
+    ks = ctx.kernel_stats()
+    launches = max(ks["launches"], 1.0)
+    avg_ms = ks["ms"] / launches
+    bytes_per_launch = ks["bytes"] / launches
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    workload = ("batch of %d float32 fields, sides 32..192, Gaussian mixture / fBm, eps log-uniform "
+                "[1e-4, 1e-2]*range, %s" % (args.batch, "critical cells + separatrix geometry"
+                                             if args.geometry else "critical cells"))
+    cpu = None
+    if args.cpu_sample > 0:
+        from oracle import oracle as O
+
+        O.build()
+        threads = O.set_threads(min(args.cpu_threads, os.cpu_count() or 1))
+        k = int(np.argmin([f.size for f, _, _, _, _ in items]))
+        f, g, eps, rng, _ = items[k]
+        log("cpu baseline: oracle on the smallest field %s, %d threads" % (f.shape, threads))
+        c0 = time.perf_counter()
+        rc = O.correct(f, g, eps, rng, O.flags(preserve_saddles=1, preserve_vpath=flags.preserve_vpath,
+                                               preserve_geometry=flags.preserve_geometry,
+                                               max_sweeps=args.max_sweeps, finish=args.finish))
+        c1 = time.perf_counter()
+        cpu = dict(value=round(f.size / (c1 - c0), 1), unit="voxels/s", cores=threads, kind="port",
+                   sample="the batch's smallest field %s: %d sweeps in %.2f s" % (f.shape, rc["sweeps"], c1 - c0))
+    out = {
+        "metric": "voxels/s to zero topology mismatch within eps (batched form, 1 GPU)",
+        "value": round(voxels * args.steps / elapsed, 1),
+        "unit": "voxels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": workload, "fields": args.batch, "voxels": voxels, "seed": args.seed,
+                   "sweeps": [r.sweeps for r in rs], "form": "batched (one call, fields in sequence)",
+                   "finish": args.finish},
+        "roofline": dict(bound="hbm", kernel="k_detect (dense mismatch detection)", achieved=round(achieved, 2),
+                         peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None,
+                         bytes_per_launch=int(bytes_per_launch), avg_launch_ms=round(avg_ms, 4),
+                         launches=int(ks["launches"])),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

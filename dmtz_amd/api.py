@@ -304,13 +304,16 @@ def classify(field_, ctx=None, masks=False):
 
 
 def correct_batch(items, flags=None, ctx=None):
-    """Correct several fields with the same flags in one call. items: list of (f, g, eps, rng)
-    numpy arrays (g is copied, not modified). Returns a list of CorrectionResult (without
+    """Correct several fields with the same flags in one call. items: list of (f, g, eps, rng),
+    either numpy arrays (g is copied, not modified) or contiguous cuda tensors on the context's
+    device (g is corrected in place, no copies). Returns a list of CorrectionResult (without
     trajectories)."""
     ctx = ctx or default_context()
     flags = flags or CorrectionFlags()
     if not items:
         return []
+    if getattr(items[0][0], "is_cuda", False):
+        return _correct_batch_device(items, flags, ctx)
     dt = np.ascontiguousarray(items[0][0]).dtype
     tag = _tag(dt)
     FieldT = N.FieldF32 if tag == "f32" else N.FieldF64
@@ -331,4 +334,29 @@ def correct_batch(items, flags=None, ctx=None):
                                                          _p(status), _p(sweeps))
     ctx.check(rc, "dmtz_correct_batch")
     return [CorrectionResult(keep[i][1], counts[i].as_tuple(), int(sweeps[i]), int(status[i]))
+            for i in range(n)]
+
+
+def _correct_batch_device(items, flags, ctx):
+    import torch
+
+    tag = {torch.float32: "f32", torch.float64: "f64"}[items[0][0].dtype]
+    FieldT = N.FieldF32 if tag == "f32" else N.FieldF64
+    arr = (FieldT * len(items))()
+    for i, (f, g, eps, rng) in enumerate(items):
+        assert f.is_cuda and g.is_cuda and f.is_contiguous() and g.is_contiguous()
+        assert f.dtype == g.dtype and f.shape == g.shape and f.dim() == 3
+        nz, ny, nx = f.shape
+        arr[i] = FieldT(f.data_ptr(), g.data_ptr(), nx, ny, nz, float(eps), float(rng), 1, 0)
+    n = len(items)
+    counts = (N.Counts * n)()
+    status = np.zeros(n, np.int32)
+    sweeps = np.zeros(n, np.int32)
+    fl = flags.c()
+    torch.cuda.synchronize(items[0][0].device)
+    rc = getattr(ctx._lib, "dmtz_correct_batch_" + tag)(ctx.handle, arr, n, ctypes.byref(fl),
+                                                         ctypes.cast(counts, ctypes.c_void_p),
+                                                         _p(status), _p(sweeps))
+    ctx.check(rc, "dmtz_correct_batch")
+    return [CorrectionResult(items[i][1], counts[i].as_tuple(), int(sweeps[i]), int(status[i]))
             for i in range(n)]
