@@ -86,11 +86,25 @@ def lib():
     L.dmtz_phase_times.argtypes = [P, P]
     L.dmtz_kernel_stats.argtypes = [P, P, ctypes.c_int]
     L.dmtz_run_info.argtypes = [P, P]
+    # learned warm start (include/dmtz_learned.h)
+    U64 = ctypes.c_uint64
+    F = ctypes.c_float
+    L.dmtz_net_create.argtypes = [P, U64, ctypes.POINTER(P)]
+    L.dmtz_net_destroy.argtypes = [P]
+    L.dmtz_net_destroy.restype = None
+    L.dmtz_net_params.argtypes = [P, P, P]
+    L.dmtz_net_set_params.argtypes = [P, P, P]
+    L.dmtz_net_features_f32.argtypes = [P, P, I64, I64, I64, D, D, P]
+    L.dmtz_net_forward.argtypes = [P, P, P, I64, I64, I64, P]
+    L.dmtz_net_gradient.argtypes = [P, P, P, P, I64, I64, I64, F, F, P, P]
+    L.dmtz_net_train_step.argtypes = [P, P, P, P, I64, I64, I64, F, F, F, P]
+    L.dmtz_correct_learned_f32.argtypes = [P, P, P, P, I64, I64, I64, D, D, ctypes.POINTER(Flags),
+                                           ctypes.POINTER(Counts), P, ctypes.POINTER(I32), P]
     _LIB = L
     return L
 
 
-# Every symbol declared in include/dmtz.h.
+# Every symbol declared in include/dmtz.h and include/dmtz_learned.h.
 EXPORTS = (
     "dmtz_ctx_create", "dmtz_ctx_destroy", "dmtz_last_error",
     "dmtz_eps_from_rel_f32", "dmtz_eps_from_rel_f64",
@@ -102,4 +116,7 @@ EXPORTS = (
     "dmtz_classify_f32", "dmtz_classify_f64",
     "dmtz_correct_batch_f32", "dmtz_correct_batch_f64",
     "dmtz_walls_f32", "dmtz_walls_f64",
+    "dmtz_net_create", "dmtz_net_destroy", "dmtz_net_params", "dmtz_net_set_params",
+    "dmtz_net_features_f32", "dmtz_net_forward", "dmtz_net_gradient", "dmtz_net_train_step",
+    "dmtz_correct_learned_f32",
 )

@@ -25,7 +25,7 @@ HIPCC_FLAGS = [
     "-I" + os.path.join(ROOT, "include"),
 ]
 
-SOURCES = [os.path.join(CSRC, "dmtz_hip.hip")]
+SOURCES = [os.path.join(CSRC, "dmtz_hip.hip"), os.path.join(CSRC, "learned.hip")]
 
 
 def hipcc():

@@ -2460,3 +2460,10 @@ int dmtz_walls_f64(dmtz_ctx* ctx, const double* f, const double* g, int64_t nx, 
 }
 
 }  // extern "C"
+
+// Context internals for the library's other translation unit (learned.hip).
+namespace dmtz_internal {
+hipStream_t stream(dmtz_ctx* ctx) { return ctx->stream; }
+int device(dmtz_ctx* ctx) { return ctx->device; }
+void set_error(dmtz_ctx* ctx, const char* msg) { ctx->err = msg; }
+}  // namespace dmtz_internal

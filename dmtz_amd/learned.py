@@ -1,0 +1,146 @@
+"""The learned warm start (include/dmtz_learned.h): host side.
+
+A four-layer 3x3x3 convolutional network, trained on (g, g') pairs that the plain correction
+produces, predicts which vertices the plain loop will edit and by how much. Its prediction is
+applied as a first edit of g (never past the snap of f - eps), and the plain loop then runs to
+zero mismatches as usual, so it stays the judge. Every kernel (features, forward, backward,
+Adam, warm start) is the library's own HIP code; torch only holds device memory.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native as N
+from .api import CorrectionFlags, CorrectionResult, correct_device, default_context
+
+PARAMS = 4 * 6928
+CHANNELS = 16
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Net:
+    """A network on the context's device. Parameters: 4 x (W[27][16][16], b[16]) fp32."""
+
+    def __init__(self, seed=0, ctx=None):
+        self.ctx = ctx or default_context()
+        self._lib = self.ctx._lib
+        h = ctypes.c_void_p()
+        self.ctx.check(self._lib.dmtz_net_create(self.ctx.handle, seed, ctypes.byref(h)), "dmtz_net_create")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            self._lib.dmtz_net_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        out = np.zeros(PARAMS, np.float32)
+        self.ctx.check(self._lib.dmtz_net_params(self.ctx.handle, self.handle, out.ctypes.data_as(ctypes.c_void_p)),
+                       "dmtz_net_params")
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == PARAMS
+        self.ctx.check(self._lib.dmtz_net_set_params(self.ctx.handle, self.handle,
+                                                     p.ctypes.data_as(ctypes.c_void_p)), "dmtz_net_set_params")
+
+    def features(self, g, eps, rng):
+        """Input channels of a float32 cuda field g [z, y, x]: int16 cuda tensor [z, y, x, 16]
+        holding the bf16 bit patterns."""
+        import torch
+
+        assert g.is_cuda and g.dtype == torch.float32 and g.is_contiguous()
+        nz, ny, nx = g.shape
+        feat = torch.empty((nz, ny, nx, CHANNELS), dtype=torch.int16, device=g.device)
+        torch.cuda.synchronize(g.device)
+        self.ctx.check(self._lib.dmtz_net_features_f32(self.ctx.handle, _ptr(g), nx, ny, nz, float(eps), float(rng),
+                                                       _ptr(feat)), "dmtz_net_features")
+        return feat
+
+    def forward(self, feat):
+        """(edit logit, edit size / eps) per vertex: float32 cuda tensor [z, y, x, 2]."""
+        import torch
+
+        nz, ny, nx, _ = feat.shape
+        out = torch.empty((nz, ny, nx, 2), dtype=torch.float32, device=feat.device)
+        torch.cuda.synchronize(feat.device)
+        self.ctx.check(self._lib.dmtz_net_forward(self.ctx.handle, self.handle, _ptr(feat), nx, ny, nz, _ptr(out)),
+                       "dmtz_net_forward")
+        return out
+
+    def gradient(self, feat, labels, pos_weight=1.0, lam=1.0):
+        """Parameter gradient (numpy float32) and mean loss terms (bce, size) without an update."""
+        import torch
+
+        nz, ny, nx, _ = feat.shape
+        g = np.zeros(PARAMS, np.float32)
+        loss = np.zeros(2, np.float64)
+        torch.cuda.synchronize(feat.device)
+        self.ctx.check(self._lib.dmtz_net_gradient(self.ctx.handle, self.handle, _ptr(feat), _ptr(labels), nx, ny, nz,
+                                                   float(pos_weight), float(lam), g.ctypes.data_as(ctypes.c_void_p),
+                                                   loss.ctypes.data_as(ctypes.c_void_p)), "dmtz_net_gradient")
+        return g, loss
+
+    def train_step(self, feat, labels, lr, pos_weight=1.0, lam=1.0):
+        import torch
+
+        nz, ny, nx, _ = feat.shape
+        loss = np.zeros(2, np.float64)
+        torch.cuda.synchronize(feat.device)
+        self.ctx.check(self._lib.dmtz_net_train_step(self.ctx.handle, self.handle, _ptr(feat), _ptr(labels), nx, ny,
+                                                     nz, float(pos_weight), float(lam), float(lr),
+                                                     loss.ctypes.data_as(ctypes.c_void_p)), "dmtz_net_train_step")
+        return loss
+
+
+def labels(g, g_corrected, eps):
+    """Training targets of one plain correction: (y, t) per vertex as a float32 cuda tensor
+    [z, y, x, 2]; y = 1 where the loop changed g, t = (g - g') / eps."""
+    import torch
+
+    y = (g_corrected != g).to(torch.float32)
+    t = ((g.double() - g_corrected.double()) / eps).to(torch.float32)
+    return torch.stack([y, t], dim=-1).contiguous()
+
+
+def training_pair(net, f, g, eps, rng, flags=None, ctx=None):
+    """Features and labels of one field from a plain correction of (f, g) (cuda tensors)."""
+    flags = flags or CorrectionFlags(preserve_vpath=1, preserve_geometry=1, finish=1)
+    gc = g.clone()
+    r = correct_device(f, gc, eps, rng, flags, ctx=ctx or net.ctx)
+    assert r.converged, r.counts
+    return net.features(g, eps, rng), labels(g, gc, eps)
+
+
+def correct_learned(net, f, g, eps, rng, flags=None, counter_out=None):
+    """Warm-started correction of cuda tensors f, g (g corrected in place). Returns
+    (CorrectionResult, number of warm-start edits)."""
+    import torch
+
+    ctx = net.ctx
+    flags = flags or CorrectionFlags()
+    assert f.is_cuda and g.is_cuda and f.dtype == torch.float32 and g.dtype == torch.float32
+    nz, ny, nx = f.shape
+    counts = N.Counts()
+    sweeps = ctypes.c_int32(0)
+    warm = ctypes.c_int64(0)
+    fl = flags.c()
+    torch.cuda.synchronize(f.device)
+    rc = net._lib.dmtz_correct_learned_f32(ctx.handle, net.handle, _ptr(f), _ptr(g), nx, ny, nz, float(eps),
+                                           float(rng), ctypes.byref(fl), ctypes.byref(counts),
+                                           _ptr(counter_out) if counter_out is not None else None,
+                                           ctypes.byref(sweeps), ctypes.byref(warm))
+    ctx.check(rc, "dmtz_correct_learned")
+    res = CorrectionResult(g, counts.as_tuple(), int(sweeps.value), rc, None, ctx.trajectory(), ctx.phases(),
+                           ctx.run_info())
+    return res, int(warm.value)

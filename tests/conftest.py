@@ -23,6 +23,12 @@ def oracle():
 
 @pytest.fixture(scope="session")
 def dev():
+    # torch (which some GPU tests use for device memory) ships its own HIP runtime; it must see the
+    # device before the library's runtime does, so it is initialised first
+    import torch
+
+    if torch.cuda.is_available():
+        torch.cuda.init()
     import dmtz_amd
 
     return dmtz_amd.default_context()

@@ -1,4 +1,4 @@
-"""The C-ABI library builds/loads and exports every entry point include/dmtz.h declares
+"""The C-ABI library builds/loads and exports every entry point include/*.h declares
 (no device calls: runs without a GPU)."""
 import os
 import re
@@ -9,8 +9,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared():
-    src = open(os.path.join(ROOT, "include", "dmtz.h")).read()
-    return sorted(set(re.findall(r"\b(dmtz_\w+)\s*\(", src)))
+    names = set()
+    for h in ("dmtz.h", "dmtz_learned.h"):
+        src = open(os.path.join(ROOT, "include", h)).read()
+        names |= set(re.findall(r"\b(dmtz_\w+)\s*\(", src))
+    return sorted(names)
 
 
 def test_header_symbols_exported():

@@ -1,0 +1,148 @@
+"""GPU checks of the learned warm start (include/dmtz_learned.h) through the C ABI: input
+channels, forward pass and parameter gradient against the CPU restatement oracle/learned_ref.py,
+training, and the warm-started correction against the plain one."""
+import time
+
+import numpy as np
+import pytest
+
+import dmtz_amd
+from dmtz_amd import fields
+from dmtz_amd import learned as L
+from oracle import learned_ref as R
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+
+    assert torch.cuda.is_available()
+    return torch.device("cuda:0")
+
+
+@pytest.fixture(scope="module")
+def net(dev, torch_dev):
+    n = L.Net(seed=3, ctx=dev)
+    yield n
+    n.close()
+
+
+def _feat_np(feat):
+    return R.bf16_bits(feat.cpu().numpy().view(np.uint16))
+
+
+def test_features_match_reference(dev, net, torch_dev):
+    import torch
+
+    for shape, rel, seed in [((13, 18, 21), 1e-2, 0), ((9, 7, 33), 1e-3, 1)]:
+        f = fields.gaussian_mixture(shape, 5, seed=seed, noise=0.02)
+        eps, rng = fields.eps_from_rel(f, rel)
+        g = fields.quantize(f, eps).astype(np.float32)
+        print("features", shape, flush=True)
+        feat = _feat_np(net.features(torch.from_numpy(g).to(torch_dev), eps, rng))
+        want = R.features(g, eps, rng)
+        for c in (0, 5, 6, 7):
+            d = np.abs(feat[..., c] - want[..., c])
+            assert (d <= np.abs(want[..., c]) * 2 ** -7 + 1e-30).all(), c  # within one bf16 step
+        counts, lo, hi = dmtz_amd.classify(g, ctx=dev, masks=True)
+        lo = lo.astype(np.uint64)
+        pc = np.vectorize(lambda x: bin(int(x)).count("1"))
+        assert (feat[..., 1] == (lo & np.uint64(1))).all()
+        assert (feat[..., 2] == pc((lo >> np.uint64(1)) & np.uint64(0x3FFF))).all()
+        assert (feat[..., 3] == pc(lo >> np.uint64(15))).all()
+        assert (feat[..., 4] == pc(hi)).all()
+        assert (feat[..., 8:] == 0).all()
+
+
+def test_forward_and_gradient_match_reference(dev, torch_dev):
+    import torch
+
+    rng = np.random.default_rng(7)
+    m = R.mask()
+    for shape in [(5, 6, 21), (11, 9, 16), (3, 4, 40)]:
+        p = (rng.normal(0, 0.25, 4 * R.LP) * m).astype(np.float32)
+        feat = np.zeros(shape + (16,), np.float32)
+        feat[..., :8] = rng.normal(0, 1, shape + (8,))
+        fb = R.bf16(feat)
+        bits = (fb.astype(np.float32).view(np.uint32) >> 16).astype(np.uint16).view(np.int16)
+        lab = np.stack([(rng.random(shape) < 0.2), rng.uniform(0, 2, shape)], axis=-1).astype(np.float32)
+        n = L.Net(seed=0, ctx=dev)
+        n.set_params(p)
+        ft = torch.from_numpy(np.ascontiguousarray(bits)).to(torch_dev)
+        lt = torch.from_numpy(lab).to(torch_dev)
+        print("net", shape, flush=True)
+        z = n.forward(ft).cpu().numpy().astype(np.float64)
+        _, zr = R.forward(fb, p)
+        assert np.abs(z - zr).max() <= 2e-2 * np.abs(zr).max() + 1e-4
+        g, loss = n.gradient(ft, lt, pos_weight=4.0, lam=0.5)
+        gr, lr_ = R.gradient(fb, lab.astype(np.float64), p, pos_weight=4.0, lam=0.5)
+        assert np.all(g[m == 0] == 0)
+        assert abs(loss[0] - lr_[0]) <= 1e-2 * abs(lr_[0]) + 1e-5
+        assert abs(loss[1] - lr_[1]) <= 1e-2 * abs(lr_[1]) + 1e-5
+        for l in range(4):
+            a, b = g[l * R.LP:(l + 1) * R.LP], gr[l * R.LP:(l + 1) * R.LP]
+            scale = np.abs(b).max()
+            assert np.abs(a - b).max() <= 3e-2 * scale + 1e-7, (shape, l, np.abs(a - b).max(), scale)
+            assert np.dot(a, b) / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30) > 0.999
+        n.close()
+
+
+def _pairs(net, torch_dev, specs):
+    import torch
+
+    out = []
+    for shape, rel, seed, kind in specs:
+        if kind == "gm":
+            f = fields.gaussian_mixture(shape, 8, seed=seed, noise=0.01)
+        else:
+            f = fields.fbm(shape, seed=seed).astype(np.float32)
+        eps, rng = fields.eps_from_rel(f, rel)
+        g = fields.quantize(f, eps).astype(np.float32)
+        fd = torch.from_numpy(np.ascontiguousarray(f, dtype=np.float32)).to(torch_dev)
+        gd = torch.from_numpy(g).to(torch_dev)
+        feat, lab = L.training_pair(net, fd, gd, eps, rng)
+        out.append((fd, gd, eps, rng, feat, lab))
+    return out
+
+
+def test_training_lowers_loss_and_warm_start_converges(dev, net, torch_dev):
+    """Training on plain-method pairs lowers the loss; the warm-started loop still ends with zero
+    mismatches within eps; the gain over the plain loop on held-out fields is reported."""
+    import torch
+
+    train = _pairs(net, torch_dev, [((48, 48, 48), 10 ** (-2 - k % 3 * 0.5), 100 + k, "gm" if k % 2 else "fbm")
+                                    for k in range(8)])
+    pos = 1.0 / max(1e-3, float(np.mean([t[5][..., 0].mean().item() for t in train])))
+    pos = min(pos, 50.0)
+    first = last = None
+    for epoch in range(30):
+        tot = 0.0
+        for fd, gd, eps, rng, feat, lab in train:
+            loss = net.train_step(feat, lab, lr=3e-3, pos_weight=pos, lam=0.5)
+            tot += loss[0] + 0.5 * loss[1]
+        first = tot if first is None else first
+        last = tot
+    print("loss %.4f -> %.4f" % (first, last), flush=True)
+    assert last < 0.8 * first
+    flags = dmtz_amd.CorrectionFlags(preserve_vpath=1, preserve_geometry=1, finish=1)
+    for shape, rel, seed, kind in [((64, 64, 64), 1e-2, 7, "gm"), ((64, 64, 64), 3e-3, 8, "fbm")]:
+        f = fields.gaussian_mixture(shape, 8, seed=seed, noise=0.01) if kind == "gm" else \
+            fields.fbm(shape, seed=seed).astype(np.float32)
+        eps, rng = fields.eps_from_rel(f, rel)
+        g = fields.quantize(f, eps).astype(np.float32)
+        fd = torch.from_numpy(np.ascontiguousarray(f, dtype=np.float32)).to(torch_dev)
+        g1 = torch.from_numpy(g).to(torch_dev)
+        g2 = g1.clone()
+        t0 = time.perf_counter()
+        plain = dmtz_amd.correct_device(fd, g1, eps, rng, flags, ctx=dev)
+        t1 = time.perf_counter()
+        warm, edits = L.correct_learned(net, fd, g2, eps, rng, flags)
+        t2 = time.perf_counter()
+        print("held-out %s %s: plain %d sweeps %.1f ms, warm %d sweeps %.1f ms (%d warm edits)"
+              % (kind, shape, plain.sweeps, (t1 - t0) * 1e3, warm.sweeps, (t2 - t1) * 1e3, edits), flush=True)
+        assert plain.converged and warm.converged
+        assert warm.counts == (0,) * 8
+        err = (g2.double() - fd.double()).abs().max().item()
+        assert err <= eps
