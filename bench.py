@@ -65,6 +65,9 @@ def main():
                     help="time the batched form instead: K fields with sides drawn from 32..192, half "
                          "Gaussian mixtures and half fBm, eps log-uniform in [1e-4, 1e-2]*range, corrected "
                          "by one dmtz_correct_batch call per step")
+    ap.add_argument("--learned", default="", metavar="NPY",
+                    help="also time the learned warm start with these network parameters (from "
+                         "python -m dmtz_amd.learned_train) on the same field, and report it beside the plain form")
     ap.add_argument("--finish", type=int, default=1,
                     help="1: resolve stalls (fixed points where the reference loops forever) by snapping "
                          "growing boxes around the stalled edit set")
@@ -154,6 +157,10 @@ def main():
                     traffic=measured_traffic(workload, args.seed), bytes_per_launch=int(bytes_per_launch),
                     avg_launch_ms=round(avg_ms, 4), launches=int(ks["launches"]))
 
+    learned = None
+    if args.learned:
+        learned = bench_learned(args, ctx, f_d, g0_d, g_d, eps, rng, flags, value)
+
     cpu = None
     if args.cpu_sample > 0:
         from oracle import oracle as O
@@ -193,7 +200,42 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if learned is not None:
+        out["learned"] = learned
     print(json.dumps(out), flush=True)
+
+
+def bench_learned(args, ctx, f_d, g0_d, g_d, eps, rng, flags, plain_value):
+    """The learned warm start on the same field: features, forward pass, warm edits and the plain
+    loop from there, timed like the plain steps."""
+    import numpy as np
+    import torch
+
+    from dmtz_amd import learned as L
+
+    net = L.Net(seed=0, ctx=ctx)
+    net.set_params(np.load(args.learned))
+
+    def step():
+        g_d.copy_(g0_d)
+        return L.correct_learned(net, f_d, g_d, eps, rng, flags)
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        res, edits = step()
+        log("learned step %d: %d sweeps, %d warm edits, counts %s" % (s, res.sweeps, edits, res.counts))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    assert res.converged, res.counts
+    assert (g_d.double() - f_d.double()).abs().max().item() <= eps
+    value = f_d.numel() * args.steps / elapsed
+    net.close()
+    return dict(value=round(value, 1), unit="voxels/s", ms_per_step=round(elapsed / args.steps * 1e3, 3),
+                sweeps=res.sweeps, warm_edits=edits, speedup_vs_plain=round(value / plain_value, 3),
+                params=os.path.basename(args.learned))
 
 
 def batch_fields(k, seed):

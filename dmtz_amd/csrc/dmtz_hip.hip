@@ -32,6 +32,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <algorithm>
+#include <atomic>
+#include <thread>
 #include <vector>
 
 #include "../../include/dmtz.h"
@@ -997,6 +1000,7 @@ struct dmtz_ctx {
     // live timing of the dense detection kernel on the context stream (HIP events)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double det_launches = 0, det_ms = 0, det_bytes = 0, det_voxels = 0;
+    std::vector<dmtz_ctx*> workers;  // batch lanes: contexts of their own (stream, buffers)
     double dirty_total = 0;  // star codes re-evaluated incrementally in the last correction
     int64_t stalls = 0, finished = 0, radius = 0;  // finisher outcome of the last correction
     // precomputed ascending separatrix paths of f (geometry mode)
@@ -2012,7 +2016,20 @@ int classify_host(dmtz_ctx* ctx, const T* fld, i64 nx, i64 ny, i64 nz, int64_t* 
     return 0;
 }
 
-// Batch correction (dmtz_correct_batch_*): the fields one after another on the context.
+// Batch correction (dmtz_correct_batch_*). Fields run concurrently in up to BATCH_LANES lanes,
+// each a host thread with a worker context of its own (its own stream and buffers), so the
+// sweeps of different fields overlap on the device and their per-sweep host round trips
+// overlap too. Lanes take the fields largest first from a shared counter. Every field's result
+// is what a single correction gives (each run is deterministic and lanes share nothing).
+constexpr int BATCH_LANES = 8;
+
+template <class T, class Field>
+int correct_one(dmtz_ctx* w, const Field& fd, const dmtz_flags* flags, dmtz_counts* cnt, int32_t* sw) {
+    return fd.on_device ? correct_dev<T>(w, fd.f, fd.g, fd.nx, fd.ny, fd.nz, fd.eps, fd.range, flags, cnt, nullptr, sw)
+                        : correct_host<T>(w, fd.f, fd.g, fd.nx, fd.ny, fd.nz, fd.eps, fd.range, flags, cnt, nullptr,
+                                          sw);
+}
+
 template <class T, class Field>
 int correct_batch(dmtz_ctx* ctx, const Field* fields, int32_t n, const dmtz_flags* flags, dmtz_counts* counts_out,
                   int32_t* status_out, int32_t* sweeps_out) {
@@ -2020,21 +2037,62 @@ int correct_batch(dmtz_ctx* ctx, const Field* fields, int32_t n, const dmtz_flag
         if (ctx) ctx->err = "null or invalid argument";
         return DMTZ_E_ARG;
     }
+    if (n == 0) return 0;
+    const int lanes = n < BATCH_LANES ? n : BATCH_LANES;
+    while ((int)ctx->workers.size() < lanes) {
+        dmtz_ctx* w = nullptr;
+        const int rc = dmtz_ctx_create(&w, ctx->device);
+        if (rc) {
+            ctx->err = "batch lane context creation failed";
+            return rc;
+        }
+        ctx->workers.push_back(w);
+    }
+    for (dmtz_ctx* w : ctx->workers) w->err.clear();
+    std::vector<int32_t> order(n);
+    for (int32_t i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        return fields[a].nx * fields[a].ny * fields[a].nz > fields[b].nx * fields[b].ny * fields[b].nz;
+    });
+    std::vector<int> rcs(n, 0);
+    std::vector<dmtz_counts> cnts(n);
+    std::vector<int32_t> sws(n, 0);
+    std::atomic<int32_t> next{0};
+    std::atomic<bool> failed{false};
+    auto lane = [&](dmtz_ctx* w) {
+        (void)hipSetDevice(w->device);
+        for (;;) {
+            const int32_t k = next.fetch_add(1);
+            if (k >= n || failed.load()) break;
+            const int32_t i = order[k];
+            rcs[i] = correct_one<T>(w, fields[i], flags, &cnts[i], &sws[i]);
+            if (rcs[i] < 0) failed.store(true);
+        }
+    };
+    std::vector<std::thread> threads;
+    for (int l = 1; l < lanes; l++) threads.emplace_back(lane, ctx->workers[l]);
+    lane(ctx->workers[0]);
+    for (auto& t : threads) t.join();
+    (void)hipSetDevice(ctx->device);
     int worst = 0;
+    for (int l = 0; l < lanes; l++) {  // kernel statistics of the lanes go to the caller's context
+        dmtz_ctx* w = ctx->workers[l];
+        ctx->det_launches += w->det_launches;
+        ctx->det_ms += w->det_ms;
+        ctx->det_bytes += w->det_bytes;
+        ctx->det_voxels += w->det_voxels;
+        w->det_launches = w->det_ms = w->det_bytes = w->det_voxels = 0;
+    }
     for (int32_t i = 0; i < n; i++) {
-        const Field& fd = fields[i];
-        dmtz_counts cnt;
-        int32_t sw = 0;
-        const int rc = fd.on_device
-                           ? correct_dev<T>(ctx, fd.f, fd.g, fd.nx, fd.ny, fd.nz, fd.eps, fd.range, flags, &cnt,
-                                            nullptr, &sw)
-                           : correct_host<T>(ctx, fd.f, fd.g, fd.nx, fd.ny, fd.nz, fd.eps, fd.range, flags, &cnt,
-                                             nullptr, &sw);
-        if (status_out) status_out[i] = rc;
-        if (rc < 0) return rc;
-        if (counts_out) counts_out[i] = cnt;
-        if (sweeps_out) sweeps_out[i] = sw;
-        if (rc != 0) worst = 1;
+        if (status_out) status_out[i] = rcs[i];
+        if (rcs[i] < 0) {
+            for (dmtz_ctx* w : ctx->workers)
+                if (!w->err.empty()) ctx->err = w->err;
+            return rcs[i];
+        }
+        if (counts_out) counts_out[i] = cnts[i];
+        if (sweeps_out) sweeps_out[i] = sws[i];
+        if (rcs[i] != 0) worst = 1;
     }
     return worst;
 }
@@ -2295,6 +2353,8 @@ int dmtz_ctx_create(dmtz_ctx** out, int device) {
 
 void dmtz_ctx_destroy(dmtz_ctx* ctx) {
     if (!ctx) return;
+    for (dmtz_ctx* w : ctx->workers) dmtz_ctx_destroy(w);
+    ctx->workers.clear();
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     free_bufs(ctx);

@@ -86,7 +86,17 @@ def quantize(f, eps):
     """SZ-style decompressed field: g = 2 eps round(f / (2 eps)), clamped into [f - eps, f + eps].
 
     The clamp works on the float32-rounded values so that |g - f| <= eps holds in double
-    arithmetic for every stored value."""
+    arithmetic for every stored value. f may also be a torch tensor (the result is then one on
+    the same device)."""
+    if not isinstance(f, np.ndarray):
+        import torch
+
+        f64 = f.double()
+        g = (2.0 * eps * torch.round(f64 / (2.0 * eps))).to(f.dtype)
+        gd = g.double()
+        g = torch.where(gd - f64 > eps, torch.nextafter(g, torch.full_like(g, -float("inf"))), g)
+        g = torch.where(f64 - gd > eps, torch.nextafter(g, torch.full_like(g, float("inf"))), g)
+        return g.contiguous()
     f64 = f.astype(np.float64)
     q = 2.0 * eps * np.round(f64 / (2.0 * eps))
     g = q.astype(f.dtype)

@@ -121,12 +121,14 @@ typedef struct dmtz_field_f64 {
     int32_t reserved;
 } dmtz_field_f64;
 
-/* Batch correction: every field of the array is corrected with the same flags, one after another
- * on the context's stream and reusing its memory (the reference runs one field per process,
- * main_3d.cu:6255). counts_out, status_out and sweeps_out (each nullable) receive one entry per
- * field; status_out[i] is field i's return code. Returns 0 when every field converged, 1 when one
- * did not (not converged or stalled), or the first negative error (the fields after it are left
- * untouched). */
+/* Batch correction: every field of the array is corrected with the same flags (the reference runs
+ * one field per process, main_3d.cu:6255). The fields run concurrently, largest first, in up to
+ * 8 lanes: host threads with worker contexts of their own (stream and buffers, kept by ctx for
+ * later batches), so that their sweeps overlap on the device. Each field's result is exactly
+ * that of a single correction. counts_out, status_out and sweeps_out (each nullable) receive one
+ * entry per field; status_out[i] is field i's return code. Returns 0 when every field converged,
+ * 1 when one did not (not converged or stalled), or the first negative error in field order
+ * (fields not yet started then stay untouched). */
 int dmtz_correct_batch_f32(dmtz_ctx* ctx, const dmtz_field_f32* fields, int32_t nfields, const dmtz_flags* flags,
                            dmtz_counts* counts_out, int32_t* status_out, int32_t* sweeps_out);
 int dmtz_correct_batch_f64(dmtz_ctx* ctx, const dmtz_field_f64* fields, int32_t nfields, const dmtz_flags* flags,
