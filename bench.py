@@ -157,6 +157,16 @@ def main():
                     traffic=measured_traffic(workload, args.seed), bytes_per_launch=int(bytes_per_launch),
                     avg_launch_ms=round(avg_ms, 4), launches=int(ks["launches"]))
 
+    # The whole correction priced like one dense sweep per sweep (f and g read once, 8 bytes per
+    # mismatch record emitted): the path's HBM fraction end to end, beside the k_detect kernel's.
+    traj = np.asarray(res.trajectory).reshape(-1, 8) if res.trajectory is not None else np.zeros((0, 8))
+    records = float(traj[:, :4].sum()) if len(traj) else 0.0
+    b_run = res.sweeps * N * 2.0 * (8 if args.dtype == "f64" else 4) + 8.0 * records
+    run_gbs = b_run / (elapsed / args.steps) / 1e9
+    sweep_roofline = dict(bound="hbm", achieved=round(run_gbs, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                          frac=round(run_gbs / HBM_PEAK_GBS, 5), bytes_per_correction=int(b_run),
+                          basis="sweeps * (f and g once) + 8 B per mismatch record, over the whole correction")
+
     learned = None
     if args.learned:
         learned = bench_learned(args, ctx, f_d, g0_d, g_d, eps, rng, flags, value)
@@ -198,6 +208,7 @@ def main():
                    "sweeps": res.sweeps, "form": "plain", "finish": args.finish,
                    "stalls": res.info.get("stalls"), "finisher_vertices": res.info.get("finished")},
         "roofline": roofline,
+        "sweep_roofline": sweep_roofline,
         "cpu_baseline": cpu,
     }
     if learned is not None:

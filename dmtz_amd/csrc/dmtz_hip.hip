@@ -708,8 +708,14 @@ __global__ __launch_bounds__(BLOCK) void k_as_branch(const uint64_t* fcf, ull n,
 
 // Precomputed ascending paths (geometry mode). A branch j is (f-critical face j / 2, cofacet
 // j % 2); its comparison steps (Repair::as_fpath) are fixed for the whole run.
-struct CountSteps {
-    DMTZ_HD void operator()(i64, i64, i64) const {}
+// The first AS_K steps of a branch are kept by the counting walk, so that only longer branches
+// are walked a second time to place their steps.
+constexpr int AS_K = 8;
+struct StoreFirst {
+    uint2* p;
+    DMTZ_HD void operator()(i64 i, i64 cur, i64 t) const {
+        if (i < AS_K) p[i] = make_uint2((uint32_t)cur, (uint32_t)t);
+    }
 };
 struct StoreSteps {
     uint2* p;
@@ -728,25 +734,32 @@ __device__ i64 branch_tet(const uint64_t* fcf, ull j, const T* f, const StarCode
 
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_as_count(const uint64_t* fcf, ull nb, const T* f, const StarCode* cf,
-                                                    Dims d, Counts c, ull* cnt, ull* walk_ctr) {
+                                                    Dims d, Counts c, ull* cnt, uint2* first, ull* walk_ctr) {
     const ull j = (ull)blockIdx.x * BLOCK + threadIdx.x;
     if (j >= nb) return;
     const i64 q = branch_tet(fcf, j, f, cf, d, c);
     Repair<T, NoSink> R(f, cf, f, cf, d, c, nullptr, 0, 0, 0, NoSink{});
     bool ovf = false;
-    cnt[j] = (ull)R.as_fpath(q, c.N, ovf, CountSteps{});
+    cnt[j] = (ull)R.as_fpath(q, c.N, ovf, StoreFirst{first + j * AS_K});
     if (ovf) atomicAdd(walk_ctr, 1ull);
 }
 
+// Steps of branch j at off[j]: copied from the counting walk's first AS_K, or walked again.
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_as_fill(const uint64_t* fcf, ull nb, const T* f, const StarCode* cf,
-                                                   Dims d, Counts c, const ull* off, uint2* steps) {
+                                                   Dims d, Counts c, const ull* off, const uint2* first,
+                                                   uint2* steps) {
     const ull j = (ull)blockIdx.x * BLOCK + threadIdx.x;
     if (j >= nb) return;
+    const ull a = off[j], n = off[j + 1] - a;
+    if (n <= (ull)AS_K) {
+        for (ull k = 0; k < n; k++) steps[a + k] = first[j * AS_K + k];
+        return;
+    }
     const i64 q = branch_tet(fcf, j, f, cf, d, c);
     Repair<T, NoSink> R(f, cf, f, cf, d, c, nullptr, 0, 0, 0, NoSink{});
     bool ovf = false;
-    R.as_fpath(q, c.N, ovf, StoreSteps{steps + off[j]});
+    R.as_fpath(q, c.N, ovf, StoreSteps{steps + a});
 }
 
 // is_false (main_3d.cu:4437-4497) over the precomputed steps: the first step whose g partner
@@ -983,6 +996,8 @@ struct dmtz_ctx {
     uint64_t* d_rec = nullptr;    // 4 mismatch record lists of rec_cap entries
     uint64_t* d_fc = nullptr;     // 4 f-critical lists of fc_cap entries
     int64_t* d_path = nullptr;    // ds id/rec, as id/rec, wall id/rec (fc_cap each)
+    uint2* d_as_first = nullptr;     // first AS_K steps per branch from the counting walk
+    size_t as_first_cap = 0;
     uint32_t* d_wall_scr = nullptr;  // wall walks: per-thread visited sets and stacks
     size_t wall_scr_cap = 0;
     uint32_t* d_wall_ovf = nullptr;  // wall walks to repeat with more slots (two lists)
@@ -1057,6 +1072,9 @@ void free_paths(dmtz_ctx* ctx) {
     (void)hipFree(ctx->d_dse);
     (void)hipFree(ctx->d_ds_res);
     (void)hipFree(ctx->d_scan_tmp);
+    (void)hipFree(ctx->d_as_first);
+    ctx->d_as_first = nullptr;
+    ctx->as_first_cap = 0;
     (void)hipFree(ctx->d_wall_scr);
     (void)hipFree(ctx->d_wall_ovf);
     ctx->d_wall_scr = ctx->d_wall_ovf = nullptr;
@@ -1326,11 +1344,18 @@ struct Driver {
     }
 
     // Re-evaluate the g star codes of the closed stars of the vertices in the changed list (the g
-    // gradient of main_3d.cu:6864-6874 restricted to where it can change); the list's length
-    // stays on the device and est, the host's estimate of it, picks between re-evaluating every
-    // star and the closed stars of the list (both give the same codes).
+    // gradient of main_3d.cu:6864-6874 restricted to where it can change). est, the host's
+    // estimate of the list's length, sizes the launches; when it is large the exact length is
+    // read back to choose between re-evaluating every star and the closed stars of the list
+    // (both give the same codes).
     int refresh(ull est) {
         if (!est) return 0;
+        if (16 * est >= (ull)c.N) {  // many stars may change: decide on the exact count
+            int rc = sync_ctr(ctx);
+            if (rc) return rc;
+            est = ctx->h_ctr[K_CHG];
+            if (!est) return 0;
+        }
         if (16 * est >= (ull)c.N) {  // most stars change: re-evaluate all of them
             int rc = g_codes();
             if (rc) return rc;
@@ -1629,9 +1654,11 @@ struct Driver {
         if (rc) return rc;
         rc = grow_buf(ctx, ctx->d_as_off, ctx->as_off_cap, nb + 1);
         if (rc) return rc;
+        rc = grow_buf(ctx, ctx->d_as_first, ctx->as_first_cap, nb * AS_K);
+        if (rc) return rc;
         CK(hipMemsetAsync(ctx->d_as_cnt + nb, 0, sizeof(ull), ctx->stream));
         k_as_count<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(FL.l[2], nb, f, ctx->d_code_f, d, c, ctx->d_as_cnt,
-                                                               ctx->d_ctr + K_WALK);
+                                                               ctx->d_as_first, ctx->d_ctr + K_WALK);
         CK(hipGetLastError());
         size_t tmp = 0;
         CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, ctx->d_as_cnt, ctx->d_as_off, (int)(nb + 1), ctx->stream));
@@ -1647,7 +1674,7 @@ struct Driver {
         rc = grow_buf(ctx, ctx->d_as_steps, ctx->as_steps_cap, total);
         if (rc) return rc;
         k_as_fill<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(FL.l[2], nb, f, ctx->d_code_f, d, c, ctx->d_as_off,
-                                                              ctx->d_as_steps);
+                                                              ctx->d_as_first, ctx->d_as_steps);
         CK(hipGetLastError());
         return 0;
     }
