@@ -796,6 +796,53 @@ __global__ __launch_bounds__(BLOCK) void k_as_check(ull nb, const ull* off, cons
     res[2 * j + 1] = dv;
 }
 
+// Compact copies for the dirty test of the checks below: 32-bit branch offsets and the cube of
+// every step (needs fewer than 2^32 steps). Thread per branch.
+__global__ __launch_bounds__(BLOCK) void k_as_compact(ull nb, const ull* off, const uint2* steps, uint32_t* off32,
+                                                      uint32_t* cube) {
+    const ull j = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= nb) return;
+    off32[j] = (uint32_t)off[j];
+    if (j == nb - 1) off32[nb] = (uint32_t)off[nb];
+    for (ull s = off[j]; s < off[j + 1]; s++) cube[s] = steps[s].x / 6u;
+}
+
+// k_as_check over the compact copies: the dirty test reads 4 bytes per step, four steps per
+// round trip.
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_as_check32(ull nb, const uint32_t* off32, const uint32_t* cube,
+                                                      const uint2* steps, const T* g, const StarCode* cg, Dims d,
+                                                      Counts c, const uint32_t* cubes, int full, int64_t* res) {
+    const ull j = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= nb) return;
+    const uint32_t a = off32[j], b = off32[j + 1];
+    bool redo = full != 0;
+    auto dirty = [&](uint32_t q) { return ((cubes[q >> 5] >> (q & 31u)) & 1u) != 0; };
+    for (uint32_t s = a; s < b && !redo; s += 4) {
+        const uint32_t q0 = cube[s];
+        const uint32_t q1 = s + 1 < b ? cube[s + 1] : q0;
+        const uint32_t q2 = s + 2 < b ? cube[s + 2] : q0;
+        const uint32_t q3 = s + 3 < b ? cube[s + 3] : q0;
+        redo = dirty(q0) || dirty(q1) || dirty(q2) || dirty(q3);
+    }
+    if (!redo) return;
+    FieldGrad<T> G(g, cg, d, c);
+    i64 found = -1, dv = -1;
+    bool done = a >= b;
+    uint32_t s = a;
+    while (!done) {
+        const uint2 st = steps[s];
+        if (G.pt((i64)st.x) != (i64)st.y) {
+            found = (i64)steps[a].x;
+            dv = (i64)st.x;
+        }
+        s++;
+        done = found >= 0 || s >= b;
+    }
+    res[2 * j] = found;
+    res[2 * j + 1] = dv;
+}
+
 // Mark each dirty vertex and the (up to 8) cubes that have it as a corner. The list length is
 // read on the device (*n_ptr).
 __global__ __launch_bounds__(BLOCK) void k_cube_mark(const uint32_t* dirty, const ull* n_ptr, Dims d,
@@ -998,6 +1045,9 @@ struct dmtz_ctx {
     int64_t* d_path = nullptr;    // ds id/rec, as id/rec, wall id/rec (fc_cap each)
     uint2* d_as_first = nullptr;     // first AS_K steps per branch from the counting walk
     size_t as_first_cap = 0;
+    uint32_t* d_as_off32 = nullptr;  // 32-bit branch offsets and per-step cubes for the dirty test
+    uint32_t* d_as_cube = nullptr;
+    size_t as_off32_cap = 0, as_cube_cap = 0;
     uint32_t* d_wall_scr = nullptr;  // wall walks: per-thread visited sets and stacks
     size_t wall_scr_cap = 0;
     uint32_t* d_wall_ovf = nullptr;  // wall walks to repeat with more slots (two lists)
@@ -1075,6 +1125,10 @@ void free_paths(dmtz_ctx* ctx) {
     (void)hipFree(ctx->d_as_first);
     ctx->d_as_first = nullptr;
     ctx->as_first_cap = 0;
+    (void)hipFree(ctx->d_as_off32);
+    (void)hipFree(ctx->d_as_cube);
+    ctx->d_as_off32 = ctx->d_as_cube = nullptr;
+    ctx->as_off32_cap = ctx->as_cube_cap = 0;
     (void)hipFree(ctx->d_wall_scr);
     (void)hipFree(ctx->d_wall_ovf);
     ctx->d_wall_scr = ctx->d_wall_ovf = nullptr;
@@ -1628,7 +1682,7 @@ struct Driver {
 
     // Ascending geometry checks over precomputed f paths: in geometry mode, when tet and face
     // ids fit 32 bits.
-    bool as_ready = false, as_full = true;
+    bool as_ready = false, as_full = true, as_compact = false;
 
     bool as_precomputable() const {
         return fl.preserve_geometry == 1 && c.T < ((i64)1 << 32) && c.F < ((i64)1 << 32) && c.E < ((i64)1 << 32) &&
@@ -1676,6 +1730,16 @@ struct Driver {
         k_as_fill<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(FL.l[2], nb, f, ctx->d_code_f, d, c, ctx->d_as_off,
                                                               ctx->d_as_first, ctx->d_as_steps);
         CK(hipGetLastError());
+        as_compact = total < ((ull)1 << 32);
+        if (as_compact) {
+            rc = grow_buf(ctx, ctx->d_as_off32, ctx->as_off32_cap, nb + 1);
+            if (rc) return rc;
+            rc = grow_buf(ctx, ctx->d_as_cube, ctx->as_cube_cap, total + 1);
+            if (rc) return rc;
+            k_as_compact<<<nblocks(nb), BLOCK, 0, ctx->stream>>>(nb, ctx->d_as_off, ctx->d_as_steps, ctx->d_as_off32,
+                                                                  ctx->d_as_cube);
+            CK(hipGetLastError());
+        }
         return 0;
     }
 
@@ -1706,8 +1770,15 @@ struct Driver {
             if (rc) return rc;
         }
         const ull nb = 2 * nfcf;
-        k_as_check<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(nb, ctx->d_as_off, ctx->d_as_steps, g, ctx->d_code_g,
-                                                               d, c, ctx->d_cubes, as_full ? 1 : 0, ctx->d_walk);
+        if (!nb) return 0;
+        if (as_compact)
+            k_as_check32<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(nb, ctx->d_as_off32, ctx->d_as_cube,
+                                                                     ctx->d_as_steps, g, ctx->d_code_g, d, c,
+                                                                     ctx->d_cubes, as_full ? 1 : 0, ctx->d_walk);
+        else
+            k_as_check<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(nb, ctx->d_as_off, ctx->d_as_steps, g,
+                                                                   ctx->d_code_g, d, c, ctx->d_cubes, as_full ? 1 : 0,
+                                                                   ctx->d_walk);
         CK(hipGetLastError());
         return 0;
     }
