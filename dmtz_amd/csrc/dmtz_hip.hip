@@ -346,6 +346,58 @@ struct TargetSink {
     }
 };
 
+// Branch classes of the repair rules: the first split of each rule, i.e. the cell's partner in f
+// (an edge with its vertex or a triangle; a triangle with an edge or a tet) or, when f leaves it
+// critical, its partner in g. The class only orders the records, and the edit set does not
+// depend on the order; but records of one class take the same top-level branch, so a wave of
+// them runs one branch body instead of all of them. It is read from the star codes: the
+// record's vertex owns the cell in g, the f-critical masks tell whether f pairs it, and the cell's
+// owner in f (its highest vertex in f) holds f's partner.
+constexpr int FIX_CLASS_BITS = 3;  // classes 1..6
+
+template <class T, int TYPE>
+__global__ __launch_bounds__(BLOCK) void k_fix_class(const uint64_t* recs, ull n, const T* f, const StarCode* cf,
+                                                     const StarCode* cg, const CritMask* critf, Dims d, Counts c,
+                                                     uint8_t* key) {
+    const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t r = recs[i];
+    const i64 v = (i64)(r >> 7);
+    const int lc = (int)(r & 127u);
+    const CritMask cm = critf[v];
+    const bool fcrit = lc < 51 ? ((cm.lo >> lc) & 1ull) != 0 : ((cm.hi >> (lc - 51)) & 1u) != 0;
+    int k;
+    if (TYPE == 3) {
+        k = fcrit ? 2 : 1;
+    } else if (fcrit) {  // branch on g's partner, from the g owner's code
+        if (TYPE == 1) {
+            const int nib = code_edge(cg[v], lc - LC_EDGE);
+            k = nib == 14 ? 4 : (nib < 14 ? 5 : 6);
+        } else {
+            const int cc = code_tri(cg[v], lc - LC_TRI);
+            k = cc < 2 ? 4 : (cc < 4 ? 5 : 6);
+        }
+    } else {  // branch on f's partner, from the f owner's code
+        const FieldGrad<T> F(f, cf, d, c);
+        i64 m = v;
+        int lm = lc;
+        cell_at_other_vertices(d, v, lc, [&](i64 w, int lw) {
+            if (F.above(w, m)) {
+                m = w;
+                lm = lw;
+            }
+        });
+        if (TYPE == 1) {
+            const int nib = code_edge(cf[m], lm - LC_EDGE);
+            k = nib == 14 ? 1 : (nib < 14 ? 2 : 3);
+        } else {
+            const int cc = code_tri(cf[m], lm - LC_TRI);
+            k = cc < 2 ? 1 : (cc < 4 ? 2 : 3);
+        }
+    }
+    key[i] = (uint8_t)k;
+}
+
 // One kernel per cell type, so each holds only its own repair rule.
 template <class T, int TYPE>
 __global__ __launch_bounds__(BLOCK) void k_fix_cp(const uint64_t* recs, ull n, const T* f, const StarCode* cf,
@@ -1043,6 +1095,10 @@ struct dmtz_ctx {
     uint64_t* d_rec = nullptr;    // 4 mismatch record lists of rec_cap entries
     uint64_t* d_fc = nullptr;     // 4 f-critical lists of fc_cap entries
     int64_t* d_path = nullptr;    // ds id/rec, as id/rec, wall id/rec (fc_cap each)
+    uint8_t* d_fix_key = nullptr;    // branch class per mismatch record, and the sorted classes
+    uint8_t* d_fix_key2 = nullptr;
+    uint64_t* d_fix_sorted = nullptr;  // records grouped by branch class
+    size_t fix_key_cap = 0, fix_key2_cap = 0, fix_sorted_cap = 0;
     uint2* d_as_first = nullptr;     // first AS_K steps per branch from the counting walk
     size_t as_first_cap = 0;
     uint32_t* d_as_off32 = nullptr;  // 32-bit branch offsets and per-step cubes for the dirty test
@@ -1369,8 +1425,33 @@ struct Driver {
             if (counts[ty] == 0) continue;
             if ((ty == 1 || ty == 2) && fl.preserve_saddles != 1) continue;
             nrec += counts[ty];
+            const uint64_t* recs = L.l[ty];
+            if (ty != 0) {  // group the records by branch class first
+                rc = grow_buf(ctx, ctx->d_fix_key, ctx->fix_key_cap, counts[ty]);
+                if (rc) return rc;
+                rc = grow_buf(ctx, ctx->d_fix_sorted, ctx->fix_sorted_cap, counts[ty]);
+                if (rc) return rc;
+                rc = grow_buf(ctx, ctx->d_fix_key2, ctx->fix_key2_cap, counts[ty]);
+                if (rc) return rc;
+                auto cls = ty == 1 ? k_fix_class<T, 1> : ty == 2 ? k_fix_class<T, 2> : k_fix_class<T, 3>;
+                cls<<<nblocks(counts[ty]), BLOCK, 0, ctx->stream>>>(recs, counts[ty], f, ctx->d_code_f, ctx->d_code_g,
+                                                                    ctx->d_critf, d, c, ctx->d_fix_key);
+                CK(hipGetLastError());
+                size_t tmp = 0;
+                CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, ctx->d_fix_key, ctx->d_fix_key2, recs,
+                                                      ctx->d_fix_sorted, (int)counts[ty], 0, FIX_CLASS_BITS,
+                                                      ctx->stream));
+                uint8_t* tp = (uint8_t*)ctx->d_scan_tmp;
+                rc = grow_buf(ctx, tp, ctx->scan_tmp_cap, tmp);
+                ctx->d_scan_tmp = tp;
+                if (rc) return rc;
+                CK(hipcub::DeviceRadixSort::SortPairs(ctx->d_scan_tmp, tmp, ctx->d_fix_key, ctx->d_fix_key2, recs,
+                                                      ctx->d_fix_sorted, (int)counts[ty], 0, FIX_CLASS_BITS,
+                                                      ctx->stream));
+                recs = ctx->d_fix_sorted;
+            }
             auto kern = ty == 0 ? k_fix_cp<T, 0> : ty == 1 ? k_fix_cp<T, 1> : ty == 2 ? k_fix_cp<T, 2> : k_fix_cp<T, 3>;
-            kern<<<nblocks(counts[ty]), BLOCK, 0, ctx->stream>>>(L.l[ty], counts[ty], f, ctx->d_code_f, g,
+            kern<<<nblocks(counts[ty]), BLOCK, 0, ctx->stream>>>(recs, counts[ty], f, ctx->d_code_f, g,
                                                                  ctx->d_code_g, d, c, ctx->d_counter, eps, range,
                                                                  fl.edit_type, sink());
             CK(hipGetLastError());
@@ -2460,6 +2541,9 @@ void dmtz_ctx_destroy(dmtz_ctx* ctx) {
     free_fc(ctx);
     free_paths(ctx);
     (void)hipFree(ctx->d_ctr);
+    (void)hipFree(ctx->d_fix_key);
+    (void)hipFree(ctx->d_fix_key2);
+    (void)hipFree(ctx->d_fix_sorted);
     (void)hipHostFree(ctx->h_ctr);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
