@@ -859,14 +859,25 @@ __global__ __launch_bounds__(BLOCK) void k_as_compact(ull nb, const ull* off, co
     for (ull s = off[j]; s < off[j + 1]; s++) cube[s] = steps[s].x / 6u;
 }
 
-// k_as_check over the compact copies: the dirty test reads 4 bytes per step, four steps per
-// round trip.
-template <class T>
-__global__ __launch_bounds__(BLOCK) void k_as_check32(ull nb, const uint32_t* off32, const uint32_t* cube,
-                                                      const uint2* steps, const T* g, const StarCode* cg, Dims d,
-                                                      Counts c, const uint32_t* cubes, int full, int64_t* res) {
+// Branch order of the checks: branches sorted by step count (capped at 255), so that a wave's
+// walks have similar lengths. Keys and identity order for the sort.
+__global__ __launch_bounds__(BLOCK) void k_as_len_keys(ull nb, const ull* cnt, uint8_t* key, uint32_t* idx) {
     const ull j = (ull)blockIdx.x * BLOCK + threadIdx.x;
     if (j >= nb) return;
+    key[j] = (uint8_t)(cnt[j] < 255 ? cnt[j] : 255);
+    idx[j] = (uint32_t)j;
+}
+
+// k_as_check over the compact copies: the dirty test reads 4 bytes per step, four steps per
+// round trip. Thread t checks branch order[t]; results stay indexed by branch.
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_as_check32(ull nb, const uint32_t* order, const uint32_t* off32,
+                                                      const uint32_t* cube, const uint2* steps, const T* g,
+                                                      const StarCode* cg, Dims d, Counts c, const uint32_t* cubes,
+                                                      int full, int64_t* res) {
+    const ull t = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= nb) return;
+    const ull j = order[t];
     const uint32_t a = off32[j], b = off32[j + 1];
     bool redo = full != 0;
     auto dirty = [&](uint32_t q) { return ((cubes[q >> 5] >> (q & 31u)) & 1u) != 0; };
@@ -1103,7 +1114,8 @@ struct dmtz_ctx {
     size_t as_first_cap = 0;
     uint32_t* d_as_off32 = nullptr;  // 32-bit branch offsets and per-step cubes for the dirty test
     uint32_t* d_as_cube = nullptr;
-    size_t as_off32_cap = 0, as_cube_cap = 0;
+    uint32_t* d_as_order = nullptr;  // branches by step count (first nb), sort input (second nb)
+    size_t as_off32_cap = 0, as_cube_cap = 0, as_order_cap = 0;
     uint32_t* d_wall_scr = nullptr;  // wall walks: per-thread visited sets and stacks
     size_t wall_scr_cap = 0;
     uint32_t* d_wall_ovf = nullptr;  // wall walks to repeat with more slots (two lists)
@@ -1183,8 +1195,9 @@ void free_paths(dmtz_ctx* ctx) {
     ctx->as_first_cap = 0;
     (void)hipFree(ctx->d_as_off32);
     (void)hipFree(ctx->d_as_cube);
-    ctx->d_as_off32 = ctx->d_as_cube = nullptr;
-    ctx->as_off32_cap = ctx->as_cube_cap = 0;
+    (void)hipFree(ctx->d_as_order);
+    ctx->d_as_off32 = ctx->d_as_cube = ctx->d_as_order = nullptr;
+    ctx->as_off32_cap = ctx->as_cube_cap = ctx->as_order_cap = 0;
     (void)hipFree(ctx->d_wall_scr);
     (void)hipFree(ctx->d_wall_ovf);
     ctx->d_wall_scr = ctx->d_wall_ovf = nullptr;
@@ -1811,7 +1824,7 @@ struct Driver {
         k_as_fill<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(FL.l[2], nb, f, ctx->d_code_f, d, c, ctx->d_as_off,
                                                               ctx->d_as_first, ctx->d_as_steps);
         CK(hipGetLastError());
-        as_compact = total < ((ull)1 << 32);
+        as_compact = total < ((ull)1 << 32) && nb < ((ull)1 << 31);
         if (as_compact) {
             rc = grow_buf(ctx, ctx->d_as_off32, ctx->as_off32_cap, nb + 1);
             if (rc) return rc;
@@ -1820,6 +1833,25 @@ struct Driver {
             k_as_compact<<<nblocks(nb), BLOCK, 0, ctx->stream>>>(nb, ctx->d_as_off, ctx->d_as_steps, ctx->d_as_off32,
                                                                   ctx->d_as_cube);
             CK(hipGetLastError());
+            // branch order by length (keys and the identity order in the scratch of the rule sort)
+            rc = grow_buf(ctx, ctx->d_as_order, ctx->as_order_cap, 2 * nb);
+            if (rc) return rc;
+            rc = grow_buf(ctx, ctx->d_fix_key, ctx->fix_key_cap, nb);
+            if (rc) return rc;
+            rc = grow_buf(ctx, ctx->d_fix_key2, ctx->fix_key2_cap, nb);
+            if (rc) return rc;
+            k_as_len_keys<<<nblocks(nb), BLOCK, 0, ctx->stream>>>(nb, ctx->d_as_cnt, ctx->d_fix_key,
+                                                                   ctx->d_as_order + nb);
+            CK(hipGetLastError());
+            size_t tmp2 = 0;
+            CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp2, ctx->d_fix_key, ctx->d_fix_key2, ctx->d_as_order + nb,
+                                                  ctx->d_as_order, (int)nb, 0, 8, ctx->stream));
+            uint8_t* tp2 = (uint8_t*)ctx->d_scan_tmp;
+            rc = grow_buf(ctx, tp2, ctx->scan_tmp_cap, tmp2);
+            ctx->d_scan_tmp = tp2;
+            if (rc) return rc;
+            CK(hipcub::DeviceRadixSort::SortPairs(ctx->d_scan_tmp, tmp2, ctx->d_fix_key, ctx->d_fix_key2,
+                                                  ctx->d_as_order + nb, ctx->d_as_order, (int)nb, 0, 8, ctx->stream));
         }
         return 0;
     }
@@ -1853,7 +1885,8 @@ struct Driver {
         const ull nb = 2 * nfcf;
         if (!nb) return 0;
         if (as_compact)
-            k_as_check32<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(nb, ctx->d_as_off32, ctx->d_as_cube,
+            k_as_check32<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(nb, ctx->d_as_order, ctx->d_as_off32,
+                                                                     ctx->d_as_cube,
                                                                      ctx->d_as_steps, g, ctx->d_code_g, d, c,
                                                                      ctx->d_cubes, as_full ? 1 : 0, ctx->d_walk);
         else
