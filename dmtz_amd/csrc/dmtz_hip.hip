@@ -348,17 +348,18 @@ struct TargetSink {
 
 // Branch classes of the repair rules: the first split of each rule, i.e. the cell's partner in f
 // (an edge with its vertex or a triangle; a triangle with an edge or a tet) or, when f leaves it
-// critical, its partner in g. The class only orders the records, and the edit set does not
-// depend on the order; but records of one class take the same top-level branch, so a wave of
-// them runs one branch body instead of all of them. It is read from the star codes: the
-// record's vertex owns the cell in g, the f-critical masks tell whether f pairs it, and the cell's
-// owner in f (its highest vertex in f) holds f's partner.
-constexpr int FIX_CLASS_BITS = 3;  // classes 1..6
+// critical, its partner in g, and for the 2-saddle and maximum rules the next split (how the
+// partner's remaining vertex lies against the cell in f or g). The class only orders the records,
+// and the edit set does not depend on the order; but records of one class take the same
+// branches, so a wave of them runs one branch body instead of all of them. It is read from the
+// star codes: the record's vertex owns the cell in g, the f-critical masks tell whether f pairs
+// it, and the cell's owner in f (its highest vertex in f) holds f's partner.
+constexpr int FIX_CLASS_BITS = 4;  // classes 1..12
 
 template <class T, int TYPE>
 __global__ __launch_bounds__(BLOCK) void k_fix_class(const uint64_t* recs, ull n, const T* f, const StarCode* cf,
-                                                     const StarCode* cg, const CritMask* critf, Dims d, Counts c,
-                                                     uint8_t* key) {
+                                                     const T* g, const StarCode* cg, const CritMask* critf, Dims d,
+                                                     Counts c, uint16_t* key) {
     const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     const uint64_t r = recs[i];
@@ -366,36 +367,101 @@ __global__ __launch_bounds__(BLOCK) void k_fix_class(const uint64_t* recs, ull n
     const int lc = (int)(r & 127u);
     const CritMask cm = critf[v];
     const bool fcrit = lc < 51 ? ((cm.lo >> lc) & 1ull) != 0 : ((cm.hi >> (lc - 51)) & 1u) != 0;
-    int k;
-    if (TYPE == 3) {
-        k = fcrit ? 2 : 1;
-    } else if (fcrit) {  // branch on g's partner, from the g owner's code
-        if (TYPE == 1) {
-            const int nib = code_edge(cg[v], lc - LC_EDGE);
-            k = nib == 14 ? 4 : (nib < 14 ? 5 : 6);
-        } else {
-            const int cc = code_tri(cg[v], lc - LC_TRI);
-            k = cc < 2 ? 4 : (cc < 4 ? 5 : 6);
-        }
-    } else {  // branch on f's partner, from the f owner's code
-        const FieldGrad<T> F(f, cf, d, c);
-        i64 m = v;
-        int lm = lc;
+    const FieldGrad<T> F(f, cf, d, c), G(g, cg, d, c);
+    auto nbv = [&](i64 u, int slot) {
+        i64 x, y, z;
+        vxyz(d, u, x, y, z);
+        return vidx(d, x + DIR[slot][0], y + DIR[slot][1], z + DIR[slot][2]);
+    };
+    // the cell's owner in f and its local index there
+    auto f_owner = [&](i64& m, int& lm) {
+        m = v;
+        lm = lc;
         cell_at_other_vertices(d, v, lc, [&](i64 w, int lw) {
             if (F.above(w, m)) {
                 m = w;
                 lm = lw;
             }
         });
-        if (TYPE == 1) {
-            const int nib = code_edge(cf[m], lm - LC_EDGE);
-            k = nib == 14 ? 1 : (nib < 14 ? 2 : 3);
+    };
+    int k = 0;
+    if (TYPE == 1) {  // fix_saddle main_3d.cu:2588-3037: partner in f, else in g
+        const i64 w = nbv(v, lc - LC_EDGE);
+        const i64 v1 = v > w ? v : w, v2 = v > w ? w : v;  // endpoints by index
+        if (fcrit) {
+            const int nib = code_edge(cg[v], lc - LC_EDGE);
+            if (nib < 14) {  // triangle with v3: are the endpoints ordered alike against v3 (:2886-2890)
+                const i64 v3 = nbv(v, nib);
+                k = G.above(v2, v3) == G.above(v1, v3) ? 6 : 7;
+            } else {
+                k = nib == 14 ? 5 : 8;
+            }
         } else {
-            const int cc = code_tri(cf[m], lm - LC_TRI);
-            k = cc < 2 ? 1 : (cc < 4 ? 2 : 3);
+            i64 m;
+            int lm;
+            f_owner(m, lm);
+            const int nib = code_edge(cf[m], lm - LC_EDGE);
+            if (nib < 14) {  // triangle with v3: is v3 below both endpoints in g or in f (:2643-2700)
+                const i64 v3 = nbv(m, nib);
+                const bool cheap = (G.above(v1, v3) && G.above(v2, v3)) || (F.above(v1, v3) && F.above(v2, v3));
+                k = cheap ? 2 : 3;
+            } else {
+                k = nib == 14 ? 1 : 4;
+            }
+        }
+    } else if (TYPE == 2) {  // fix_2saddle main_3d.cu:3039-3983
+        if (fcrit) {  // partner in g, from the g owner v
+            const int t = lc - LC_TRI;
+            const int cc = code_tri(cg[v], t);
+            if (cc < 2) {  // edge (v, n): its partner in f (:3488-3610)
+                const int sl = cc == 0 ? ST.tri_i[t] : ST.tri_j[t];
+                const i64 w = nbv(v, sl);
+                const int nib = F.above(w, v) ? code_edge(cf[w], sl ^ 1) : code_edge(cf[v], sl);
+                k = nib < 14 ? 7 : (nib == 14 ? 8 : 9);
+            } else if (cc < 4) {  // tet through n4: is n4 below the face in g (:3613-3980)
+                const i64 a = nbv(v, ST.tri_i[t]), b = nbv(v, ST.tri_j[t]), v4 = nbv(v, ST.tri_k[t][cc - 2]);
+                k = (G.above(a, v4) && G.above(b, v4)) ? 10 : 11;
+            } else {
+                k = 12;
+            }
+        } else {  // partner in f, from the f owner m
+            i64 m;
+            int lm;
+            f_owner(m, lm);
+            const int t = lm - LC_TRI;
+            const int cc = code_tri(cf[m], t);
+            const i64 a = nbv(m, ST.tri_i[t]), b = nbv(m, ST.tri_j[t]);
+            if (cc < 2) {  // edge (m, e) and the third vertex u (:3057-3223)
+                const i64 e = cc == 0 ? a : b, u = cc == 0 ? b : a;
+                const i64 v1 = m > e ? m : e, v2 = m > e ? e : m;
+                const bool v1e = G.above(v2, u), v2e = G.above(v1, u);
+                const bool o1 = F.above(v2, u), o2 = F.above(v1, u);
+                k = (v1e && v2e) ? 1 : ((o1 != o2 && v1e != v2e) ? 2 : 3);
+            } else if (cc < 4) {  // tet through n4: is n4 below the face in f (:3226-3477)
+                const i64 v4 = nbv(m, ST.tri_k[t][cc - 2]);
+                k = (F.above(a, v4) && F.above(b, v4)) ? 4 : 5;
+            } else {
+                k = 6;
+            }
+        }
+    } else if (TYPE == 3) {  // fix_maximum main_3d.cu:3985-4303: is the left-out vertex below the face
+        i64 m = v;
+        int lm = lc;
+        if (!fcrit) f_owner(m, lm);
+        const int q = lm - LC_TET;
+        const int cc = code_tet(fcrit ? cg[m] : cf[m], q);
+        if (cc < 3) {
+            const int s0 = ST.tet_s[q][0], s1 = ST.tet_s[q][1], s2 = ST.tet_s[q][2];
+            const int out = cc == 0 ? s0 : (cc == 1 ? s1 : s2);
+            const i64 w = nbv(m, out);
+            const i64 x = nbv(m, cc == 0 ? s1 : s0), y = nbv(m, cc == 2 ? s1 : s2);
+            const FieldGrad<T>& X = fcrit ? G : F;
+            k = (fcrit ? 3 : 1) + ((X.above(x, w) && X.above(y, w)) ? 0 : 1);
+        } else {
+            k = fcrit ? 5 : 6;
         }
     }
-    key[i] = (uint8_t)k;
+    key[i] = (uint16_t)k;
 }
 
 // One kernel per cell type, so each holds only its own repair rule.
@@ -861,10 +927,10 @@ __global__ __launch_bounds__(BLOCK) void k_as_compact(ull nb, const ull* off, co
 
 // Branch order of the checks: branches sorted by step count (capped at 255), so that a wave's
 // walks have similar lengths. Keys and identity order for the sort.
-__global__ __launch_bounds__(BLOCK) void k_as_len_keys(ull nb, const ull* cnt, uint8_t* key, uint32_t* idx) {
+__global__ __launch_bounds__(BLOCK) void k_as_len_keys(ull nb, const ull* cnt, uint16_t* key, uint32_t* idx) {
     const ull j = (ull)blockIdx.x * BLOCK + threadIdx.x;
     if (j >= nb) return;
-    key[j] = (uint8_t)(cnt[j] < 255 ? cnt[j] : 255);
+    key[j] = (uint16_t)(cnt[j] < 255 ? cnt[j] : 255);
     idx[j] = (uint32_t)j;
 }
 
@@ -1106,8 +1172,8 @@ struct dmtz_ctx {
     uint64_t* d_rec = nullptr;    // 4 mismatch record lists of rec_cap entries
     uint64_t* d_fc = nullptr;     // 4 f-critical lists of fc_cap entries
     int64_t* d_path = nullptr;    // ds id/rec, as id/rec, wall id/rec (fc_cap each)
-    uint8_t* d_fix_key = nullptr;    // branch class per mismatch record, and the sorted classes
-    uint8_t* d_fix_key2 = nullptr;
+    uint16_t* d_fix_key = nullptr;   // sort keys (branch class per mismatch record, path length per
+    uint16_t* d_fix_key2 = nullptr;  // ascending branch) and their sorted copy
     uint64_t* d_fix_sorted = nullptr;  // records grouped by branch class
     size_t fix_key_cap = 0, fix_key2_cap = 0, fix_sorted_cap = 0;
     uint2* d_as_first = nullptr;     // first AS_K steps per branch from the counting walk
@@ -1447,8 +1513,8 @@ struct Driver {
                 rc = grow_buf(ctx, ctx->d_fix_key2, ctx->fix_key2_cap, counts[ty]);
                 if (rc) return rc;
                 auto cls = ty == 1 ? k_fix_class<T, 1> : ty == 2 ? k_fix_class<T, 2> : k_fix_class<T, 3>;
-                cls<<<nblocks(counts[ty]), BLOCK, 0, ctx->stream>>>(recs, counts[ty], f, ctx->d_code_f, ctx->d_code_g,
-                                                                    ctx->d_critf, d, c, ctx->d_fix_key);
+                cls<<<nblocks(counts[ty]), BLOCK, 0, ctx->stream>>>(recs, counts[ty], f, ctx->d_code_f, g,
+                                                                    ctx->d_code_g, ctx->d_critf, d, c, ctx->d_fix_key);
                 CK(hipGetLastError());
                 size_t tmp = 0;
                 CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, ctx->d_fix_key, ctx->d_fix_key2, recs,
