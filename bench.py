@@ -65,9 +65,10 @@ def main():
                     help="time the batched form instead: K fields with sides drawn from 32..192, half "
                          "Gaussian mixtures and half fBm, eps log-uniform in [1e-4, 1e-2]*range, corrected "
                          "by one dmtz_correct_batch call per step")
-    ap.add_argument("--learned", default="", metavar="NPY",
+    ap.add_argument("--learned", default=os.path.join(ROOT, "dmtz_amd", "net_weights.npy"), metavar="NPY",
                     help="also time the learned warm start with these network parameters (from "
-                         "python -m dmtz_amd.learned_train) on the same field, and report it beside the plain form")
+                         "python -m dmtz_amd.learned_train) on the same field, and report it beside the plain "
+                         "form; 'none' skips it")
     ap.add_argument("--finish", type=int, default=1,
                     help="1: resolve stalls (fixed points where the reference loops forever) by snapping "
                          "growing boxes around the stalled edit set")
@@ -168,7 +169,7 @@ def main():
                           basis="sweeps * (f and g once) + 8 B per mismatch record, over the whole correction")
 
     learned = None
-    if args.learned:
+    if args.learned and args.learned != "none" and os.path.exists(args.learned):
         learned = bench_learned(args, ctx, f_d, g0_d, g_d, eps, rng, flags, value)
 
     cpu = None
