@@ -51,6 +51,7 @@ constexpr int NL = 4;
 constexpr int CIN0 = 8;                  // used input channels
 constexpr int COUT_LAST = 2;             // used output channels
 constexpr int BLOCK = 256;
+constexpr int TXC = 18;                  // a 16-voxel run and its halo along x
 
 static_assert(LP == WSZ + CH, "layer parameter layout");
 
@@ -128,93 +129,125 @@ __global__ __launch_bounds__(BLOCK) void k_features(const float* g, Dims d, i64 
 // MODE 0: Y = ReLU(conv(X, W) + b) as bf16; MODE 1: Y = conv(X, W) + b, channels 0..1 as fp32;
 // MODE 2: dX(v) = ReLU'(M(v)) * sum_tap W[tap]^T dY(v - off(tap)) as bf16 (X is dY, M the
 // layer's input, whose positive entries are those where the ReLU below passed its input).
+// Waves stride over 16-voxel runs (a block reads the weights once for several runs). Per run a
+// wave stages the run's 9 rows of X (y - 1..y + 1, z - 1..z + 1, over x0 - 1..x0 + 16,
+// channel-last as in memory) in its own part of LDS with 16-byte loads, then reads each tap's A
+// fragment (4 channels of one voxel) from there.
 template <int MODE>
 __global__ __launch_bounds__(BLOCK) void k_conv(const uint16_t* __restrict__ X, const uint16_t* __restrict__ W,
                                                 const float* __restrict__ bias, const uint16_t* __restrict__ M,
                                                 int nx, int ny, int nz, uint16_t* Yb, float* Yf) {
     __shared__ uint16_t ws[WSZ];
+    __shared__ __attribute__((aligned(16))) uint16_t xt[BLOCK / 64][9][TXC][CH];
     for (int k = threadIdx.x; k < WSZ; k += BLOCK) ws[k] = W[k];
     __syncthreads();
-    const int lane = threadIdx.x & 63, i = lane & 15, q = lane >> 4;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, i = lane & 15, q = lane >> 4;
     const int nxs = (nx + 15) >> 4;
     const long S = (long)nxs * ny * nz;
-    const long w = (long)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
-    if (w >= S) return;  // after the block's only barrier
-    int x0, y, z;
-    run_origin(w, nxs, ny, x0, y, z);
-    const int x = x0 + i;
     const int sgn = MODE == 2 ? -1 : 1;
-    float4_t acc = {0.f, 0.f, 0.f, 0.f};
-    for (int tap = 0; tap < TAPS; tap++) {
-        int dx, dy, dz;
-        tap_offset(tap, dx, dy, dz);
-        const int xx = x + sgn * dx, yy = y + sgn * dy, zz = z + sgn * dz;
-        short4_t a = {0, 0, 0, 0};
-        if (x < nx && xx >= 0 && xx < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz)
-            a = *(const short4_t*)(X + (((long)zz * ny + yy) * nx + xx) * CH + 4 * q);
-        short4_t b;
-        for (int r = 0; r < 4; r++)
-            b[r] = (short)(MODE == 2 ? ws[tap * CH * CH + i * CH + 4 * q + r]      // W[tap][ci = i][co = 4q + r]
-                                     : ws[tap * CH * CH + (4 * q + r) * CH + i]);  // W[tap][ci = 4q + r][co = i]
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-    }
-    for (int r = 0; r < 4; r++) {
-        const int vx = x0 + 4 * q + r;
-        if (vx >= nx) continue;
-        const long v = ((long)z * ny + y) * nx + vx;
-        const float val = acc[r];
-        if (MODE == 0) {
-            const float o = val + bias[i];
-            Yb[v * CH + i] = f2bf(o > 0.f ? o : 0.f);
-        } else if (MODE == 1) {
-            if (i < COUT_LAST) Yf[v * COUT_LAST + i] = val + bias[i];
-        } else {
-            Yb[v * CH + i] = bf2f(M[v * CH + i]) > 0.f ? f2bf(val) : (uint16_t)0;
+    const float bi = MODE == 2 ? 0.f : bias[i];
+    // runs of one wave are ordered by wave barriers only (waves of a block take different numbers)
+    for (long w = (long)blockIdx.x * (BLOCK / 64) + wv; w < S; w += (long)gridDim.x * (BLOCK / 64)) {
+        int x0, y, z;
+        run_origin(w, nxs, ny, x0, y, z);
+        for (int e = lane; e < 9 * TXC * 2; e += 64) {  // 16-byte chunks (8 channels)
+            const int row = e / (TXC * 2), rem = e % (TXC * 2), xi = rem >> 1, half = rem & 1;
+            const int yy = y + row % 3 - 1, zz = z + row / 3 - 1, xx = x0 - 1 + xi;
+            uint4 val = make_uint4(0u, 0u, 0u, 0u);
+            if (xx >= 0 && xx < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz)
+                val = *(const uint4*)(X + (((long)zz * ny + yy) * nx + xx) * CH + 8 * half);
+            *(uint4*)&xt[wv][row][xi][8 * half] = val;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const int x = x0 + i;
+        float4_t acc = {0.f, 0.f, 0.f, 0.f};
+        for (int tap = 0; tap < TAPS; tap++) {
+            int dx, dy, dz;
+            tap_offset(tap, dx, dy, dz);
+            const int row = (sgn * dz + 1) * 3 + (sgn * dy + 1);
+            short4_t a = {0, 0, 0, 0};
+            if (x < nx) a = *(const short4_t*)&xt[wv][row][i + sgn * dx + 1][4 * q];
+            short4_t b;
+            for (int r = 0; r < 4; r++)
+                b[r] = (short)(MODE == 2 ? ws[tap * CH * CH + i * CH + 4 * q + r]      // W[tap][ci = i][co = 4q + r]
+                                         : ws[tap * CH * CH + (4 * q + r) * CH + i]);  // W[tap][ci = 4q + r][co = i]
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        for (int r = 0; r < 4; r++) {
+            const int vx = x0 + 4 * q + r;
+            if (vx >= nx) continue;
+            const long v = ((long)z * ny + y) * nx + vx;
+            const float val = acc[r];
+            if (MODE == 0) {
+                const float o = val + bi;
+                Yb[v * CH + i] = f2bf(o > 0.f ? o : 0.f);
+            } else if (MODE == 1) {
+                if (i < COUT_LAST) Yf[v * COUT_LAST + i] = val + bi;
+            } else {
+                Yb[v * CH + i] = bf2f(M[v * CH + i]) > 0.f ? f2bf(val) : (uint16_t)0;
+            }
         }
     }
 }
 
 // dW[tap][ci][co] += sum_v X(v + off(tap))[ci] * dY(v)[co], db[co] += sum_v dY(v)[co]. Each wave
-// strides over 16-voxel runs; per run the B operand (dY of the run) is loaded once and the 27
-// taps accumulate into 27 fragments; the block sums its waves in LDS before one global atomic
-// per entry.
+// strides over 16-voxel runs. Per run it stages, in its own part of LDS and channel-major, the 9
+// rows (y - 1..y + 1, z - 1..z + 1) of X over x0 - 1..x0 + 16 and the run's dY, loading 16-byte
+// chunks; the 27 taps then read their A fragments (4 voxels of one channel) from LDS and
+// accumulate into 27 fragments. The block sums its waves in LDS before one global atomic per
+// entry. Runs of one wave are ordered by wave barriers only (waves of a block take different
+// numbers of runs).
 __global__ __launch_bounds__(BLOCK) void k_wgrad(const uint16_t* __restrict__ X, const uint16_t* __restrict__ dY,
                                                  int nx, int ny, int nz, float* dW, float* db) {
     __shared__ float wacc[WSZ];
     __shared__ float bacc[CH];
+    __shared__ uint16_t xt[BLOCK / 64][9][CH][TXC];
+    __shared__ __attribute__((aligned(16))) uint16_t yt[BLOCK / 64][CH][16];
     for (int k = threadIdx.x; k < WSZ; k += BLOCK) wacc[k] = 0.f;
     if (threadIdx.x < CH) bacc[threadIdx.x] = 0.f;
     __syncthreads();
-    const int lane = threadIdx.x & 63, i = lane & 15, q = lane >> 4;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, i = lane & 15, q = lane >> 4;
     const int nxs = (nx + 15) >> 4;
     const long S = (long)nxs * ny * nz;
     float4_t acc[TAPS];  // fully unrolled loops keep the 27 fragments in registers
 #pragma unroll
     for (int t = 0; t < TAPS; t++) acc[t] = (float4_t){0.f, 0.f, 0.f, 0.f};
     float bsum = 0.f;
-    for (long w = (long)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6); w < S; w += (long)gridDim.x * (BLOCK / 64)) {
+    for (long w = (long)blockIdx.x * (BLOCK / 64) + wv; w < S; w += (long)gridDim.x * (BLOCK / 64)) {
         int x0, y, z;
         run_origin(w, nxs, ny, x0, y, z);
-        short4_t b;
-        for (int r = 0; r < 4; r++) {
-            const int vx = x0 + 4 * q + r;
-            const uint16_t h = vx < nx ? dY[(((long)z * ny + y) * nx + vx) * CH + i] : (uint16_t)0;
-            b[r] = (short)h;
-            bsum += bf2f(h);
+        for (int e = lane; e < 9 * TXC * 2; e += 64) {  // X rows: 16-byte chunks (8 channels)
+            const int row = e / (TXC * 2), rem = e % (TXC * 2), xi = rem >> 1, half = rem & 1;
+            const int yy = y + row % 3 - 1, zz = z + row / 3 - 1, xx = x0 - 1 + xi;
+            uint4 val = make_uint4(0u, 0u, 0u, 0u);
+            if (xx >= 0 && xx < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz)
+                val = *(const uint4*)(X + (((long)zz * ny + yy) * nx + xx) * CH + 8 * half);
+            const uint32_t wd[4] = {val.x, val.y, val.z, val.w};
+            for (int c = 0; c < 8; c++) xt[wv][row][8 * half + c][xi] = (uint16_t)(wd[c >> 1] >> (16 * (c & 1)));
         }
+        if (lane < 32) {  // dY of the run
+            const int xi = lane >> 1, half = lane & 1, xx = x0 + xi;
+            uint4 val = make_uint4(0u, 0u, 0u, 0u);
+            if (xx < nx) val = *(const uint4*)(dY + (((long)z * ny + y) * nx + xx) * CH + 8 * half);
+            const uint32_t wd[4] = {val.x, val.y, val.z, val.w};
+            for (int c = 0; c < 8; c++) yt[wv][8 * half + c][xi] = (uint16_t)(wd[c >> 1] >> (16 * (c & 1)));
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const short4_t b = *(const short4_t*)&yt[wv][i][4 * q];  // B[k = 4q + r][co = i]
+        for (int r = 0; r < 4; r++) bsum += bf2f((uint16_t)b[r]);
 #pragma unroll
         for (int tap = 0; tap < TAPS; tap++) {
-            const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
-            const int yy = y + dy, zz = z + dz;
-            const bool row = yy >= 0 && yy < ny && zz >= 0 && zz < nz;
+            const int dx = tap % 3 - 1, row = tap / 3;  // row = (dz + 1) * 3 + (dy + 1)
             short4_t a;
-            for (int r = 0; r < 4; r++) {
-                const int vx = x0 + 4 * q + r, xx = vx + dx;
-                a[r] = (short)(row && vx < nx && xx >= 0 && xx < nx ? X[(((long)zz * ny + yy) * nx + xx) * CH + i]
-                                                                    : (uint16_t)0);
-            }
+            for (int r = 0; r < 4; r++) a[r] = (short)xt[wv][row][i][4 * q + r + dx + 1];  // A[ci = i][k]
             acc[tap] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[tap], 0, 0, 0);
         }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
     }
 #pragma unroll
     for (int tap = 0; tap < TAPS; tap++)
@@ -373,9 +406,11 @@ int check_grid(dmtz_ctx* ctx, int64_t nx, int64_t ny, int64_t nz) {
     return 0;
 }
 
+// Blocks of the convolution kernels: four waves each, about eight runs per wave, at most 4096.
 unsigned conv_blocks(int64_t nx, int64_t ny, int64_t nz) {
     const long runs = ((nx + 15) / 16) * ny * nz;
-    return (unsigned)((runs + 3) / 4);
+    const long b = (runs + 31) / 32;
+    return (unsigned)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
 }
 
 // Forward pass from feat; the output layer's result goes to out (2 floats per vertex).
