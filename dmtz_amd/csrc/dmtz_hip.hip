@@ -826,15 +826,22 @@ __global__ __launch_bounds__(BLOCK) void k_as_branch(const uint64_t* fcf, ull n,
 
 // Precomputed ascending paths (geometry mode). A branch j is (f-critical face j / 2, cofacet
 // j % 2); its comparison steps (Repair::as_fpath) are fixed for the whole run.
-// The first AS_K steps of a branch are kept by the counting walk, so that only longer branches
-// are walked a second time to place their steps.
-constexpr int AS_K = 8;
+// The first k steps of a branch are kept by the counting walk, so that only longer branches are
+// walked a second time to place their steps; k is chosen per run from a memory budget
+// (as_keep_steps), between AS_K_MIN and AS_K_MAX.
+constexpr int AS_K_MIN = 8, AS_K_MAX = 64;
+constexpr size_t AS_KEEP_BYTES = (size_t)4 << 30;
 struct StoreFirst {
     uint2* p;
+    int k;
     DMTZ_HD void operator()(i64 i, i64 cur, i64 t) const {
-        if (i < AS_K) p[i] = make_uint2((uint32_t)cur, (uint32_t)t);
+        if (i < k) p[i] = make_uint2((uint32_t)cur, (uint32_t)t);
     }
 };
+inline int as_keep_steps(ull nb) {
+    const ull k = nb ? (ull)(AS_KEEP_BYTES / sizeof(uint2)) / nb : (ull)AS_K_MAX;
+    return (int)(k < (ull)AS_K_MIN ? AS_K_MIN : (k > (ull)AS_K_MAX ? AS_K_MAX : k));
+}
 struct StoreSteps {
     uint2* p;
     DMTZ_HD void operator()(i64 i, i64 cur, i64 t) const { p[i] = make_uint2((uint32_t)cur, (uint32_t)t); }
@@ -852,26 +859,29 @@ __device__ i64 branch_tet(const uint64_t* fcf, ull j, const T* f, const StarCode
 
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_as_count(const uint64_t* fcf, ull nb, const T* f, const StarCode* cf,
-                                                    Dims d, Counts c, ull* cnt, uint2* first, ull* walk_ctr) {
+                                                    Dims d, Counts c, ull* cnt, uint2* first, int keep, ull* walk_ctr) {
     const ull j = (ull)blockIdx.x * BLOCK + threadIdx.x;
     if (j >= nb) return;
     const i64 q = branch_tet(fcf, j, f, cf, d, c);
     Repair<T, NoSink> R(f, cf, f, cf, d, c, nullptr, 0, 0, 0, NoSink{});
     bool ovf = false;
-    cnt[j] = (ull)R.as_fpath(q, c.N, ovf, StoreFirst{first + j * AS_K});
+    cnt[j] = (ull)R.as_fpath(q, c.N, ovf, StoreFirst{first + j * (ull)keep, keep});
     if (ovf) atomicAdd(walk_ctr, 1ull);
 }
 
-// Steps of branch j at off[j]: copied from the counting walk's first AS_K, or walked again.
+// Steps of branch j at off[j]: copied from the counting walk's first keep steps, or walked again.
+// Thread t takes branch order[t] (branches by path length, so that a wave's walks are alike) or
+// branch t when order is null.
 template <class T>
-__global__ __launch_bounds__(BLOCK) void k_as_fill(const uint64_t* fcf, ull nb, const T* f, const StarCode* cf,
-                                                   Dims d, Counts c, const ull* off, const uint2* first,
-                                                   uint2* steps) {
-    const ull j = (ull)blockIdx.x * BLOCK + threadIdx.x;
-    if (j >= nb) return;
+__global__ __launch_bounds__(BLOCK) void k_as_fill(const uint64_t* fcf, ull nb, const uint32_t* order, const T* f,
+                                                   const StarCode* cf, Dims d, Counts c, const ull* off,
+                                                   const uint2* first, int keep, uint2* steps) {
+    const ull t = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= nb) return;
+    const ull j = order ? (ull)order[t] : t;
     const ull a = off[j], n = off[j + 1] - a;
-    if (n <= (ull)AS_K) {
-        for (ull k = 0; k < n; k++) steps[a + k] = first[j * AS_K + k];
+    if (n <= (ull)keep) {
+        for (ull k = 0; k < n; k++) steps[a + k] = first[j * (ull)keep + k];
         return;
     }
     const i64 q = branch_tet(fcf, j, f, cf, d, c);
@@ -1176,7 +1186,7 @@ struct dmtz_ctx {
     uint16_t* d_fix_key2 = nullptr;  // ascending branch) and their sorted copy
     uint64_t* d_fix_sorted = nullptr;  // records grouped by branch class
     size_t fix_key_cap = 0, fix_key2_cap = 0, fix_sorted_cap = 0;
-    uint2* d_as_first = nullptr;     // first AS_K steps per branch from the counting walk
+    uint2* d_as_first = nullptr;     // first steps per branch from the counting walk
     size_t as_first_cap = 0;
     uint32_t* d_as_off32 = nullptr;  // 32-bit branch offsets and per-step cubes for the dirty test
     uint32_t* d_as_cube = nullptr;
@@ -1868,11 +1878,12 @@ struct Driver {
         if (rc) return rc;
         rc = grow_buf(ctx, ctx->d_as_off, ctx->as_off_cap, nb + 1);
         if (rc) return rc;
-        rc = grow_buf(ctx, ctx->d_as_first, ctx->as_first_cap, nb * AS_K);
+        const int keep = as_keep_steps(nb);
+        rc = grow_buf(ctx, ctx->d_as_first, ctx->as_first_cap, nb * (ull)keep);
         if (rc) return rc;
         CK(hipMemsetAsync(ctx->d_as_cnt + nb, 0, sizeof(ull), ctx->stream));
         k_as_count<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(FL.l[2], nb, f, ctx->d_code_f, d, c, ctx->d_as_cnt,
-                                                               ctx->d_as_first, ctx->d_ctr + K_WALK);
+                                                               ctx->d_as_first, keep, ctx->d_ctr + K_WALK);
         CK(hipGetLastError());
         size_t tmp = 0;
         CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, ctx->d_as_cnt, ctx->d_as_off, (int)(nb + 1), ctx->stream));
@@ -1887,19 +1898,8 @@ struct Driver {
         CK(hipStreamSynchronize(ctx->stream));
         rc = grow_buf(ctx, ctx->d_as_steps, ctx->as_steps_cap, total);
         if (rc) return rc;
-        k_as_fill<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(FL.l[2], nb, f, ctx->d_code_f, d, c, ctx->d_as_off,
-                                                              ctx->d_as_first, ctx->d_as_steps);
-        CK(hipGetLastError());
         as_compact = total < ((ull)1 << 32) && nb < ((ull)1 << 31);
-        if (as_compact) {
-            rc = grow_buf(ctx, ctx->d_as_off32, ctx->as_off32_cap, nb + 1);
-            if (rc) return rc;
-            rc = grow_buf(ctx, ctx->d_as_cube, ctx->as_cube_cap, total + 1);
-            if (rc) return rc;
-            k_as_compact<<<nblocks(nb), BLOCK, 0, ctx->stream>>>(nb, ctx->d_as_off, ctx->d_as_steps, ctx->d_as_off32,
-                                                                  ctx->d_as_cube);
-            CK(hipGetLastError());
-            // branch order by length (keys and the identity order in the scratch of the rule sort)
+        if (as_compact) {  // branch order by path length (for the fill and the checks)
             rc = grow_buf(ctx, ctx->d_as_order, ctx->as_order_cap, 2 * nb);
             if (rc) return rc;
             rc = grow_buf(ctx, ctx->d_fix_key, ctx->fix_key_cap, nb);
@@ -1918,6 +1918,19 @@ struct Driver {
             if (rc) return rc;
             CK(hipcub::DeviceRadixSort::SortPairs(ctx->d_scan_tmp, tmp2, ctx->d_fix_key, ctx->d_fix_key2,
                                                   ctx->d_as_order + nb, ctx->d_as_order, (int)nb, 0, 8, ctx->stream));
+        }
+        k_as_fill<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(FL.l[2], nb, as_compact ? ctx->d_as_order : nullptr, f,
+                                                              ctx->d_code_f, d, c, ctx->d_as_off, ctx->d_as_first,
+                                                              keep, ctx->d_as_steps);
+        CK(hipGetLastError());
+        if (as_compact) {
+            rc = grow_buf(ctx, ctx->d_as_off32, ctx->as_off32_cap, nb + 1);
+            if (rc) return rc;
+            rc = grow_buf(ctx, ctx->d_as_cube, ctx->as_cube_cap, total + 1);
+            if (rc) return rc;
+            k_as_compact<<<nblocks(nb), BLOCK, 0, ctx->stream>>>(nb, ctx->d_as_off, ctx->d_as_steps, ctx->d_as_off32,
+                                                                  ctx->d_as_cube);
+            CK(hipGetLastError());
         }
         return 0;
     }
