@@ -58,7 +58,8 @@ enum : int {
     K_STARS = 16,                               // star codes re-evaluated from dirty lists
     K_DSW = 17,                                 // 2-saddle wall mismatch list
     K_WOVF = 18,                                // wall walks that outgrew their visited set
-    K_NCTR = 19
+    K_ASNEXT = 19,                              // next branch of the work-stealing path walk
+    K_NCTR = 20
 };
 
 constexpr uint8_t FROZEN = 7;  // counter value of a vertex restored to f by the finisher
@@ -830,6 +831,7 @@ __global__ __launch_bounds__(BLOCK) void k_as_branch(const uint64_t* fcf, ull n,
 // walked a second time to place their steps; k is chosen per run from a memory budget
 // (as_keep_steps), between AS_K_MIN and AS_K_MAX.
 constexpr int AS_K_MIN = 8, AS_K_MAX = 64;
+constexpr unsigned AS_WS_BLOCKS = 2048;  // blocks of the work-stealing walk (eight per CU)
 constexpr size_t AS_KEEP_BYTES = (size_t)4 << 30;
 struct StoreFirst {
     uint2* p;
@@ -867,6 +869,56 @@ __global__ __launch_bounds__(BLOCK) void k_as_count(const uint64_t* fcf, ull nb,
     bool ovf = false;
     cnt[j] = (ull)R.as_fpath(q, c.N, ovf, StoreFirst{first + j * (ull)keep, keep});
     if (ovf) atomicAdd(walk_ctr, 1ull);
+}
+
+// k_as_count with work stealing: path lengths are heavy-tailed, so instead of one branch per
+// thread, every lane takes a new branch from a shared counter (*next, one atomic per wave and
+// refill) whenever its walk ends, and the waves run until the branches are exhausted. Each loop
+// round every busy lane advances its walk by one step, so a wave's lanes stay busy until the
+// last branches. Same per-branch results as k_as_count.
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_as_count_ws(const uint64_t* fcf, ull nb, const T* f, const StarCode* cf,
+                                                       Dims d, Counts c, ull* cnt, uint2* first, int keep, ull* next,
+                                                       ull* walk_ctr) {
+    Repair<T, NoSink> R(f, cf, f, cf, d, c, nullptr, 0, 0, 0, NoSink{});
+    const int lane = __lane_id();
+    bool busy = false;
+    ull j = 0;
+    i64 cur = -1, n = 0;
+    for (;;) {
+        const ull idle = __ballot(!busy);
+        bool exhausted = false;
+        if (idle) {
+            const int leader = __ffsll((unsigned long long)idle) - 1;
+            ull base = 0;
+            if (lane == leader) base = atomicAdd(next, (ull)__popcll(idle));
+            const unsigned lo = __shfl((unsigned)(base & 0xffffffffu), leader, 64);
+            const unsigned hi = __shfl((unsigned)(base >> 32), leader, 64);
+            base = ((ull)hi << 32) | lo;
+            exhausted = base >= nb;
+            if (!busy) {
+                j = base + (ull)__popcll(idle & ((1ull << lane) - 1ull));
+                if (j < nb) {
+                    cur = branch_tet(fcf, j, f, cf, d, c);
+                    n = 0;
+                    if (cur < 0) cnt[j] = 0;  // no cofacet on this side: an empty path
+                    else busy = true;
+                }
+            }
+        }
+        if (__ballot(busy) == 0 && exhausted) break;
+        if (busy) {
+            bool fin = R.as_fpath_step(cur, n, StoreFirst{first + j * (ull)keep, keep});
+            if (!fin && n > c.N) {  // as_fpath's cap
+                atomicAdd(walk_ctr, 1ull);
+                fin = true;
+            }
+            if (fin) {
+                cnt[j] = (ull)n;
+                busy = false;
+            }
+        }
+    }
 }
 
 // Steps of branch j at off[j]: copied from the counting walk's first keep steps, or walked again.
@@ -1882,8 +1934,11 @@ struct Driver {
         rc = grow_buf(ctx, ctx->d_as_first, ctx->as_first_cap, nb * (ull)keep);
         if (rc) return rc;
         CK(hipMemsetAsync(ctx->d_as_cnt + nb, 0, sizeof(ull), ctx->stream));
-        k_as_count<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(FL.l[2], nb, f, ctx->d_code_f, d, c, ctx->d_as_cnt,
-                                                               ctx->d_as_first, keep, ctx->d_ctr + K_WALK);
+        CK(hipMemsetAsync(ctx->d_ctr + K_ASNEXT, 0, sizeof(ull), ctx->stream));
+        const unsigned wsb = nblocks(nb) < AS_WS_BLOCKS ? nblocks(nb) : AS_WS_BLOCKS;
+        k_as_count_ws<T><<<wsb, BLOCK, 0, ctx->stream>>>(FL.l[2], nb, f, ctx->d_code_f, d, c, ctx->d_as_cnt,
+                                                          ctx->d_as_first, keep, ctx->d_ctr + K_ASNEXT,
+                                                          ctx->d_ctr + K_WALK);
         CK(hipGetLastError());
         size_t tmp = 0;
         CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, ctx->d_as_cnt, ctx->d_as_off, (int)(nb + 1), ctx->stream));

@@ -621,26 +621,34 @@ struct Repair {
         i64 cur = id, n = 0;
         bool done = id < 0;
         while (!done) {
-            const i64 t = F.pt(cur);
-            bool chain = false;
-            i64 nxt = -1;
-            if (t != -1) {
-                out(n, cur, t);
-                n++;
-                const i64 pft = F.pf(t);
-                chain = pft != -1 && F.pt(pft) == t;
-                i64 t1 = -1, t2 = -1;
-                if (chain) cofacets(t, t1, t2);
-                nxt = (t1 == cur) ? t2 : t1;
-            }
-            done = t == -1 || !chain || nxt == cur || nxt == -1;
-            cur = nxt;
+            done = as_fpath_step(cur, n, out);
             if (!done && n > cap) {
                 overflow = true;
                 done = true;
             }
         }
         return n;
+    }
+
+    // One comparison step of as_fpath from tet cur (n steps so far): records (n, cur, t), moves
+    // cur to the next tet and returns true when the path ends there.
+    template <class Out>
+    DMTZ_HD bool as_fpath_step(i64& cur, i64& n, Out out) const {
+        const i64 t = F.pt(cur);
+        bool chain = false;
+        i64 nxt = -1;
+        if (t != -1) {
+            out(n, cur, t);
+            n++;
+            const i64 pft = F.pf(t);
+            chain = pft != -1 && F.pt(pft) == t;
+            i64 t1 = -1, t2 = -1;
+            if (chain) cofacets(t, t1, t2);
+            nxt = (t1 == cur) ? t2 : t1;
+        }
+        const bool done = t == -1 || !chain || nxt == cur || nxt == -1;
+        cur = nxt;
+        return done;
     }
 
     // get_paired_id main_3d.cu:4499-4537 on field which (0 = f, 1 = g): the tet reached by
