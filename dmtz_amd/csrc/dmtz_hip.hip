@@ -190,7 +190,7 @@ struct __align__(16) CritMask {
 // The lower-star classification is fully unrolled; left alone the compiler spends ~300 VGPRs on
 // it (one wave per SIMD). CODE_BLOCKS_PER_CU bounds the registers so that more waves hide the
 // load latency.
-constexpr int CODE_BLOCKS_PER_CU = 2;
+constexpr int CODE_BLOCKS_PER_CU = 4;
 
 template <class T>
 __global__ __launch_bounds__(BLOCK, CODE_BLOCKS_PER_CU) void k_code(const T* fld, Dims d, Counts c, StarCode* code) {

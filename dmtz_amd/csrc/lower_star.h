@@ -86,147 +86,194 @@ struct CentreLoop {
     }
 };
 
-// Lowest element of S (S a subset of the star slots), or 15 if S is empty. Candidates are
-// restricted to the compile-time list of the slot's link.
-DMTZ_HD int lowest_in(uint32_t S, const uint32_t* blw, const uint8_t* cand, int ncand) {
-    int m = 15;
+// The lowest element of S (a subset of the star slots) as a one-hot mask, 0 if S is empty:
+// the element of S with nothing of S below it. Candidates are restricted to the compile-time
+// list of the slot's link.
+DMTZ_HD uint32_t lowest_bit(uint32_t S, const uint32_t* blw, const uint8_t* cand, int ncand) {
+    uint32_t m = 0;
     for (int c = 0; c < ncand; c++) {
-        int k = cand[c];
-        bool is = ((S >> k) & 1u) && (blw[k] & S) == 0u;
-        m = is ? k : m;
+        const int k = cand[c];
+        m |= ((S >> k) & (uint32_t)((blw[k] & S) == 0u) & 1u) << k;
     }
     return m;
 }
 
+// Fresh copies of the slot masks for one pass: bit tests inside the pass are not shared with
+// other passes, so they do not stay live between them.
+struct PassMasks {
+    uint32_t b[14];
+    uint32_t L;
+    DMTZ_HD PassMasks(const uint32_t* blw, uint32_t L0) {
+#pragma unroll
+        for (int i = 0; i < 14; i++) {
+            b[i] = blw[i];
+            DMTZ_OPAQUE(b[i]);
+        }
+        L = L0;
+        DMTZ_OPAQUE(L);
+    }
+};
+
+DMTZ_HD uint32_t bit1(uint32_t x, int k) { return (x >> k) & 1u; }
+DMTZ_HD uint32_t bit1(uint64_t x, int k) { return (uint32_t)(x >> k) & 1u; }
+
 }  // namespace detail
 
 // Evaluate the five passes on the lower star of the centre of val[15] (val[14] = centre).
+// Every per-cell condition is computed as 0/1 integer arithmetic on the slot masks, with the
+// selections resolved at compile time: "below both n_i and n_j" is blw[i] & blw[j], "strictly
+// between them" is blw[i] ^ blw[j] (the order is total, so whatever is below the lower of the
+// two is below the higher), and the triangles paired in E0 / E1 are set from the slot each edge
+// chose, through the compile-time triangle index of (edge slot, chosen slot).
 template <class T>
 DMTZ_HD void lower_star(const T* val, LowerStar& r) {
+    using detail::bit1;
     uint32_t blw[14];
 #pragma unroll
     for (int i = 0; i < 14; i++) blw[i] = 0;
     detail::PairLoop<0, 1>::run(val, blw);
     uint32_t L = 0;
     detail::CentreLoop<0>::run(val, L);
+#pragma unroll
+    for (int i = 0; i < 14; i++) DMTZ_OPAQUE(blw[i]);
+    DMTZ_OPAQUE(L);
     r.L = L;
 #pragma unroll
     for (int i = 0; i < 14; i++) r.blw[i] = blw[i];
 
     // V pass: steepest descent (main_3d.cu:1953-1975)
-    int s = -1;
+    uint32_t smask = 0;
 #pragma unroll
-    for (int k = 0; k < 14; k++) {
-        bool is = ((L >> k) & 1u) && (blw[k] & L) == 0u;
-        s = is ? k : s;
-    }
-    r.s = s;
+    for (int k = 0; k < 14; k++) smask |= (bit1(L, k) & (uint32_t)((blw[k] & L) == 0u)) << k;
+    r.s = smask ? (int)__builtin_ctz(smask) : -1;
+    const uint32_t Ls = L & ~smask;  // edges of the lower star not taken by V
 
     // E0 (main_3d.cu:2057-2071): edge {v, n_a} with a != s takes its lowest link vertex below n_a.
-    uint64_t e0 = ~0ull;
+    // The triangle of (a, chosen slot) is paired with the edge.
+    uint64_t e0 = 0xFFull << 56, tE0 = 0;  // bits above the 14 nibbles stay set
+    uint32_t pe0 = 0;  // edges paired in E0
+    {
+    const detail::PassMasks P(blw, L);
 #pragma unroll
     for (int a = 0; a < 14; a++) {
-        uint32_t Ca = L & ST.adj[a] & blw[a];
-        int c = detail::lowest_in(Ca, blw, ST.adj_l[a], ST.adj_n[a]);
-        bool ok = ((L >> a) & 1u) && a != s && c != 15;
-        uint64_t nib = ok ? (uint64_t)c : 15ull;
-        e0 &= ~(15ull << (4 * a));
-        e0 |= nib << (4 * a);
+        const uint32_t Ca = P.L & ST.adj[a] & P.b[a];
+        uint32_t m = detail::lowest_bit(Ca, P.b, ST.adj_l[a], ST.adj_n[a]);
+        m &= 0u - bit1(Ls, a);
+        uint32_t nib = 15;
+#pragma unroll
+        for (int c = 0; c < 6; c++) {
+            if (c < ST.adj_n[a]) {
+                const int k = ST.adj_l[a][c];
+                const uint32_t sel = bit1(m, k);
+                nib ^= (15u ^ (uint32_t)k) & (0u - sel);
+                tE0 |= (uint64_t)sel << ST.adj_t[a][c];
+            }
+        }
+        e0 |= (uint64_t)nib << (4 * a);
+        pe0 |= (uint32_t)(m != 0u) << a;
+        DMTZ_OPAQUE(e0);
+        DMTZ_OPAQUE(tE0);
+        DMTZ_OPAQUE(pe0);
+    }
     }
     r.e0 = e0;
 
-    // Triangles: orientation, E0 status, T0 (main_3d.cu:2269-2279). T0 can only reach a vertex
-    // below both other vertices of the triangle, so the triangle is the top face of the tet it
-    // chooses and no other face can choose that tet: the tet is claimed exactly when its top face
-    // pairs with it, and the claims are collected here.
-    uint64_t tin = 0, thi = 0, tE0 = 0, tT0 = 0, tD0 = 0;
+    // Triangles: membership, orientation and T0 (main_3d.cu:2269-2279). T0 can only reach a
+    // vertex below both other vertices of the triangle, so the triangle is the top face of the
+    // tet it chooses and no other face can choose that tet: the tet is claimed exactly when its
+    // top face pairs with it, and the claims are collected here.
+    uint64_t tin = 0, thi = 0, tT0 = 0, tD0 = 0;
     uint32_t qT0 = 0;
+    {
+    const detail::PassMasks P(blw, L);
 #pragma unroll
     for (int t = 0; t < 36; t++) {
         const int i = ST.tri_i[t], j = ST.tri_j[t];
         const int k0 = ST.tri_k[t][0], k1 = ST.tri_k[t][1];
-        bool in = ((L >> i) & 1u) && ((L >> j) & 1u);
-        bool hi = (blw[i] >> j) & 1u;  // n_j below n_i: a = i, b = j
-        int a = hi ? i : j, b = hi ? j : i;
-        int e0a = (int)((e0 >> (4 * a)) & 15u);
-        bool pE0 = in && e0a == b;
-        uint32_t blwb = hi ? blw[j] : blw[i];
-        bool c0 = ((L >> k0) & 1u) && ((blwb >> k0) & 1u);
-        bool c1 = ((L >> k1) & 1u) && ((blwb >> k1) & 1u);
-        bool pick1 = c1 && (!c0 || ((blw[k0] >> k1) & 1u));
-        bool pT0 = in && !pE0 && (c0 || c1);
-        qT0 |= (uint32_t)pT0 << (pick1 ? ST.tri_q[t][1] : ST.tri_q[t][0]);
+        const uint32_t in = bit1(P.L, i) & bit1(P.L, j);
+        const uint32_t BB = P.L & P.b[i] & P.b[j];
+        const uint32_t c0 = bit1(BB, k0), c1 = bit1(BB, k1);
+        const uint32_t pick1 = c1 & ((c0 ^ 1u) | bit1(P.b[k0], k1));
+        const uint32_t pick0 = c0 & (pick1 ^ 1u);
+        const uint32_t fr = in & (bit1(tE0, t) ^ 1u);
+        const uint32_t p1 = fr & pick1, p0 = fr & pick0;
+        qT0 |= (p0 << ST.tri_q[t][0]) | (p1 << ST.tri_q[t][1]);
         tin |= (uint64_t)in << t;
-        thi |= (uint64_t)hi << t;
-        tE0 |= (uint64_t)pE0 << t;
-        tT0 |= (uint64_t)pT0 << t;
-        tD0 |= (uint64_t)(pT0 && pick1) << t;
+        thi |= (uint64_t)bit1(P.b[i], j) << t;  // n_j below n_i
+        tT0 |= (uint64_t)(p0 | p1) << t;
+        tD0 |= (uint64_t)p1 << t;
+        DMTZ_OPAQUE(qT0);
+        DMTZ_OPAQUE(tin);
+        DMTZ_OPAQUE(thi);
+        DMTZ_OPAQUE(tT0);
+        DMTZ_OPAQUE(tD0);
+    }
     }
 
     // E1 (main_3d.cu:2073-2096): a free edge {v, n_a} takes the lowest link vertex between
     // n_a and v, if that triangle is still free after E0 and T0.
-    uint64_t e1 = ~0ull;
+    const uint64_t before = tE0 | tT0;
+    const uint32_t freeE = Ls & ~pe0;
+    uint64_t e1 = 0xFFull << 56, tE1 = 0;
+    {
+    const detail::PassMasks P(blw, L);
 #pragma unroll
     for (int a = 0; a < 14; a++) {
-        bool freeE = ((L >> a) & 1u) && a != s && (int)((e0 >> (4 * a)) & 15u) == 15;
-        uint32_t Ma = L & ST.adj[a] & ~blw[a];
-        int m = 15;
-        bool taken = false;
+        const uint32_t Ma = P.L & ST.adj[a] & ~P.b[a];
+        uint32_t m = detail::lowest_bit(Ma, P.b, ST.adj_l[a], ST.adj_n[a]);
+        m &= 0u - bit1(freeE, a);
+        uint32_t nib = 15;
 #pragma unroll
         for (int c = 0; c < 6; c++) {
             if (c < ST.adj_n[a]) {
                 const int k = ST.adj_l[a][c];
                 const int t = ST.adj_t[a][c];
-                bool is = ((Ma >> k) & 1u) && (blw[k] & Ma) == 0u;
-                m = is ? k : m;
-                taken = is ? (((tE0 | tT0) >> t) & 1ull) != 0 : taken;
+                const uint32_t ok = bit1(m, k) & (bit1(before, t) ^ 1u);
+                nib ^= (15u ^ (uint32_t)k) & (0u - ok);
+                tE1 |= (uint64_t)ok << t;
             }
         }
-        bool ok = freeE && m != 15 && !taken;
-        uint64_t nib = ok ? (uint64_t)m : 15ull;
-        e1 &= ~(15ull << (4 * a));
-        e1 |= nib << (4 * a);
+        e1 |= (uint64_t)nib << (4 * a);
+        DMTZ_OPAQUE(e1);
+        DMTZ_OPAQUE(tE1);
+    }
     }
     r.e1 = e1;
 
     // Tets of the lower star.
     uint32_t qin = 0;
 #pragma unroll
-    for (int q = 0; q < 24; q++) {
-        const int i = ST.tet_s[q][0], j = ST.tet_s[q][1], k = ST.tet_s[q][2];
-        qin |= (uint32_t)(((L >> i) & 1u) && ((L >> j) & 1u) && ((L >> k) & 1u)) << q;
-    }
+    for (int q = 0; q < 24; q++)
+        qin |= (bit1(L, ST.tet_s[q][0]) & bit1(L, ST.tet_s[q][1]) & bit1(L, ST.tet_s[q][2])) << q;
 
-    // Triangles: E1 status and T1 (main_3d.cu:2282-2299). T1 reaches the vertex strictly between
-    // the triangle's other two, so the triangle is the face of the chosen tet without its middle
+    // Triangles: T1 (main_3d.cu:2282-2299). T1 reaches the vertex strictly between the
+    // triangle's other two, so the triangle is the face of the chosen tet without its middle
     // vertex, the only face that can claim it in T1: the claims are collected here.
-    uint64_t tE1 = 0, tT1 = 0, tD1 = 0;
+    const uint64_t taken = before | tE1;
+    uint64_t tT1 = 0, tD1 = 0;
     uint32_t qT1 = 0;
+    {
+    const detail::PassMasks P(blw, L);
 #pragma unroll
     for (int t = 0; t < 36; t++) {
         const int i = ST.tri_i[t], j = ST.tri_j[t];
         const int k0 = ST.tri_k[t][0], k1 = ST.tri_k[t][1];
-        bool in = (tin >> t) & 1ull;
-        bool hi = (thi >> t) & 1ull;
-        int a = hi ? i : j, b = hi ? j : i;
-        bool before = (((tE0 | tT0) >> t) & 1ull) != 0;
-        int e1b = (int)((e1 >> (4 * b)) & 15u);
-        bool pE1 = in && !before && e1b == a;
-        uint32_t blwa = hi ? blw[i] : blw[j];
-        uint32_t blwb = hi ? blw[j] : blw[i];
-        // strictly between n_b and n_a
-        bool c0 = ((L >> k0) & 1u) && ((blwa >> k0) & 1u) && !((blwb >> k0) & 1u);
-        bool c1 = ((L >> k1) & 1u) && ((blwa >> k1) & 1u) && !((blwb >> k1) & 1u);
-        bool pick1 = c1 && (!c0 || ((blw[k0] >> k1) & 1u));
         const int q0 = ST.tri_q[t][0], q1 = ST.tri_q[t][1];
-        bool tetFree = pick1 ? !((qT0 >> q1) & 1u) : !((qT0 >> q0) & 1u);
-        bool pT1 = in && !before && !pE1 && (c0 || c1) && tetFree;
-        qT1 |= (uint32_t)pT1 << (pick1 ? q1 : q0);
-        tE1 |= (uint64_t)pE1 << t;
-        tT1 |= (uint64_t)pT1 << t;
-        tD1 |= (uint64_t)(pT1 && pick1) << t;
+        const uint32_t BT = P.L & (P.b[i] ^ P.b[j]);
+        const uint32_t c0 = bit1(BT, k0), c1 = bit1(BT, k1);
+        const uint32_t pick1 = c1 & ((c0 ^ 1u) | bit1(P.b[k0], k1));
+        const uint32_t pick0 = c0 & (pick1 ^ 1u);
+        const uint32_t fr = bit1(tin, t) & (bit1(taken, t) ^ 1u);
+        const uint32_t p1 = fr & pick1 & (bit1(qT0, q1) ^ 1u);
+        const uint32_t p0 = fr & pick0 & (bit1(qT0, q0) ^ 1u);
+        qT1 |= (p0 << q0) | (p1 << q1);
+        tT1 |= (uint64_t)(p0 | p1) << t;
+        tD1 |= (uint64_t)p1 << t;
+        DMTZ_OPAQUE(qT1);
+        DMTZ_OPAQUE(tT1);
+        DMTZ_OPAQUE(tD1);
     }
-
+    }
 
     r.tin = tin;
     r.thi = thi;

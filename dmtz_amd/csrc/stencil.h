@@ -21,6 +21,23 @@
 #define DMTZ_HD_CALL __attribute__((noinline))
 #endif
 
+// Scheduling fence for long unrolled bitmask code: keeps the compiler from hoisting the
+// independent work of later iterations (which multiplies the live registers); no effect on host.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define DMTZ_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
+#else
+#define DMTZ_SCHED_FENCE() ((void)0)
+#endif
+
+// Value barrier: the compiler treats x as an unknown value from here on (no instruction is
+// emitted). Used on bitmasks built from many comparisons, so that later bit tests read the mask
+// instead of keeping every comparison result alive.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define DMTZ_OPAQUE(x) asm volatile("" : "+v"(x))
+#else
+#define DMTZ_OPAQUE(x) ((void)0)
+#endif
+
 namespace dmtz {
 
 typedef int64_t i64;
