@@ -996,31 +996,50 @@ __global__ __launch_bounds__(BLOCK) void k_as_len_keys(ull nb, const ull* cnt, u
     idx[j] = (uint32_t)j;
 }
 
-// k_as_check over the compact copies: the dirty test reads 4 bytes per step, four steps per
-// round trip. Thread t checks branch order[t]; results stay indexed by branch.
+// Per-step dirty bits of the compact path copies: bit s of sd is set when the tet of step s
+// lies in a marked cube. One lane per step and one 64-bit word per wave (a ballot), so the cube
+// ids are read coalesced, 4 bytes per step.
+__global__ __launch_bounds__(BLOCK) void k_as_step_dirty(ull total, const uint32_t* cube, const uint32_t* cubes,
+                                                         uint64_t* sd) {
+    const ull s = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    bool dq = false;
+    if (s < total) {
+        const uint32_t q = cube[s];
+        dq = ((cubes[q >> 5] >> (q & 31u)) & 1u) != 0;
+    }
+    const uint64_t m = __ballot(dq);
+    if (__lane_id() == 0 && s < total) sd[s >> 6] = m;
+}
+
+// k_as_check over the compact copies. Thread t checks branch order[t]; results stay indexed by
+// branch. Unless full, a branch with no dirty step keeps its last result, and a branch whose
+// last result was "no mismatch" starts its walk at its first dirty step: the steps before it
+// matched then and read unchanged star codes now.
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_as_check32(ull nb, const uint32_t* order, const uint32_t* off32,
-                                                      const uint32_t* cube, const uint2* steps, const T* g,
-                                                      const StarCode* cg, Dims d, Counts c, const uint32_t* cubes,
-                                                      int full, int64_t* res) {
+                                                      const uint64_t* sd, const uint2* steps, const T* g,
+                                                      const StarCode* cg, Dims d, Counts c, int full, int64_t* res) {
     const ull t = (ull)blockIdx.x * BLOCK + threadIdx.x;
     if (t >= nb) return;
     const ull j = order[t];
     const uint32_t a = off32[j], b = off32[j + 1];
-    bool redo = full != 0;
-    auto dirty = [&](uint32_t q) { return ((cubes[q >> 5] >> (q & 31u)) & 1u) != 0; };
-    for (uint32_t s = a; s < b && !redo; s += 4) {
-        const uint32_t q0 = cube[s];
-        const uint32_t q1 = s + 1 < b ? cube[s + 1] : q0;
-        const uint32_t q2 = s + 2 < b ? cube[s + 2] : q0;
-        const uint32_t q3 = s + 3 < b ? cube[s + 3] : q0;
-        redo = dirty(q0) || dirty(q1) || dirty(q2) || dirty(q3);
+    uint32_t s = a;
+    if (!full) {
+        if (a >= b) return;
+        uint32_t first = b;
+        for (uint32_t w = a >> 6; w <= (b - 1) >> 6 && first == b; w++) {
+            const uint32_t lo = w << 6;
+            uint64_t m = sd[w];
+            if (lo < a) m &= ~0ull << (a - lo);
+            if (b - lo < 64) m &= (1ull << (b - lo)) - 1ull;
+            if (m) first = lo + (uint32_t)__builtin_ctzll(m);
+        }
+        if (first == b) return;
+        if (res[2 * j] < 0) s = first;
     }
-    if (!redo) return;
     FieldGrad<T> G(g, cg, d, c);
     i64 found = -1, dv = -1;
-    bool done = a >= b;
-    uint32_t s = a;
+    bool done = s >= b;
     while (!done) {
         const uint2 st = steps[s];
         if (G.pt((i64)st.x) != (i64)st.y) {
@@ -1243,7 +1262,8 @@ struct dmtz_ctx {
     uint32_t* d_as_off32 = nullptr;  // 32-bit branch offsets and per-step cubes for the dirty test
     uint32_t* d_as_cube = nullptr;
     uint32_t* d_as_order = nullptr;  // branches by step count (first nb), sort input (second nb)
-    size_t as_off32_cap = 0, as_cube_cap = 0, as_order_cap = 0;
+    uint64_t* d_as_sd = nullptr;     // per-step dirty bits
+    size_t as_off32_cap = 0, as_cube_cap = 0, as_order_cap = 0, as_sd_cap = 0;
     uint32_t* d_wall_scr = nullptr;  // wall walks: per-thread visited sets and stacks
     size_t wall_scr_cap = 0;
     uint32_t* d_wall_ovf = nullptr;  // wall walks to repeat with more slots (two lists)
@@ -1324,8 +1344,10 @@ void free_paths(dmtz_ctx* ctx) {
     (void)hipFree(ctx->d_as_off32);
     (void)hipFree(ctx->d_as_cube);
     (void)hipFree(ctx->d_as_order);
+    (void)hipFree(ctx->d_as_sd);
     ctx->d_as_off32 = ctx->d_as_cube = ctx->d_as_order = nullptr;
-    ctx->as_off32_cap = ctx->as_cube_cap = ctx->as_order_cap = 0;
+    ctx->d_as_sd = nullptr;
+    ctx->as_off32_cap = ctx->as_cube_cap = ctx->as_order_cap = ctx->as_sd_cap = 0;
     (void)hipFree(ctx->d_wall_scr);
     (void)hipFree(ctx->d_wall_ovf);
     ctx->d_wall_scr = ctx->d_wall_ovf = nullptr;
@@ -1905,6 +1927,7 @@ struct Driver {
     // Ascending geometry checks over precomputed f paths: in geometry mode, when tet and face
     // ids fit 32 bits.
     bool as_ready = false, as_full = true, as_compact = false;
+    ull as_total = 0;  // steps of the compact path copies
 
     bool as_precomputable() const {
         return fl.preserve_geometry == 1 && c.T < ((i64)1 << 32) && c.F < ((i64)1 << 32) && c.E < ((i64)1 << 32) &&
@@ -1986,6 +2009,9 @@ struct Driver {
             k_as_compact<<<nblocks(nb), BLOCK, 0, ctx->stream>>>(nb, ctx->d_as_off, ctx->d_as_steps, ctx->d_as_off32,
                                                                   ctx->d_as_cube);
             CK(hipGetLastError());
+            rc = grow_buf(ctx, ctx->d_as_sd, ctx->as_sd_cap, (total + 63) / 64 + 1);
+            if (rc) return rc;
+            as_total = total;
         }
         return 0;
     }
@@ -2018,12 +2044,17 @@ struct Driver {
         }
         const ull nb = 2 * nfcf;
         if (!nb) return 0;
-        if (as_compact)
+        if (as_compact) {
+            if (!as_full && as_total) {
+                k_as_step_dirty<<<nblocks(as_total), BLOCK, 0, ctx->stream>>>(as_total, ctx->d_as_cube, ctx->d_cubes,
+                                                                             ctx->d_as_sd);
+                CK(hipGetLastError());
+            }
             k_as_check32<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(nb, ctx->d_as_order, ctx->d_as_off32,
-                                                                     ctx->d_as_cube,
-                                                                     ctx->d_as_steps, g, ctx->d_code_g, d, c,
-                                                                     ctx->d_cubes, as_full ? 1 : 0, ctx->d_walk);
-        else
+                                                                     ctx->d_as_sd, ctx->d_as_steps, g,
+                                                                     ctx->d_code_g, d, c, as_full ? 1 : 0,
+                                                                     ctx->d_walk);
+        } else
             k_as_check<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(nb, ctx->d_as_off, ctx->d_as_steps, g,
                                                                    ctx->d_code_g, d, c, ctx->d_cubes, as_full ? 1 : 0,
                                                                    ctx->d_walk);
