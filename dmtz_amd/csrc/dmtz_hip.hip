@@ -202,20 +202,36 @@ __global__ __launch_bounds__(BLOCK, CODE_BLOCKS_PER_CU) void k_code(const T* fld
 }
 
 // g star code of vertex v and its mismatch mask: the cells v owns in g whose critical status in
-// g differs from f's (owner_mismatch, main_3d.cu:2492-2550).
+// g differs from f's (owner_mismatch, main_3d.cu:2492-2550). Returns whether the code changed.
 template <class T>
-__device__ __forceinline__ void code_g_at(const T* g, Dims d, i64 v, StarCode* code, const CritMask* crit_f,
+__device__ __forceinline__ bool code_g_at(const T* g, Dims d, i64 v, StarCode* code, const CritMask* crit_f,
                                           CritMask* mm) {
     LowerStar ls;
     star_of(g, d, v, ls);
     StarCode sc;
     encode_star(ls, sc);
+    const StarCode old = code[v];
     code[v] = sc;
     const CritMask cf = crit_f[v];
     CritMask m;
     owner_mismatch(sc, cf.lo, cf.hi, m.lo, m.hi);
     m.pad = 0;
     mm[v] = m;
+    return ((old.w[0] ^ sc.w[0]) | (old.w[1] ^ sc.w[1]) | (old.w[2] ^ sc.w[2]) | (old.w[3] ^ sc.w[3])) != 0;
+}
+
+// Mark vertex v and the (up to 8) cubes that have it as a corner in the dirty bitmaps of the
+// path checks.
+__device__ __forceinline__ void mark_vertex(Dims d, uint32_t v, uint32_t* cubes, uint32_t* vbits) {
+    atomicOr(vbits + (v >> 5), 1u << (v & 31u));
+    i64 x, y, z;
+    vxyz(d, (i64)v, x, y, z);
+    for (int k = 0; k < 8; k++) {
+        const i64 ax = x - (k & 1), ay = y - ((k >> 1) & 1), az = z - ((k >> 2) & 1);
+        if (ax < 0 || ay < 0 || az < 0 || ax > d.W - 2 || ay > d.H - 2 || az > d.D - 2) continue;
+        const ull cube = (ull)(az * (d.W - 1) * (d.H - 1) + ay * (d.W - 1) + ax);
+        atomicOr(cubes + (cube >> 5), 1u << (cube & 31u));
+    }
 }
 
 template <class T>
@@ -227,16 +243,21 @@ __global__ __launch_bounds__(BLOCK, CODE_BLOCKS_PER_CU) void k_code_g(const T* g
 
 // The list length is read on the device (*n_ptr, written by the kernel that filled the list), so
 // the host launches it without waiting; the grid strides over the list. Block 0 adds the length
-// to *stars, the count of star codes re-evaluated in the run.
+// to *stars, the count of star codes re-evaluated in the run. With cubes != null, every vertex
+// whose code changed is marked for the path checks (their results read nothing but star codes).
 template <class T>
 __global__ __launch_bounds__(BLOCK, CODE_BLOCKS_PER_CU) void k_code_list_g(const T* g, Dims d, StarCode* code,
                                                                          const CritMask* crit_f, CritMask* mm,
                                                                          const uint32_t* list, const ull* n_ptr,
-                                                                         ull* stars) {
+                                                                         ull* stars, uint32_t* cubes,
+                                                                         uint32_t* vbits) {
     const ull n = *n_ptr;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(stars, n);
-    for (ull i = (ull)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (ull)gridDim.x * BLOCK)
-        code_g_at(g, d, (i64)list[i], code, crit_f, mm);
+    for (ull i = (ull)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (ull)gridDim.x * BLOCK) {
+        const uint32_t v = list[i];
+        const bool changed = code_g_at(g, d, (i64)v, code, crit_f, mm);
+        if (changed && cubes) mark_vertex(d, v, cubes, vbits);
+    }
 }
 
 // f-critical status of the local cells of every vertex (once per run), in two passes: each
@@ -1011,10 +1032,24 @@ __global__ __launch_bounds__(BLOCK) void k_as_step_dirty(ull total, const uint32
     if (__lane_id() == 0 && s < total) sd[s >> 6] = m;
 }
 
+// First step in [s, b) whose dirty bit is set, or b.
+__device__ __forceinline__ uint32_t next_dirty(const uint64_t* sd, uint32_t s, uint32_t b) {
+    uint32_t r = b;
+    for (uint32_t w = s >> 6; s < b && w <= (b - 1) >> 6 && r == b; w++) {
+        const uint32_t lo = w << 6;
+        uint64_t m = sd[w];
+        if (lo < s) m &= ~0ull << (s - lo);
+        if (b - lo < 64) m &= (1ull << (b - lo)) - 1ull;
+        if (m) r = lo + (uint32_t)__builtin_ctzll(m);
+    }
+    return r;
+}
+
 // k_as_check over the compact copies. Thread t checks branch order[t]; results stay indexed by
 // branch. Unless full, a branch with no dirty step keeps its last result, and a branch whose
-// last result was "no mismatch" starts its walk at its first dirty step: the steps before it
-// matched then and read unchanged star codes now.
+// last result was "no mismatch" compares only its dirty steps, in order: the others matched then
+// and read unchanged star codes now. A branch that had a mismatch and has a dirty step is walked
+// again from its start.
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_as_check32(ull nb, const uint32_t* order, const uint32_t* off32,
                                                       const uint64_t* sd, const uint2* steps, const T* g,
@@ -1024,52 +1059,25 @@ __global__ __launch_bounds__(BLOCK) void k_as_check32(ull nb, const uint32_t* or
     const ull j = order[t];
     const uint32_t a = off32[j], b = off32[j + 1];
     uint32_t s = a;
+    bool sparse = false;  // visit dirty steps only
     if (!full) {
-        if (a >= b) return;
-        uint32_t first = b;
-        for (uint32_t w = a >> 6; w <= (b - 1) >> 6 && first == b; w++) {
-            const uint32_t lo = w << 6;
-            uint64_t m = sd[w];
-            if (lo < a) m &= ~0ull << (a - lo);
-            if (b - lo < 64) m &= (1ull << (b - lo)) - 1ull;
-            if (m) first = lo + (uint32_t)__builtin_ctzll(m);
-        }
-        if (first == b) return;
-        if (res[2 * j] < 0) s = first;
+        s = next_dirty(sd, a, b);
+        if (s >= b) return;
+        sparse = res[2 * j] < 0;
+        if (!sparse) s = a;
     }
     FieldGrad<T> G(g, cg, d, c);
     i64 found = -1, dv = -1;
-    bool done = s >= b;
-    while (!done) {
+    while (s < b && found < 0) {
         const uint2 st = steps[s];
         if (G.pt((i64)st.x) != (i64)st.y) {
             found = (i64)steps[a].x;
             dv = (i64)st.x;
         }
-        s++;
-        done = found >= 0 || s >= b;
+        s = sparse ? next_dirty(sd, s + 1, b) : s + 1;
     }
     res[2 * j] = found;
     res[2 * j + 1] = dv;
-}
-
-// Mark each dirty vertex and the (up to 8) cubes that have it as a corner. The list length is
-// read on the device (*n_ptr).
-__global__ __launch_bounds__(BLOCK) void k_cube_mark(const uint32_t* dirty, const ull* n_ptr, Dims d,
-                                                     uint32_t* cubes, uint32_t* vbits) {
-    const ull n = *n_ptr;
-    for (ull i = (ull)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (ull)gridDim.x * BLOCK) {
-        const uint32_t v = dirty[i];
-        atomicOr(vbits + (v >> 5), 1u << (v & 31u));
-        i64 x, y, z;
-        vxyz(d, (i64)v, x, y, z);
-        for (int k = 0; k < 8; k++) {
-            const i64 ax = x - (k & 1), ay = y - ((k >> 1) & 1), az = z - ((k >> 2) & 1);
-            if (ax < 0 || ay < 0 || az < 0 || ax > d.W - 2 || ay > d.H - 2 || az > d.D - 2) continue;
-            const ull cube = (ull)(az * (d.W - 1) * (d.H - 1) + ay * (d.W - 1) + ax);
-            atomicOr(cubes + (cube >> 5), 1u << (cube & 31u));
-        }
-    }
 }
 
 __global__ __launch_bounds__(BLOCK) void k_as_combine(const int64_t* res, ull n, PathList out) {
@@ -1666,13 +1674,9 @@ struct Driver {
                                                                    ctx->d_ctr + K_DIRTY);
         CK(hipGetLastError());
         k_code_list_g<T><<<stride_blocks(15 * est), BLOCK, 0, ctx->stream>>>(
-            g, d, ctx->d_code_g, ctx->d_critf, ctx->d_mm, ctx->d_dirty, ctx->d_ctr + K_DIRTY, ctx->d_ctr + K_STARS);
+            g, d, ctx->d_code_g, ctx->d_critf, ctx->d_mm, ctx->d_dirty, ctx->d_ctr + K_DIRTY, ctx->d_ctr + K_STARS,
+            as_ready ? ctx->d_cubes : nullptr, as_ready ? ctx->d_vbits : nullptr);
         CK(hipGetLastError());
-        if (as_ready) {
-            k_cube_mark<<<stride_blocks(15 * est), BLOCK, 0, ctx->stream>>>(ctx->d_dirty, ctx->d_ctr + K_DIRTY, d,
-                                                                          ctx->d_cubes, ctx->d_vbits);
-            CK(hipGetLastError());
-        }
         return 0;
     }
 
