@@ -213,8 +213,13 @@ __device__ __forceinline__ bool code_g_at(const T* g, Dims d, i64 v, StarCode* c
     const StarCode old = code[v];
     code[v] = sc;
     const CritMask cf = crit_f[v];
+    uint64_t c_lo, i_lo;  // owner_mismatch of sc, from the lower star's own masks
+    uint32_t c_hi, i_hi;
+    crit_mask(ls, c_lo, c_hi);
+    member_mask(ls, i_lo, i_hi);
     CritMask m;
-    owner_mismatch(sc, cf.lo, cf.hi, m.lo, m.hi);
+    m.lo = (c_lo ^ cf.lo) & i_lo;
+    m.hi = (c_hi ^ cf.hi) & i_hi;
     m.pad = 0;
     mm[v] = m;
     return ((old.w[0] ^ sc.w[0]) | (old.w[1] ^ sc.w[1]) | (old.w[2] ^ sc.w[2]) | (old.w[3] ^ sc.w[3])) != 0;
@@ -1263,8 +1268,16 @@ struct dmtz_ctx {
     int64_t* d_path = nullptr;    // ds id/rec, as id/rec, wall id/rec (fc_cap each)
     uint16_t* d_fix_key = nullptr;   // sort keys (branch class per mismatch record, path length per
     uint16_t* d_fix_key2 = nullptr;  // ascending branch) and their sorted copy
-    uint64_t* d_fix_sorted = nullptr;  // records grouped by branch class
-    size_t fix_key_cap = 0, fix_key2_cap = 0, fix_sorted_cap = 0;
+    size_t fix_key_cap = 0, fix_key2_cap = 0;
+    // Repairs of record types 1..3: class keys, sorted records and sort scratch per type, and one
+    // stream per type for sweeps with few records, when the types run concurrently.
+    hipStream_t fix_stream[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t fix_fork = nullptr, fix_join[3] = {nullptr, nullptr, nullptr};
+    uint16_t* d_fk[3] = {nullptr, nullptr, nullptr};
+    uint16_t* d_fk2[3] = {nullptr, nullptr, nullptr};
+    uint64_t* d_fsorted[3] = {nullptr, nullptr, nullptr};
+    uint8_t* d_ftmp[3] = {nullptr, nullptr, nullptr};
+    size_t fk_cap[3] = {0, 0, 0}, fk2_cap[3] = {0, 0, 0}, fsorted_cap[3] = {0, 0, 0}, ftmp_cap[3] = {0, 0, 0};
     uint2* d_as_first = nullptr;     // first steps per branch from the counting walk
     size_t as_first_cap = 0;
     uint32_t* d_as_off32 = nullptr;  // 32-bit branch offsets and per-step cubes for the dirty test
@@ -1587,45 +1600,58 @@ struct Driver {
     }
 
     // Repair phase of one critical-point sweep (main_3d.cu:6801-6833) + apply (:6837).
+    // With few records per type a repair launch takes about the time of its longest chain of
+    // dependent reads, so the types run concurrently (the edits form a set: their order does not
+    // matter). With many records the launches fill the GPU and run one after another.
+    static constexpr ull FIX_CONCURRENT_MAX = 1ull << 15;
+
     int fix_cp(const ull counts[4]) {
         int rc = begin_edits();
         if (rc) return rc;
         Lists4 L = rec_lists(ctx);
         ull nrec = 0;
+        bool conc = true;
+        for (int ty = 1; ty < 4; ty++) conc = conc && counts[ty] < FIX_CONCURRENT_MAX;
+        if (conc) CK(hipEventRecord(ctx->fix_fork, ctx->stream));
         for (int ty = 0; ty < 4; ty++) {
             if (counts[ty] == 0) continue;
             if ((ty == 1 || ty == 2) && fl.preserve_saddles != 1) continue;
             nrec += counts[ty];
             const uint64_t* recs = L.l[ty];
+            const int k = ty - 1;
+            hipStream_t st = ctx->stream;
+            if (ty != 0 && conc) {
+                st = ctx->fix_stream[k];
+                CK(hipStreamWaitEvent(st, ctx->fix_fork, 0));
+            }
             if (ty != 0) {  // group the records by branch class first
-                rc = grow_buf(ctx, ctx->d_fix_key, ctx->fix_key_cap, counts[ty]);
+                rc = grow_buf(ctx, ctx->d_fk[k], ctx->fk_cap[k], counts[ty]);
                 if (rc) return rc;
-                rc = grow_buf(ctx, ctx->d_fix_sorted, ctx->fix_sorted_cap, counts[ty]);
+                rc = grow_buf(ctx, ctx->d_fsorted[k], ctx->fsorted_cap[k], counts[ty]);
                 if (rc) return rc;
-                rc = grow_buf(ctx, ctx->d_fix_key2, ctx->fix_key2_cap, counts[ty]);
+                rc = grow_buf(ctx, ctx->d_fk2[k], ctx->fk2_cap[k], counts[ty]);
                 if (rc) return rc;
                 auto cls = ty == 1 ? k_fix_class<T, 1> : ty == 2 ? k_fix_class<T, 2> : k_fix_class<T, 3>;
-                cls<<<nblocks(counts[ty]), BLOCK, 0, ctx->stream>>>(recs, counts[ty], f, ctx->d_code_f, g,
-                                                                    ctx->d_code_g, ctx->d_critf, d, c, ctx->d_fix_key);
+                cls<<<nblocks(counts[ty]), BLOCK, 0, st>>>(recs, counts[ty], f, ctx->d_code_f, g, ctx->d_code_g,
+                                                           ctx->d_critf, d, c, ctx->d_fk[k]);
                 CK(hipGetLastError());
                 size_t tmp = 0;
-                CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, ctx->d_fix_key, ctx->d_fix_key2, recs,
-                                                      ctx->d_fix_sorted, (int)counts[ty], 0, FIX_CLASS_BITS,
-                                                      ctx->stream));
-                uint8_t* tp = (uint8_t*)ctx->d_scan_tmp;
-                rc = grow_buf(ctx, tp, ctx->scan_tmp_cap, tmp);
-                ctx->d_scan_tmp = tp;
+                CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, ctx->d_fk[k], ctx->d_fk2[k], recs,
+                                                      ctx->d_fsorted[k], (int)counts[ty], 0, FIX_CLASS_BITS, st));
+                rc = grow_buf(ctx, ctx->d_ftmp[k], ctx->ftmp_cap[k], tmp);
                 if (rc) return rc;
-                CK(hipcub::DeviceRadixSort::SortPairs(ctx->d_scan_tmp, tmp, ctx->d_fix_key, ctx->d_fix_key2, recs,
-                                                      ctx->d_fix_sorted, (int)counts[ty], 0, FIX_CLASS_BITS,
-                                                      ctx->stream));
-                recs = ctx->d_fix_sorted;
+                CK(hipcub::DeviceRadixSort::SortPairs(ctx->d_ftmp[k], tmp, ctx->d_fk[k], ctx->d_fk2[k], recs,
+                                                      ctx->d_fsorted[k], (int)counts[ty], 0, FIX_CLASS_BITS, st));
+                recs = ctx->d_fsorted[k];
             }
             auto kern = ty == 0 ? k_fix_cp<T, 0> : ty == 1 ? k_fix_cp<T, 1> : ty == 2 ? k_fix_cp<T, 2> : k_fix_cp<T, 3>;
-            kern<<<nblocks(counts[ty]), BLOCK, 0, ctx->stream>>>(recs, counts[ty], f, ctx->d_code_f, g,
-                                                                 ctx->d_code_g, d, c, ctx->d_counter, eps, range,
-                                                                 fl.edit_type, sink());
+            kern<<<nblocks(counts[ty]), BLOCK, 0, st>>>(recs, counts[ty], f, ctx->d_code_f, g, ctx->d_code_g, d, c,
+                                                        ctx->d_counter, eps, range, fl.edit_type, sink());
             CK(hipGetLastError());
+            if (st != ctx->stream) {  // the main stream waits for this type before applying the edits
+                CK(hipEventRecord(ctx->fix_join[k], st));
+                CK(hipStreamWaitEvent(ctx->stream, ctx->fix_join[k], 0));
+            }
         }
         return apply(nrec);
     }
@@ -2723,6 +2749,11 @@ int dmtz_ctx_create(dmtz_ctx** out, int device) {
     if (e == hipSuccess) e = hipHostMalloc(&ctx->h_ctr, sizeof(ull) * K_NCTR);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fix_fork, hipEventDisableTiming);
+    for (int k = 0; k < 3; k++) {
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->fix_stream[k], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fix_join[k], hipEventDisableTiming);
+    }
     if (e != hipSuccess) {
         std::fprintf(stderr, "dmtz_ctx_create: %s\n", hipGetErrorString(e));
         delete ctx;
@@ -2745,7 +2776,16 @@ void dmtz_ctx_destroy(dmtz_ctx* ctx) {
     (void)hipFree(ctx->d_ctr);
     (void)hipFree(ctx->d_fix_key);
     (void)hipFree(ctx->d_fix_key2);
-    (void)hipFree(ctx->d_fix_sorted);
+    for (int k = 0; k < 3; k++) {
+        if (ctx->fix_stream[k]) (void)hipStreamSynchronize(ctx->fix_stream[k]);
+        (void)hipFree(ctx->d_fk[k]);
+        (void)hipFree(ctx->d_fk2[k]);
+        (void)hipFree(ctx->d_fsorted[k]);
+        (void)hipFree(ctx->d_ftmp[k]);
+        if (ctx->fix_join[k]) (void)hipEventDestroy(ctx->fix_join[k]);
+        if (ctx->fix_stream[k]) (void)hipStreamDestroy(ctx->fix_stream[k]);
+    }
+    if (ctx->fix_fork) (void)hipEventDestroy(ctx->fix_fork);
     (void)hipHostFree(ctx->h_ctr);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);

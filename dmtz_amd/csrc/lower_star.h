@@ -47,18 +47,12 @@ struct LowerStar {
     uint64_t tD0, tD1;  // tet chosen in T0 / T1 is via tri_k[t][1] (else tri_k[t][0])
     uint32_t qin;       // tets (24 bits) in the lower star
     uint32_t qT0, qT1;  // tet claimed in T0 / T1
+    uint32_t pe;        // edges paired with the centre (V) or a triangle (E0 / E1)
     uint32_t blw[14];   // blw[i] bit j: n_j below n_i
 
     DMTZ_HD int e0_of(int a) const { return (int)((e0 >> (4 * a)) & 15u); }
     DMTZ_HD int e1_of(int a) const { return (int)((e1 >> (4 * a)) & 15u); }
-    DMTZ_HD uint32_t crit_edges() const {
-        uint32_t m = 0;
-        for (int a = 0; a < 14; a++) {
-            bool c = ((L >> a) & 1u) && a != s && e0_of(a) == 15 && e1_of(a) == 15;
-            m |= (uint32_t)c << a;
-        }
-        return m;
-    }
+    DMTZ_HD uint32_t crit_edges() const { return L & ~pe; }
     DMTZ_HD uint64_t crit_tris() const { return tin & ~(tE0 | tT0 | tE1 | tT1); }
     DMTZ_HD uint32_t crit_tets() const { return qin & ~(qT0 | qT1); }
 };
@@ -215,6 +209,7 @@ DMTZ_HD void lower_star(const T* val, LowerStar& r) {
     const uint64_t before = tE0 | tT0;
     const uint32_t freeE = Ls & ~pe0;
     uint64_t e1 = 0xFFull << 56, tE1 = 0;
+    uint32_t pe1 = 0;  // edges paired in E1
     {
     const detail::PassMasks P(blw, L);
 #pragma unroll
@@ -231,11 +226,13 @@ DMTZ_HD void lower_star(const T* val, LowerStar& r) {
                 const uint32_t ok = bit1(m, k) & (bit1(before, t) ^ 1u);
                 nib ^= (15u ^ (uint32_t)k) & (0u - ok);
                 tE1 |= (uint64_t)ok << t;
+                pe1 |= ok << a;
             }
         }
         e1 |= (uint64_t)nib << (4 * a);
         DMTZ_OPAQUE(e1);
         DMTZ_OPAQUE(tE1);
+        DMTZ_OPAQUE(pe1);
     }
     }
     r.e1 = e1;
@@ -286,6 +283,7 @@ DMTZ_HD void lower_star(const T* val, LowerStar& r) {
     r.qin = qin;
     r.qT0 = qT0;
     r.qT1 = qT1;
+    r.pe = smask | pe0 | pe1;
 }
 
 // Local cell numbering of the 75 cells whose highest vertex is the centre:
@@ -325,40 +323,73 @@ DMTZ_HD int code_tri(const StarCode& c, int t) {
     return t < 21 ? (int)((c.w[2] >> (3 * t)) & 7u) : (int)((c.w[3] >> (3 * (t - 21))) & 7u);
 }
 
+namespace detail {
+
+// Bit i of the low 21 bits of x moved to bit 3i (Morton-code bit spreading).
+DMTZ_HD uint64_t spread3(uint64_t x) {
+    x &= 0x1fffffull;
+    x = (x | x << 32) & 0x1f00000000ffffull;
+    x = (x | x << 16) & 0x1f0000ff0000ffull;
+    x = (x | x << 8) & 0x100f00f00f00f00full;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+    x = (x | x << 2) & 0x1249249249249249ull;
+    return x;
+}
+
+// Bit i of the low 32 bits of x moved to bit 2i.
+DMTZ_HD uint64_t spread2(uint64_t x) {
+    x &= 0xffffffffull;
+    x = (x | x << 16) & 0x0000ffff0000ffffull;
+    x = (x | x << 8) & 0x00ff00ff00ff00ffull;
+    x = (x | x << 4) & 0x0f0f0f0f0f0f0f0full;
+    x = (x | x << 2) & 0x3333333333333333ull;
+    x = (x | x << 1) & 0x5555555555555555ull;
+    return x;
+}
+
+}  // namespace detail
+
+// The code's fields are built as bit planes over all edges / triangles / tets at once; the pass
+// results are mutually exclusive per cell (each cell is paired at most once), so each plane is an
+// OR of masked pass masks.
 DMTZ_HD void encode_star(const LowerStar& r, StarCode& out) {
-    uint64_t w0 = 0, w1 = (uint64_t)r.L, w2 = 0, w3 = 0;
-#pragma unroll
-    for (int a = 0; a < 14; a++) {
-        int e0 = r.e0_of(a), e1 = r.e1_of(a);
-        int nib = a == r.s ? 14 : (e0 != 15 ? e0 : (e1 != 15 ? e1 : 15));
-        nib = ((r.L >> a) & 1u) ? nib : 15;
-        w0 |= (uint64_t)nib << (4 * a);
-    }
+    // Edges: the E0 and E1 nibbles are 15 except where that pass paired the edge (never both),
+    // so the partner nibble is their AND; the steepest slot (15 in both) gets 14.
+    uint64_t w0 = r.e0 & r.e1 & ((1ull << 56) - 1ull);
+    if (r.s >= 0) w0 &= ~(1ull << (4 * r.s));
     w0 |= (uint64_t)(r.s < 0 ? 15 : r.s) << 56;
-#pragma unroll
-    for (int t = 0; t < 36; t++) {
-        bool hi = (r.thi >> t) & 1ull;  // tri_i above tri_j
-        int code = 7;
-        if ((r.tE0 >> t) & 1ull) code = hi ? 0 : 1;        // edge to the higher slot
-        else if ((r.tT0 >> t) & 1ull) code = 2 + (int)((r.tD0 >> t) & 1ull);
-        else if ((r.tE1 >> t) & 1ull) code = hi ? 1 : 0;   // edge to the lower slot
-        else if ((r.tT1 >> t) & 1ull) code = 2 + (int)((r.tD1 >> t) & 1ull);
-        if (t < 21) w2 |= (uint64_t)code << (3 * t);
-        else w3 |= (uint64_t)code << (3 * (t - 21));
-    }
+
+    // Triangles: 0/1 = edge to tri_i / tri_j, 2/3 = tet via tri_k[t][0] / [1], 7 = unpaired.
+    const uint64_t T36 = (1ull << 36) - 1ull;
+    const uint64_t un = ~(r.tE0 | r.tT0 | r.tE1 | r.tT1) & T36;
+    const uint64_t b2 = un;
+    const uint64_t b1 = (r.tT0 | r.tT1 | un) & T36;
+    const uint64_t b0 = ((r.tE0 & ~r.thi) | (r.tE1 & r.thi) | (r.tT0 & r.tD0) | (r.tT1 & r.tD1) | un) & T36;
+    const uint64_t w2 = detail::spread3(b0) | (detail::spread3(b1) << 1) | (detail::spread3(b2) << 2);
+    const uint64_t w3 =
+        detail::spread3(b0 >> 21) | (detail::spread3(b1 >> 21) << 1) | (detail::spread3(b2 >> 21) << 2);
+
+    // Tets: T0 leaves out the lowest slot of (tet_s[q][0], [1], [2]), T1 the middle one; 3 =
+    // unpaired. ji / ki / kj: n_j below n_i, n_k below n_i, n_k below n_j.
+    uint32_t JI = 0, KI = 0, KJ = 0;
 #pragma unroll
     for (int q = 0; q < 24; q++) {
         const int i = ST.tet_s[q][0], j = ST.tet_s[q][1], k = ST.tet_s[q][2];
-        bool ji = (r.blw[i] >> j) & 1u, ki = (r.blw[i] >> k) & 1u, kj = (r.blw[j] >> k) & 1u;
-        // T0: the face without the lowest slot; T1: the face without the middle slot
-        int low = (ki && kj) ? 2 : ((ji && !kj) ? 1 : 0);
-        bool midi = ji != ki, midj = (!ji) != kj;
-        int mid = midi ? 0 : (midj ? 1 : 2);
-        int code = 3;
-        if ((r.qT0 >> q) & 1u) code = low;
-        else if ((r.qT1 >> q) & 1u) code = mid;
-        w1 |= (uint64_t)code << (14 + 2 * q);
+        JI |= ((r.blw[i] >> j) & 1u) << q;
+        KI |= ((r.blw[i] >> k) & 1u) << q;
+        KJ |= ((r.blw[j] >> k) & 1u) << q;
+        DMTZ_OPAQUE(JI);
+        DMTZ_OPAQUE(KI);
+        DMTZ_OPAQUE(KJ);
     }
+    const uint32_t low1 = KI & KJ, low0 = JI & ~KJ;
+    const uint32_t midi = JI ^ KI, midj = ~JI ^ KJ;
+    const uint32_t mid0 = ~midi & midj, mid1 = ~midi & ~midj;
+    const uint32_t fr = ~(r.qT0 | r.qT1) & 0xffffffu;
+    const uint32_t c0 = ((r.qT0 & low0) | (r.qT1 & mid0) | fr) & 0xffffffu;
+    const uint32_t c1 = ((r.qT0 & low1) | (r.qT1 & mid1) | fr) & 0xffffffu;
+    const uint64_t w1 = (uint64_t)r.L | ((detail::spread2(c0) | (detail::spread2(c1) << 1)) << 14);
+
     out.w[0] = w0;
     out.w[1] = w1;
     out.w[2] = w2;
