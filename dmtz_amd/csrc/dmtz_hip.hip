@@ -947,9 +947,9 @@ __global__ __launch_bounds__(BLOCK) void k_as_count_ws(const uint64_t* fcf, ull 
     }
 }
 
-// Steps of branch j at off[j]: copied from the counting walk's first keep steps, or walked again.
-// Thread t takes branch order[t] (branches by path length, so that a wave's walks are alike) or
-// branch t when order is null.
+// Steps of thread t's branch at off[t]: copied from the counting walk's first keep steps, or
+// walked again. Thread t takes branch order[t] (branches by path length, so that a wave's walks
+// are alike; off then holds the offsets in that order) or branch t when order is null.
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_as_fill(const uint64_t* fcf, ull nb, const uint32_t* order, const T* f,
                                                    const StarCode* cf, Dims d, Counts c, const ull* off,
@@ -957,7 +957,7 @@ __global__ __launch_bounds__(BLOCK) void k_as_fill(const uint64_t* fcf, ull nb, 
     const ull t = (ull)blockIdx.x * BLOCK + threadIdx.x;
     if (t >= nb) return;
     const ull j = order ? (ull)order[t] : t;
-    const ull a = off[j], n = off[j + 1] - a;
+    const ull a = off[t], n = off[t + 1] - a;
     if (n <= (ull)keep) {
         for (ull k = 0; k < n; k++) steps[a + k] = first[j * (ull)keep + k];
         return;
@@ -1013,6 +1013,13 @@ __global__ __launch_bounds__(BLOCK) void k_as_compact(ull nb, const ull* off, co
     for (ull s = off[j]; s < off[j + 1]; s++) cube[s] = steps[s].x / 6u;
 }
 
+// Step counts in check order (cnt of branch order[t] at t), and a zero at nb for the scan.
+__global__ __launch_bounds__(BLOCK) void k_as_gather_cnt(ull nb, const uint32_t* order, const ull* cnt, ull* out) {
+    const ull t = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    if (t < nb) out[t] = cnt[order[t]];
+    else if (t == nb) out[nb] = 0;
+}
+
 // Branch order of the checks: branches sorted by step count (capped at 255), so that a wave's
 // walks have similar lengths. Keys and identity order for the sort.
 __global__ __launch_bounds__(BLOCK) void k_as_len_keys(ull nb, const ull* cnt, uint16_t* key, uint32_t* idx) {
@@ -1050,8 +1057,9 @@ __device__ __forceinline__ uint32_t next_dirty(const uint64_t* sd, uint32_t s, u
     return r;
 }
 
-// k_as_check over the compact copies. Thread t checks branch order[t]; results stay indexed by
-// branch. Unless full, a branch with no dirty step keeps its last result, and a branch whose
+// k_as_check over the compact copies. Thread t checks branch order[t], whose steps are at
+// [off32[t], off32[t + 1]) (the copies are laid out in check order, so a wave's offsets and dirty
+// words are adjacent); results stay indexed by branch. Unless full, a branch with no dirty step keeps its last result, and a branch whose
 // last result was "no mismatch" compares only its dirty steps, in order: the others matched then
 // and read unchanged star codes now. A branch that had a mismatch and has a dirty step is walked
 // again from its start.
@@ -1061,13 +1069,15 @@ __global__ __launch_bounds__(BLOCK) void k_as_check32(ull nb, const uint32_t* or
                                                       const StarCode* cg, Dims d, Counts c, int full, int64_t* res) {
     const ull t = (ull)blockIdx.x * BLOCK + threadIdx.x;
     if (t >= nb) return;
-    const ull j = order[t];
-    const uint32_t a = off32[j], b = off32[j + 1];
+    const uint32_t a = off32[t], b = off32[t + 1];
     uint32_t s = a;
     bool sparse = false;  // visit dirty steps only
     if (!full) {
         s = next_dirty(sd, a, b);
         if (s >= b) return;
+    }
+    const ull j = order[t];
+    if (!full) {
         sparse = res[2 * j] < 0;
         if (!sparse) s = a;
     }
@@ -1272,6 +1282,7 @@ struct dmtz_ctx {
     // Repairs of record types 1..3: class keys, sorted records and sort scratch per type, and one
     // stream per type for sweeps with few records, when the types run concurrently.
     hipStream_t fix_stream[3] = {nullptr, nullptr, nullptr};
+    bool fix_concurrent = true;  // off in batch lanes: their own streams already overlap
     hipEvent_t fix_fork = nullptr, fix_join[3] = {nullptr, nullptr, nullptr};
     uint16_t* d_fk[3] = {nullptr, nullptr, nullptr};
     uint16_t* d_fk2[3] = {nullptr, nullptr, nullptr};
@@ -1610,8 +1621,17 @@ struct Driver {
         if (rc) return rc;
         Lists4 L = rec_lists(ctx);
         ull nrec = 0;
-        bool conc = true;
+        bool conc = ctx->fix_concurrent;
         for (int ty = 1; ty < 4; ty++) conc = conc && counts[ty] < FIX_CONCURRENT_MAX;
+        if (conc && !ctx->fix_stream[2]) {
+            // created on first use: streams take the process's hardware queues in turn, so
+            // contexts that never run concurrent repairs (batch lanes) must not create them
+            CK(hipEventCreateWithFlags(&ctx->fix_fork, hipEventDisableTiming));
+            for (int k = 0; k < 3; k++) {
+                CK(hipEventCreateWithFlags(&ctx->fix_join[k], hipEventDisableTiming));
+                CK(hipStreamCreateWithFlags(&ctx->fix_stream[k], hipStreamNonBlocking));
+            }
+        }
         if (conc) CK(hipEventRecord(ctx->fix_fork, ctx->stream));
         for (int ty = 0; ty < 4; ty++) {
             if (counts[ty] == 0) continue;
@@ -2026,6 +2046,14 @@ struct Driver {
             if (rc) return rc;
             CK(hipcub::DeviceRadixSort::SortPairs(ctx->d_scan_tmp, tmp2, ctx->d_fix_key, ctx->d_fix_key2,
                                                   ctx->d_as_order + nb, ctx->d_as_order, (int)nb, 0, 8, ctx->stream));
+            // steps in check order: offsets from the counts taken in that order
+            k_as_gather_cnt<<<nblocks(nb + 1), BLOCK, 0, ctx->stream>>>(nb, ctx->d_as_order, ctx->d_as_cnt,
+                                                                       ctx->d_as_off);
+            CK(hipGetLastError());
+            CK(hipcub::DeviceScan::ExclusiveSum(ctx->d_scan_tmp, tmp, ctx->d_as_off, ctx->d_as_cnt, (int)(nb + 1),
+                                                ctx->stream));
+            std::swap(ctx->d_as_cnt, ctx->d_as_off);
+            std::swap(ctx->as_cnt_cap, ctx->as_off_cap);
         }
         k_as_fill<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(FL.l[2], nb, as_compact ? ctx->d_as_order : nullptr, f,
                                                               ctx->d_code_f, d, c, ctx->d_as_off, ctx->d_as_first,
@@ -2453,6 +2481,7 @@ int correct_batch(dmtz_ctx* ctx, const Field* fields, int32_t n, const dmtz_flag
             ctx->err = "batch lane context creation failed";
             return rc;
         }
+        w->fix_concurrent = false;  // eight lanes with four streams each would share four hardware queues
         ctx->workers.push_back(w);
     }
     for (dmtz_ctx* w : ctx->workers) w->err.clear();
@@ -2749,11 +2778,6 @@ int dmtz_ctx_create(dmtz_ctx** out, int device) {
     if (e == hipSuccess) e = hipHostMalloc(&ctx->h_ctr, sizeof(ull) * K_NCTR);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fix_fork, hipEventDisableTiming);
-    for (int k = 0; k < 3; k++) {
-        if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->fix_stream[k], hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fix_join[k], hipEventDisableTiming);
-    }
     if (e != hipSuccess) {
         std::fprintf(stderr, "dmtz_ctx_create: %s\n", hipGetErrorString(e));
         delete ctx;
