@@ -1611,19 +1611,46 @@ struct Driver {
     }
 
     // Repair phase of one critical-point sweep (main_3d.cu:6801-6833) + apply (:6837).
-    // With few records per type a repair launch takes about the time of its longest chain of
-    // dependent reads, so the types run concurrently (the edits form a set: their order does not
-    // matter). With many records the launches fill the GPU and run one after another.
+    // The repair kinds run concurrently, one stream each (the edits form a set: their order does
+    // not matter), so that one kind's longest chains of dependent reads overlap the others' work.
+    // With few records per kind each kind classifies and sorts on its own stream too; with many,
+    // the sorts run first on the context's stream, because the radix sort's look-back waits on
+    // blocks that would compete with the other kinds' repairs.
     static constexpr ull FIX_CONCURRENT_MAX = 1ull << 15;
+
+    // Group the records of kind ty (1..3) by branch class on stream st; recs then points at the
+    // sorted copy.
+    int sort_class(int ty, hipStream_t st, const uint64_t*& recs, ull n) {
+        const int k = ty - 1;
+        int rc = grow_buf(ctx, ctx->d_fk[k], ctx->fk_cap[k], n);
+        if (rc) return rc;
+        rc = grow_buf(ctx, ctx->d_fsorted[k], ctx->fsorted_cap[k], n);
+        if (rc) return rc;
+        rc = grow_buf(ctx, ctx->d_fk2[k], ctx->fk2_cap[k], n);
+        if (rc) return rc;
+        auto cls = ty == 1 ? k_fix_class<T, 1> : ty == 2 ? k_fix_class<T, 2> : k_fix_class<T, 3>;
+        cls<<<nblocks(n), BLOCK, 0, st>>>(recs, n, f, ctx->d_code_f, g, ctx->d_code_g, ctx->d_critf, d, c,
+                                          ctx->d_fk[k]);
+        CK(hipGetLastError());
+        size_t tmp = 0;
+        CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, ctx->d_fk[k], ctx->d_fk2[k], recs, ctx->d_fsorted[k],
+                                              (int)n, 0, FIX_CLASS_BITS, st));
+        rc = grow_buf(ctx, ctx->d_ftmp[k], ctx->ftmp_cap[k], tmp);
+        if (rc) return rc;
+        CK(hipcub::DeviceRadixSort::SortPairs(ctx->d_ftmp[k], tmp, ctx->d_fk[k], ctx->d_fk2[k], recs,
+                                              ctx->d_fsorted[k], (int)n, 0, FIX_CLASS_BITS, st));
+        recs = ctx->d_fsorted[k];
+        return 0;
+    }
 
     int fix_cp(const ull counts[4]) {
         int rc = begin_edits();
         if (rc) return rc;
         Lists4 L = rec_lists(ctx);
-        ull nrec = 0;
-        bool conc = ctx->fix_concurrent;
-        for (int ty = 1; ty < 4; ty++) conc = conc && counts[ty] < FIX_CONCURRENT_MAX;
-        if (conc && !ctx->fix_stream[2]) {
+        bool small = true;
+        for (int ty = 1; ty < 4; ty++) small = small && counts[ty] < FIX_CONCURRENT_MAX;
+        const bool side = ctx->fix_concurrent;
+        if (side && !ctx->fix_stream[2]) {
             // created on first use: streams take the process's hardware queues in turn, so
             // contexts that never run concurrent repairs (batch lanes) must not create them
             CK(hipEventCreateWithFlags(&ctx->fix_fork, hipEventDisableTiming));
@@ -1632,43 +1659,38 @@ struct Driver {
                 CK(hipStreamCreateWithFlags(&ctx->fix_stream[k], hipStreamNonBlocking));
             }
         }
-        if (conc) CK(hipEventRecord(ctx->fix_fork, ctx->stream));
+        bool active[4];
+        const uint64_t* recs[4];
+        ull nrec = 0;
         for (int ty = 0; ty < 4; ty++) {
-            if (counts[ty] == 0) continue;
-            if ((ty == 1 || ty == 2) && fl.preserve_saddles != 1) continue;
-            nrec += counts[ty];
-            const uint64_t* recs = L.l[ty];
+            active[ty] = counts[ty] != 0 && !((ty == 1 || ty == 2) && fl.preserve_saddles != 1);
+            recs[ty] = L.l[ty];
+            if (active[ty]) nrec += counts[ty];
+        }
+        if (!small)
+            for (int ty = 1; ty < 4; ty++)
+                if (active[ty]) {
+                    rc = sort_class(ty, ctx->stream, recs[ty], counts[ty]);
+                    if (rc) return rc;
+                }
+        if (side) CK(hipEventRecord(ctx->fix_fork, ctx->stream));
+        for (int ty = 0; ty < 4; ty++) {
+            if (!active[ty]) continue;
             const int k = ty - 1;
             hipStream_t st = ctx->stream;
-            if (ty != 0 && conc) {
+            if (ty != 0 && side) {
                 st = ctx->fix_stream[k];
                 CK(hipStreamWaitEvent(st, ctx->fix_fork, 0));
             }
-            if (ty != 0) {  // group the records by branch class first
-                rc = grow_buf(ctx, ctx->d_fk[k], ctx->fk_cap[k], counts[ty]);
+            if (ty != 0 && small) {
+                rc = sort_class(ty, st, recs[ty], counts[ty]);
                 if (rc) return rc;
-                rc = grow_buf(ctx, ctx->d_fsorted[k], ctx->fsorted_cap[k], counts[ty]);
-                if (rc) return rc;
-                rc = grow_buf(ctx, ctx->d_fk2[k], ctx->fk2_cap[k], counts[ty]);
-                if (rc) return rc;
-                auto cls = ty == 1 ? k_fix_class<T, 1> : ty == 2 ? k_fix_class<T, 2> : k_fix_class<T, 3>;
-                cls<<<nblocks(counts[ty]), BLOCK, 0, st>>>(recs, counts[ty], f, ctx->d_code_f, g, ctx->d_code_g,
-                                                           ctx->d_critf, d, c, ctx->d_fk[k]);
-                CK(hipGetLastError());
-                size_t tmp = 0;
-                CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, ctx->d_fk[k], ctx->d_fk2[k], recs,
-                                                      ctx->d_fsorted[k], (int)counts[ty], 0, FIX_CLASS_BITS, st));
-                rc = grow_buf(ctx, ctx->d_ftmp[k], ctx->ftmp_cap[k], tmp);
-                if (rc) return rc;
-                CK(hipcub::DeviceRadixSort::SortPairs(ctx->d_ftmp[k], tmp, ctx->d_fk[k], ctx->d_fk2[k], recs,
-                                                      ctx->d_fsorted[k], (int)counts[ty], 0, FIX_CLASS_BITS, st));
-                recs = ctx->d_fsorted[k];
             }
             auto kern = ty == 0 ? k_fix_cp<T, 0> : ty == 1 ? k_fix_cp<T, 1> : ty == 2 ? k_fix_cp<T, 2> : k_fix_cp<T, 3>;
-            kern<<<nblocks(counts[ty]), BLOCK, 0, st>>>(recs, counts[ty], f, ctx->d_code_f, g, ctx->d_code_g, d, c,
-                                                        ctx->d_counter, eps, range, fl.edit_type, sink());
+            kern<<<nblocks(counts[ty]), BLOCK, 0, st>>>(recs[ty], counts[ty], f, ctx->d_code_f, g, ctx->d_code_g, d,
+                                                        c, ctx->d_counter, eps, range, fl.edit_type, sink());
             CK(hipGetLastError());
-            if (st != ctx->stream) {  // the main stream waits for this type before applying the edits
+            if (st != ctx->stream) {  // the context's stream waits for this kind before applying the edits
                 CK(hipEventRecord(ctx->fix_join[k], st));
                 CK(hipStreamWaitEvent(ctx->stream, ctx->fix_join[k], 0));
             }
