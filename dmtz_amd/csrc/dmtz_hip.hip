@@ -187,9 +187,9 @@ struct __align__(16) CritMask {
     uint32_t hi, pad;
 };
 
-// The lower-star classification is fully unrolled; left alone the compiler spends ~300 VGPRs on
-// it (one wave per SIMD). CODE_BLOCKS_PER_CU bounds the registers so that more waves hide the
-// load latency.
+// The lower-star classification is fully unrolled straight-line code of about 5k instructions
+// needing about 100 VGPRs (lower_star.h keeps its live state small). CODE_BLOCKS_PER_CU = 4 caps
+// the code kernels at 128 VGPRs: four waves per SIMD to hide the loads of the star values.
 constexpr int CODE_BLOCKS_PER_CU = 4;
 
 template <class T>
@@ -1066,7 +1066,8 @@ __device__ __forceinline__ uint32_t next_dirty(const uint64_t* sd, uint32_t s, u
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_as_check32(ull nb, const uint32_t* order, const uint32_t* off32,
                                                       const uint64_t* sd, const uint2* steps, const T* g,
-                                                      const StarCode* cg, Dims d, Counts c, int full, int64_t* res) {
+                                                      const StarCode* cg, Dims d, Counts c, int full, int64_t* res,
+                                                      uint32_t* fails) {
     const ull t = (ull)blockIdx.x * BLOCK + threadIdx.x;
     if (t >= nb) return;
     const uint32_t a = off32[t], b = off32[t + 1];
@@ -1091,15 +1092,20 @@ __global__ __launch_bounds__(BLOCK) void k_as_check32(ull nb, const uint32_t* or
         }
         s = sparse ? next_dirty(sd, s + 1, b) : s + 1;
     }
+    // fails bit j mirrors res[2 * j] >= 0 (zeroed before a full check)
+    const bool was = !full && !sparse, now = found >= 0;
+    if (was != now) atomicXor(fails + (j >> 5), 1u << (j & 31u));
     res[2 * j] = found;
     res[2 * j + 1] = dv;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_as_combine(const int64_t* res, ull n, PathList out) {
+// With fails != null (bit j set when branch j failed), faces with neither branch failing skip the
+// result reads.
+__global__ __launch_bounds__(BLOCK) void k_as_combine(const int64_t* res, ull n, PathList out, const uint32_t* fails) {
     const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
     bool has = false;
     i64 a = -1, b = -1;
-    if (i < n) {
+    if (i < n && (!fails || ((fails[(2 * i) >> 5] >> ((2 * i) & 31u)) & 3u) != 0)) {
         const int64_t* p = res + 4 * i;
         has = p[0] >= 0 || p[2] >= 0;
         a = p[0] >= 0 ? p[0] : p[2];
@@ -1295,7 +1301,8 @@ struct dmtz_ctx {
     uint32_t* d_as_cube = nullptr;
     uint32_t* d_as_order = nullptr;  // branches by step count (first nb), sort input (second nb)
     uint64_t* d_as_sd = nullptr;     // per-step dirty bits
-    size_t as_off32_cap = 0, as_cube_cap = 0, as_order_cap = 0, as_sd_cap = 0;
+    uint32_t* d_as_fails = nullptr;  // per-branch bit: the last check found a mismatch
+    size_t as_off32_cap = 0, as_cube_cap = 0, as_order_cap = 0, as_sd_cap = 0, as_fails_cap = 0;
     uint32_t* d_wall_scr = nullptr;  // wall walks: per-thread visited sets and stacks
     size_t wall_scr_cap = 0;
     uint32_t* d_wall_ovf = nullptr;  // wall walks to repeat with more slots (two lists)
@@ -1377,9 +1384,10 @@ void free_paths(dmtz_ctx* ctx) {
     (void)hipFree(ctx->d_as_cube);
     (void)hipFree(ctx->d_as_order);
     (void)hipFree(ctx->d_as_sd);
-    ctx->d_as_off32 = ctx->d_as_cube = ctx->d_as_order = nullptr;
+    (void)hipFree(ctx->d_as_fails);
+    ctx->d_as_off32 = ctx->d_as_cube = ctx->d_as_order = ctx->d_as_fails = nullptr;
     ctx->d_as_sd = nullptr;
-    ctx->as_off32_cap = ctx->as_cube_cap = ctx->as_order_cap = ctx->as_sd_cap = 0;
+    ctx->as_off32_cap = ctx->as_cube_cap = ctx->as_order_cap = ctx->as_sd_cap = ctx->as_fails_cap = 0;
     (void)hipFree(ctx->d_wall_scr);
     (void)hipFree(ctx->d_wall_ovf);
     ctx->d_wall_scr = ctx->d_wall_ovf = nullptr;
@@ -1917,7 +1925,8 @@ struct Driver {
             CK(hipGetLastError());
         }
         if (nfcf) {
-            k_as_combine<<<nblocks(nfcf), BLOCK, 0, ctx->stream>>>(ctx->d_walk, nfcf, path_list(1));
+            k_as_combine<<<nblocks(nfcf), BLOCK, 0, ctx->stream>>>(ctx->d_walk, nfcf, path_list(1),
+                                                                   pre && as_compact ? ctx->d_as_fails : nullptr);
             CK(hipGetLastError());
         }
         if (nfce && pre) {
@@ -2091,6 +2100,8 @@ struct Driver {
             CK(hipGetLastError());
             rc = grow_buf(ctx, ctx->d_as_sd, ctx->as_sd_cap, (total + 63) / 64 + 1);
             if (rc) return rc;
+            rc = grow_buf(ctx, ctx->d_as_fails, ctx->as_fails_cap, (nb + 31) / 32 + 1);
+            if (rc) return rc;
             as_total = total;
         }
         return 0;
@@ -2125,6 +2136,7 @@ struct Driver {
         const ull nb = 2 * nfcf;
         if (!nb) return 0;
         if (as_compact) {
+            if (as_full) CK(hipMemsetAsync(ctx->d_as_fails, 0, sizeof(uint32_t) * ((nb + 31) / 32), ctx->stream));
             if (!as_full && as_total) {
                 k_as_step_dirty<<<nblocks(as_total), BLOCK, 0, ctx->stream>>>(as_total, ctx->d_as_cube, ctx->d_cubes,
                                                                              ctx->d_as_sd);
@@ -2133,7 +2145,7 @@ struct Driver {
             k_as_check32<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(nb, ctx->d_as_order, ctx->d_as_off32,
                                                                      ctx->d_as_sd, ctx->d_as_steps, g,
                                                                      ctx->d_code_g, d, c, as_full ? 1 : 0,
-                                                                     ctx->d_walk);
+                                                                     ctx->d_walk, ctx->d_as_fails);
         } else
             k_as_check<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(nb, ctx->d_as_off, ctx->d_as_steps, g,
                                                                    ctx->d_code_g, d, c, ctx->d_cubes, as_full ? 1 : 0,
