@@ -1030,18 +1030,29 @@ __global__ __launch_bounds__(BLOCK) void k_as_len_keys(ull nb, const ull* cnt, u
 }
 
 // Per-step dirty bits of the compact path copies: bit s of sd is set when the tet of step s
-// lies in a marked cube. One lane per step and one 64-bit word per wave (a ballot), so the cube
-// ids are read coalesced, 4 bytes per step.
+// lies in a marked cube. Each wave covers 4 x 64 consecutive steps, lane l the steps
+// 64k + l (k = 0..3): the cube ids are read coalesced, 4 bytes per step, with four independent
+// loads in flight per lane, and each 64-step word is one ballot.
+constexpr int SD_WORDS_PER_WAVE = 4;
+
 __global__ __launch_bounds__(BLOCK) void k_as_step_dirty(ull total, const uint32_t* cube, const uint32_t* cubes,
                                                          uint64_t* sd) {
-    const ull s = (ull)blockIdx.x * BLOCK + threadIdx.x;
-    bool dq = false;
-    if (s < total) {
-        const uint32_t q = cube[s];
-        dq = ((cubes[q >> 5] >> (q & 31u)) & 1u) != 0;
+    const ull w = (ull)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
+    const int lane = (int)(threadIdx.x & 63u);
+    const ull base = w * (64 * SD_WORDS_PER_WAVE);
+    uint32_t q[SD_WORDS_PER_WAVE];
+#pragma unroll
+    for (int k = 0; k < SD_WORDS_PER_WAVE; k++) {
+        const ull s = base + 64 * k + lane;
+        q[k] = s < total ? cube[s] : 0u;
     }
-    const uint64_t m = __ballot(dq);
-    if (__lane_id() == 0 && s < total) sd[s >> 6] = m;
+#pragma unroll
+    for (int k = 0; k < SD_WORDS_PER_WAVE; k++) {
+        const ull s = base + 64 * k + lane;
+        const bool dq = s < total && ((cubes[q[k] >> 5] >> (q[k] & 31u)) & 1u) != 0;
+        const uint64_t m = __ballot(dq);
+        if (lane == 0 && base + 64 * k < total) sd[(base >> 6) + k] = m;
+    }
 }
 
 // First step in [s, b) whose dirty bit is set, or b.
@@ -2138,8 +2149,8 @@ struct Driver {
         if (as_compact) {
             if (as_full) CK(hipMemsetAsync(ctx->d_as_fails, 0, sizeof(uint32_t) * ((nb + 31) / 32), ctx->stream));
             if (!as_full && as_total) {
-                k_as_step_dirty<<<nblocks(as_total), BLOCK, 0, ctx->stream>>>(as_total, ctx->d_as_cube, ctx->d_cubes,
-                                                                             ctx->d_as_sd);
+                k_as_step_dirty<<<nblocks((as_total + SD_WORDS_PER_WAVE - 1) / SD_WORDS_PER_WAVE), BLOCK, 0,
+                                  ctx->stream>>>(as_total, ctx->d_as_cube, ctx->d_cubes, ctx->d_as_sd);
                 CK(hipGetLastError());
             }
             k_as_check32<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(nb, ctx->d_as_order, ctx->d_as_off32,
