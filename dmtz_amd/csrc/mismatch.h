@@ -4,7 +4,7 @@
 // The reference tests crit_f(cell) XOR crit_g(cell) for every simplex. A cell's status in a field
 // is decided in the lower star of its highest vertex in that field (its owner), and the owners in
 // f and g can differ. f does not change during the correction, so crit_f is evaluated once for
-// every local cell of every vertex (local_crit); each sweep then judges every cell at its owner in
+// every local cell of every vertex (local_crit); each sweep then tests every cell at its owner in
 // g from the g star code alone (owner_mismatch), with no lookups.
 #pragma once
 #include "gradient.h"

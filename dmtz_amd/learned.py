@@ -3,7 +3,7 @@
 A four-layer 3x3x3 convolutional network, trained on (g, g') pairs that the plain correction
 produces, predicts which vertices the plain loop will edit and by how much. Its prediction is
 applied as a first edit of g (never past the snap of f - eps), and the plain loop then runs to
-zero mismatches as usual, so it stays the judge. Every kernel (features, forward, backward,
+zero mismatches as usual, so it decides the result. Every kernel (features, forward, backward,
 Adam, warm start) is the library's own HIP code; torch only holds device memory.
 """
 import ctypes

@@ -4,7 +4,7 @@
  * network predicts, from the decompressed field g alone, which vertices the plain loop will edit
  * and by how much; the prediction is applied as a first edit of g (never past the snap of
  * f - eps, so |g - f| <= eps still holds), and the plain loop (dmtz_correct_*) then runs from
- * there and remains the judge: the result has zero topology mismatches exactly as without the
+ * there and decides the result: it has zero topology mismatches exactly as without the
  * network.
  *
  * Network: four 3x3x3 convolutions with zero padding, 16 channels (8 used input channels,
