@@ -1,3 +1,8 @@
+// driver.h: device kernels, the context and the host driver of the topology-correction path,
+// shared by the library's translation units (dmtz_core.hip: contexts; dmtz_f32.hip and
+// dmtz_f64.hip: the entry points of include/dmtz.h per field type). Everything here has internal
+// linkage, so each unit compiles the templates it instantiates.
+//
 // MI355X (gfx950) implementation of the DMTz topology-correction path behind include/dmtz.h.
 //
 // Device work (one HIP stream per context):
@@ -24,6 +29,7 @@
 // vertex is the cell's owner in g (in f for f-critical lists) and the local cell numbers the 75
 // cells around a vertex (lower_star.h). Lists are filled with wave-aggregated atomics: the set is
 // deterministic, the order is not, and every consumer is order independent.
+#pragma once
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -2806,217 +2812,3 @@ int eps_from_rel(const T* f, int64_t n, double rel, double* eps_out, double* ran
 }
 
 }  // namespace
-
-// ================================================================================================
-// C ABI
-// ================================================================================================
-extern "C" {
-
-int dmtz_ctx_create(dmtz_ctx** out, int device) {
-    if (!out) return DMTZ_E_ARG;
-    *out = nullptr;
-    dmtz_ctx* ctx = new dmtz_ctx();
-    ctx->device = device;
-    hipError_t e = hipSetDevice(device);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc(&ctx->d_ctr, sizeof(ull) * K_NCTR);
-    if (e == hipSuccess) e = hipHostMalloc(&ctx->h_ctr, sizeof(ull) * K_NCTR);
-    if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
-    if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
-    if (e != hipSuccess) {
-        std::fprintf(stderr, "dmtz_ctx_create: %s\n", hipGetErrorString(e));
-        delete ctx;
-        return DMTZ_E_HIP;
-    }
-    *out = ctx;
-    return 0;
-}
-
-void dmtz_ctx_destroy(dmtz_ctx* ctx) {
-    if (!ctx) return;
-    for (dmtz_ctx* w : ctx->workers) dmtz_ctx_destroy(w);
-    ctx->workers.clear();
-    (void)hipSetDevice(ctx->device);
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    free_bufs(ctx);
-    free_rec(ctx);
-    free_fc(ctx);
-    free_paths(ctx);
-    (void)hipFree(ctx->d_ctr);
-    (void)hipFree(ctx->d_fix_key);
-    (void)hipFree(ctx->d_fix_key2);
-    for (int k = 0; k < 3; k++) {
-        if (ctx->fix_stream[k]) (void)hipStreamSynchronize(ctx->fix_stream[k]);
-        (void)hipFree(ctx->d_fk[k]);
-        (void)hipFree(ctx->d_fk2[k]);
-        (void)hipFree(ctx->d_fsorted[k]);
-        (void)hipFree(ctx->d_ftmp[k]);
-        if (ctx->fix_join[k]) (void)hipEventDestroy(ctx->fix_join[k]);
-        if (ctx->fix_stream[k]) (void)hipStreamDestroy(ctx->fix_stream[k]);
-    }
-    if (ctx->fix_fork) (void)hipEventDestroy(ctx->fix_fork);
-    (void)hipHostFree(ctx->h_ctr);
-    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
-    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
-}
-
-const char* dmtz_last_error(const dmtz_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
-
-int dmtz_eps_from_rel_f32(const float* f, int64_t n, double rel, double* e, double* r) {
-    return eps_from_rel(f, n, rel, e, r);
-}
-int dmtz_eps_from_rel_f64(const double* f, int64_t n, double rel, double* e, double* r) {
-    return eps_from_rel(f, n, rel, e, r);
-}
-
-int dmtz_correct_f32(dmtz_ctx* ctx, const float* f, float* g, int64_t nx, int64_t ny, int64_t nz, double eps,
-                     double range, const dmtz_flags* fl, dmtz_counts* out, uint8_t* counter_out, int32_t* sweeps) {
-    if (!ctx) return DMTZ_E_ARG;
-    (void)hipSetDevice(ctx->device);
-    return correct_host<float>(ctx, f, g, nx, ny, nz, eps, range, fl, out, counter_out, sweeps);
-}
-int dmtz_correct_f64(dmtz_ctx* ctx, const double* f, double* g, int64_t nx, int64_t ny, int64_t nz, double eps,
-                     double range, const dmtz_flags* fl, dmtz_counts* out, uint8_t* counter_out, int32_t* sweeps) {
-    if (!ctx) return DMTZ_E_ARG;
-    (void)hipSetDevice(ctx->device);
-    return correct_host<double>(ctx, f, g, nx, ny, nz, eps, range, fl, out, counter_out, sweeps);
-}
-int dmtz_correct_dev_f32(dmtz_ctx* ctx, const float* f, float* g, int64_t nx, int64_t ny, int64_t nz, double eps,
-                         double range, const dmtz_flags* fl, dmtz_counts* out, uint8_t* counter_out, int32_t* sweeps) {
-    if (!ctx) return DMTZ_E_ARG;
-    (void)hipSetDevice(ctx->device);
-    return correct_dev<float>(ctx, f, g, nx, ny, nz, eps, range, fl, out, counter_out, sweeps);
-}
-int dmtz_correct_dev_f64(dmtz_ctx* ctx, const double* f, double* g, int64_t nx, int64_t ny, int64_t nz, double eps,
-                         double range, const dmtz_flags* fl, dmtz_counts* out, uint8_t* counter_out, int32_t* sweeps) {
-    if (!ctx) return DMTZ_E_ARG;
-    (void)hipSetDevice(ctx->device);
-    return correct_dev<double>(ctx, f, g, nx, ny, nz, eps, range, fl, out, counter_out, sweeps);
-}
-
-int64_t dmtz_trajectory(const dmtz_ctx* ctx, int64_t* out, int64_t cap) {
-    if (!ctx) return 0;
-    int64_t n = (int64_t)ctx->traj.size() / 8;
-    if (out)
-        for (int64_t i = 0; i < n && i < cap; i++)
-            for (int k = 0; k < 8; k++) out[i * 8 + k] = ctx->traj[(size_t)i * 8 + k];
-    return n;
-}
-
-int dmtz_phase_times(const dmtz_ctx* ctx, double* out6) {
-    if (!ctx || !out6) return DMTZ_E_ARG;
-    for (int k = 0; k < 6; k++) out6[k] = ctx->phase[k];
-    return 0;
-}
-
-int dmtz_kernel_stats(dmtz_ctx* ctx, double* out4, int reset) {
-    if (!ctx || !out4) return DMTZ_E_ARG;
-    out4[0] = ctx->det_launches;
-    out4[1] = ctx->det_ms;
-    out4[2] = ctx->det_bytes;
-    out4[3] = ctx->det_voxels;
-    if (reset) ctx->det_launches = ctx->det_ms = ctx->det_bytes = ctx->det_voxels = 0;
-    return 0;
-}
-
-int dmtz_correct_batch_f32(dmtz_ctx* ctx, const dmtz_field_f32* fields, int32_t nfields, const dmtz_flags* flags,
-                           dmtz_counts* counts_out, int32_t* status_out, int32_t* sweeps_out) {
-    if (!ctx) return DMTZ_E_ARG;
-    (void)hipSetDevice(ctx->device);
-    return correct_batch<float>(ctx, fields, nfields, flags, counts_out, status_out, sweeps_out);
-}
-int dmtz_correct_batch_f64(dmtz_ctx* ctx, const dmtz_field_f64* fields, int32_t nfields, const dmtz_flags* flags,
-                           dmtz_counts* counts_out, int32_t* status_out, int32_t* sweeps_out) {
-    if (!ctx) return DMTZ_E_ARG;
-    (void)hipSetDevice(ctx->device);
-    return correct_batch<double>(ctx, fields, nfields, flags, counts_out, status_out, sweeps_out);
-}
-
-int dmtz_classify_f32(dmtz_ctx* ctx, const float* field, int64_t nx, int64_t ny, int64_t nz, int64_t* counts4,
-                      uint64_t* crit_lo, uint32_t* crit_hi) {
-    if (!ctx) return DMTZ_E_ARG;
-    (void)hipSetDevice(ctx->device);
-    return classify_host<float>(ctx, field, nx, ny, nz, counts4, crit_lo, crit_hi);
-}
-int dmtz_classify_f64(dmtz_ctx* ctx, const double* field, int64_t nx, int64_t ny, int64_t nz, int64_t* counts4,
-                      uint64_t* crit_lo, uint32_t* crit_hi) {
-    if (!ctx) return DMTZ_E_ARG;
-    (void)hipSetDevice(ctx->device);
-    return classify_host<double>(ctx, field, nx, ny, nz, counts4, crit_lo, crit_hi);
-}
-
-int dmtz_run_info(const dmtz_ctx* ctx, int64_t* out4) {
-    if (!ctx || !out4) return DMTZ_E_ARG;
-    out4[0] = ctx->stalls;
-    out4[1] = ctx->finished;
-    out4[2] = ctx->radius;
-    out4[3] = (int64_t)ctx->dirty_total;
-    return 0;
-}
-
-int dmtz_gradient_f32(dmtz_ctx* ctx, const float* fld, int64_t nx, int64_t ny, int64_t nz, int64_t* pv, int64_t* pe,
-                      int64_t* pf, int64_t* pt) {
-    if (!ctx) return DMTZ_E_ARG;
-    (void)hipSetDevice(ctx->device);
-    return gradient_host<float>(ctx, fld, nx, ny, nz, pv, pe, pf, pt);
-}
-int dmtz_gradient_f64(dmtz_ctx* ctx, const double* fld, int64_t nx, int64_t ny, int64_t nz, int64_t* pv, int64_t* pe,
-                      int64_t* pf, int64_t* pt) {
-    if (!ctx) return DMTZ_E_ARG;
-    (void)hipSetDevice(ctx->device);
-    return gradient_host<double>(ctx, fld, nx, ny, nz, pv, pe, pf, pt);
-}
-
-int dmtz_targets_f32(dmtz_ctx* ctx, const float* f, const float* g, int64_t nx, int64_t ny, int64_t nz, double eps,
-                     double range, const dmtz_flags* fl, uint8_t* targets, int64_t* counts4) {
-    if (!ctx) return DMTZ_E_ARG;
-    (void)hipSetDevice(ctx->device);
-    return targets_host<float>(ctx, f, g, nx, ny, nz, eps, range, fl, targets, counts4);
-}
-int dmtz_targets_f64(dmtz_ctx* ctx, const double* f, const double* g, int64_t nx, int64_t ny, int64_t nz, double eps,
-                     double range, const dmtz_flags* fl, uint8_t* targets, int64_t* counts4) {
-    if (!ctx) return DMTZ_E_ARG;
-    (void)hipSetDevice(ctx->device);
-    return targets_host<double>(ctx, f, g, nx, ny, nz, eps, range, fl, targets, counts4);
-}
-
-int dmtz_paths_f32(dmtz_ctx* ctx, const float* f, const float* g, int64_t nx, int64_t ny, int64_t nz,
-                   const dmtz_flags* fl, int64_t* counts2, int64_t* ds_id, int64_t* ds_rec, int64_t* as_id,
-                   int64_t* as_rec, int64_t cap, uint8_t* targets) {
-    if (!ctx) return DMTZ_E_ARG;
-    (void)hipSetDevice(ctx->device);
-    return paths_host<float>(ctx, f, g, nx, ny, nz, fl, counts2, ds_id, ds_rec, as_id, as_rec, cap, targets);
-}
-int dmtz_paths_f64(dmtz_ctx* ctx, const double* f, const double* g, int64_t nx, int64_t ny, int64_t nz,
-                   const dmtz_flags* fl, int64_t* counts2, int64_t* ds_id, int64_t* ds_rec, int64_t* as_id,
-                   int64_t* as_rec, int64_t cap, uint8_t* targets) {
-    if (!ctx) return DMTZ_E_ARG;
-    (void)hipSetDevice(ctx->device);
-    return paths_host<double>(ctx, f, g, nx, ny, nz, fl, counts2, ds_id, ds_rec, as_id, as_rec, cap, targets);
-}
-
-int dmtz_walls_f32(dmtz_ctx* ctx, const float* f, const float* g, int64_t nx, int64_t ny, int64_t nz, double eps,
-                   double range, const dmtz_flags* flags, int64_t* count, int64_t* face, int64_t* edge, int64_t cap,
-                   uint8_t* targets) {
-    if (!ctx) return DMTZ_E_ARG;
-    (void)hipSetDevice(ctx->device);
-    return walls_host<float>(ctx, f, g, nx, ny, nz, eps, range, flags, count, face, edge, cap, targets);
-}
-int dmtz_walls_f64(dmtz_ctx* ctx, const double* f, const double* g, int64_t nx, int64_t ny, int64_t nz, double eps,
-                   double range, const dmtz_flags* flags, int64_t* count, int64_t* face, int64_t* edge, int64_t cap,
-                   uint8_t* targets) {
-    if (!ctx) return DMTZ_E_ARG;
-    (void)hipSetDevice(ctx->device);
-    return walls_host<double>(ctx, f, g, nx, ny, nz, eps, range, flags, count, face, edge, cap, targets);
-}
-
-}  // extern "C"
-
-// Context internals for the library's other translation unit (learned.hip).
-namespace dmtz_internal {
-hipStream_t stream(dmtz_ctx* ctx) { return ctx->stream; }
-int device(dmtz_ctx* ctx) { return ctx->device; }
-void set_error(dmtz_ctx* ctx, const char* msg) { ctx->err = msg; }
-}  // namespace dmtz_internal
