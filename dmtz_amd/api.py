@@ -19,8 +19,10 @@ class DmtzError(RuntimeError):
     pass
 
 
-# Non-negative status codes of a correction (include/dmtz.h).
+# Non-negative status codes of a correction (include/dmtz.h), and the batch status of a field
+# that never started because another field failed.
 CONVERGED, NOT_CONVERGED, STALLED = 0, 1, 2
+NOT_RUN = -6
 
 
 @dataclass
@@ -150,11 +152,50 @@ def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
+def _same_shape(f, g, what):
+    if tuple(g.shape) != tuple(f.shape):
+        raise ValueError("%s: g has shape %s, f has %s" % (what, tuple(g.shape), tuple(f.shape)))
+    if len(f.shape) != 3:
+        raise ValueError("%s: fields must be 3-D [z, y, x]" % what)
+
+
+def _single_runtime():
+    """Torch tensors are handed to the library only when torch and the library run on one HIP
+    runtime. torch bundles its own libamdhip64; when torch is imported first, the library binds
+    to that copy (same soname), otherwise the process holds two runtimes whose streams do not
+    order against each other."""
+    paths = set()
+    with open("/proc/self/maps") as fh:
+        for line in fh:
+            parts = line.split()
+            if len(parts) >= 6 and "libamdhip64.so" in parts[-1]:
+                paths.add(parts[-1])
+    if len(paths) > 1:
+        raise DmtzError("two HIP runtimes are loaded (%s): import torch before dmtz_amd, or pass "
+                        "dmtz_amd.devmem.DeviceArray buffers instead of torch tensors" % ", ".join(sorted(paths)))
+
+
+def _device_arg(x):
+    """(pointer, shape, dtype tag) of a device buffer: a devmem.DeviceArray or a contiguous cuda
+    torch tensor (whose pending work is finished first)."""
+    if hasattr(x, "ptr") and hasattr(x, "dtype") and not hasattr(x, "data_ptr"):
+        tag = {np.dtype(np.float32): "f32", np.dtype(np.float64): "f64", np.dtype(np.uint8): "u8"}
+        return x.ptr, tuple(x.shape), tag.get(np.dtype(x.dtype), "other")
+    import torch
+
+    assert x.is_cuda and x.is_contiguous()
+    _single_runtime()
+    torch.cuda.synchronize(x.device)
+    tag = {torch.float32: "f32", torch.float64: "f64", torch.uint8: "u8"}.get(x.dtype, "other")
+    return x.data_ptr(), tuple(x.shape), tag
+
+
 def correct(f, g, eps, rng, flags=None, ctx=None, return_counter=False):
     """Correct g toward f within eps until its discrete gradient matches f's.
 
     f, g: numpy arrays indexed [z, y, x] (x fastest), same float dtype. Returns a
     CorrectionResult with the corrected copy of g."""
+    _same_shape(f, g, "correct")
     ctx = ctx or default_context()
     flags = flags or CorrectionFlags()
     f = np.ascontiguousarray(f)
@@ -173,26 +214,28 @@ def correct(f, g, eps, rng, flags=None, ctx=None, return_counter=False):
 
 
 def correct_device(f, g, eps, rng, flags=None, ctx=None, counter_out=None):
-    """Correct device-resident torch tensors in place (g is modified). f, g: cuda tensors
-    [z, y, x] of float32 or float64 on the context's device. counter_out: optional uint8 cuda
-    tensor of the same shape."""
-    import torch
-
+    """Correct device-resident fields in place (g is modified). f, g: devmem.DeviceArray buffers
+    or contiguous cuda torch tensors [z, y, x] of float32 or float64 on the context's device.
+    counter_out: optional uint8 device buffer of the same shape."""
     ctx = ctx or default_context()
     flags = flags or CorrectionFlags()
-    assert f.is_cuda and g.is_cuda and f.is_contiguous() and g.is_contiguous()
-    assert f.dtype == g.dtype and f.shape == g.shape
-    tag = {torch.float32: "f32", torch.float64: "f64"}[f.dtype]
-    nz, ny, nx = f.shape
+    pf, shape, tag = _device_arg(f)
+    pg, gshape, gtag = _device_arg(g)
+    if gshape != shape or gtag != tag or tag not in ("f32", "f64") or len(shape) != 3:
+        raise ValueError("correct_device: f and g must be 3-D float32 or float64 buffers of one shape")
+    pc = None
+    if counter_out is not None:
+        pc, cshape, _ = _device_arg(counter_out)
+        if int(np.prod(cshape)) != int(np.prod(shape)):
+            raise ValueError("correct_device: counter_out must hold one byte per vertex")
+    nz, ny, nx = shape
     counts = N.Counts()
     sweeps = ctypes.c_int32(0)
     fn = getattr(ctx._lib, "dmtz_correct_dev_" + tag)
     fl = flags.c()
-    torch.cuda.synchronize(f.device)
-    rc = fn(ctx.handle, ctypes.c_void_p(f.data_ptr()), ctypes.c_void_p(g.data_ptr()), nx, ny, nz,
+    rc = fn(ctx.handle, ctypes.c_void_p(pf), ctypes.c_void_p(pg), nx, ny, nz,
             float(eps), float(rng), ctypes.byref(fl), ctypes.byref(counts),
-            ctypes.c_void_p(counter_out.data_ptr()) if counter_out is not None else None,
-            ctypes.byref(sweeps))
+            ctypes.c_void_p(pc) if pc is not None else None, ctypes.byref(sweeps))
     ctx.check(rc, "dmtz_correct_dev")
     return CorrectionResult(g, counts.as_tuple(), int(sweeps.value), rc, None, ctx.trajectory(),
                             ctx.phases(), ctx.run_info())
@@ -217,6 +260,7 @@ def gradient(field_, ctx=None):
 
 def targets(f, g, eps, rng, flags=None, ctx=None):
     """One detection + repair phase: (edit-set mask [z,y,x] bool, counts4)."""
+    _same_shape(f, g, "targets")
     ctx = ctx or default_context()
     flags = flags or CorrectionFlags()
     f = np.ascontiguousarray(f)
@@ -245,6 +289,7 @@ def eps_from_rel(f, rel):
 def paths(f, g, flags=None, ctx=None, with_targets=False):
     """Separatrix mismatch lists on (f, g): dict(ds=(edge ids), ds_rec, as_=(tet ids), as_rec,
     targets=edit-set mask or None)."""
+    _same_shape(f, g, "paths")
     ctx = ctx or default_context()
     flags = flags or CorrectionFlags(preserve_vpath=1, preserve_geometry=1)
     f = np.ascontiguousarray(f)
@@ -268,6 +313,7 @@ def walls(f, g, eps=0.0, rng=0.0, flags=None, ctx=None, with_targets=False):
     dict(face=(f-critical face ids), edge=(first edge whose f and g partners differ),
     targets=edit set of fix_ds_wall main_3d.cu:5297-5413 or None). eps and range enter the
     repair's threshold only."""
+    _same_shape(f, g, "walls")
     ctx = ctx or default_context()
     f = np.ascontiguousarray(f)
     g = np.ascontiguousarray(g, dtype=f.dtype)
@@ -305,14 +351,20 @@ def classify(field_, ctx=None, masks=False):
 
 def correct_batch(items, flags=None, ctx=None):
     """Correct several fields with the same flags in one call. items: list of (f, g, eps, rng),
-    either numpy arrays (g is copied, not modified) or contiguous cuda tensors on the context's
-    device (g is corrected in place, no copies). Returns a list of CorrectionResult (without
-    trajectories)."""
-    ctx = ctx or default_context()
-    flags = flags or CorrectionFlags()
+    either numpy arrays (g is copied, not modified) or device buffers on the context's device
+    (devmem.DeviceArray or contiguous cuda tensors; g is corrected in place, no copies). Returns a
+    list of CorrectionResult (without trajectories)."""
     if not items:
         return []
-    if getattr(items[0][0], "is_cuda", False):
+    if isinstance(items[0][0], np.ndarray):
+        dt0 = np.asarray(items[0][0]).dtype
+        for f, g, _, _ in items:
+            if np.asarray(f).dtype != dt0 or np.asarray(g).dtype != dt0:
+                raise TypeError("correct_batch: every field must have the dtype of the first (%s)" % dt0)
+            _same_shape(f, g, "correct_batch")
+    ctx = ctx or default_context()
+    flags = flags or CorrectionFlags()
+    if not isinstance(items[0][0], np.ndarray):
         return _correct_batch_device(items, flags, ctx)
     dt = np.ascontiguousarray(items[0][0]).dtype
     tag = _tag(dt)
@@ -332,31 +384,42 @@ def correct_batch(items, flags=None, ctx=None):
     rc = getattr(ctx._lib, "dmtz_correct_batch_" + tag)(ctx.handle, arr, n, ctypes.byref(fl),
                                                          ctypes.cast(counts, ctypes.c_void_p),
                                                          _p(status), _p(sweeps))
-    ctx.check(rc, "dmtz_correct_batch")
-    return [CorrectionResult(keep[i][1], counts[i].as_tuple(), int(sweeps[i]), int(status[i]))
-            for i in range(n)]
+    return _batch_results(ctx, rc, [keep[i][1] for i in range(n)], counts, status, sweeps)
 
 
 def _correct_batch_device(items, flags, ctx):
-    import torch
-
-    tag = {torch.float32: "f32", torch.float64: "f64"}[items[0][0].dtype]
+    """Batch over device buffers (devmem.DeviceArray or cuda torch tensors), corrected in place."""
+    tag = _device_arg(items[0][0])[2]
+    if tag not in ("f32", "f64"):
+        raise TypeError("correct_batch: fields must be float32 or float64")
     FieldT = N.FieldF32 if tag == "f32" else N.FieldF64
     arr = (FieldT * len(items))()
     for i, (f, g, eps, rng) in enumerate(items):
-        assert f.is_cuda and g.is_cuda and f.is_contiguous() and g.is_contiguous()
-        assert f.dtype == g.dtype and f.shape == g.shape and f.dim() == 3
-        nz, ny, nx = f.shape
-        arr[i] = FieldT(f.data_ptr(), g.data_ptr(), nx, ny, nz, float(eps), float(rng), 1, 0)
+        pf, shape, tf = _device_arg(f)
+        pg, gshape, tg = _device_arg(g)
+        if tf != tag or tg != tag:
+            raise TypeError("correct_batch: every field must have the dtype of the first (%s)" % tag)
+        if gshape != shape or len(shape) != 3:
+            raise ValueError("correct_batch: f and g of field %d must be 3-D of one shape" % i)
+        nz, ny, nx = shape
+        arr[i] = FieldT(pf, pg, nx, ny, nz, float(eps), float(rng), 1, 0)
     n = len(items)
     counts = (N.Counts * n)()
     status = np.zeros(n, np.int32)
     sweeps = np.zeros(n, np.int32)
     fl = flags.c()
-    torch.cuda.synchronize(items[0][0].device)
     rc = getattr(ctx._lib, "dmtz_correct_batch_" + tag)(ctx.handle, arr, n, ctypes.byref(fl),
                                                          ctypes.cast(counts, ctypes.c_void_p),
                                                          _p(status), _p(sweeps))
-    ctx.check(rc, "dmtz_correct_batch")
-    return [CorrectionResult(items[i][1], counts[i].as_tuple(), int(sweeps[i]), int(status[i]))
-            for i in range(n)]
+    return _batch_results(ctx, rc, [items[i][1] for i in range(n)], counts, status, sweeps)
+
+
+def _batch_results(ctx, rc, gs, counts, status, sweeps):
+    """Per-field results of a batch call. On an error the DmtzError carries them as .results:
+    fields that never started have status NOT_RUN (include/dmtz.h DMTZ_E_NOT_RUN)."""
+    out = [CorrectionResult(gs[i], counts[i].as_tuple(), int(sweeps[i]), int(status[i])) for i in range(len(gs))]
+    if rc < 0:
+        err = DmtzError("dmtz_correct_batch failed (%d): %s" % (rc, ctx.error()))
+        err.results = out
+        raise err
+    return out

@@ -1601,11 +1601,12 @@ struct Driver {
         if (rc) return rc;
         float ms = 0;
         CK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-        // algorithmic bytes: f and g read once + one 8-byte record per mismatch emitted
+        // bytes the kernel must move: one mismatch mask per voxel read + one 8-byte record
+        // written per mismatch emitted
         ull m = ctx->h_ctr[0] + ctx->h_ctr[1] + ctx->h_ctr[2] + ctx->h_ctr[3];
         ctx->det_launches += 1;
         ctx->det_ms += ms;
-        ctx->det_bytes += (double)c.N * 2.0 * sizeof(T) + 8.0 * (double)m;
+        ctx->det_bytes += (double)c.N * (double)sizeof(CritMask) + 8.0 * (double)m;
         ctx->det_voxels += (double)c.N;
         return 0;
     }
@@ -2541,7 +2542,7 @@ int correct_batch(dmtz_ctx* ctx, const Field* fields, int32_t n, const dmtz_flag
     std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
         return fields[a].nx * fields[a].ny * fields[a].nz > fields[b].nx * fields[b].ny * fields[b].nz;
     });
-    std::vector<int> rcs(n, 0);
+    std::vector<int> rcs(n, DMTZ_E_NOT_RUN);  // fields that never start keep this status
     std::vector<dmtz_counts> cnts(n);
     std::vector<int32_t> sws(n, 0);
     std::atomic<int32_t> next{0};
@@ -2570,16 +2571,21 @@ int correct_batch(dmtz_ctx* ctx, const Field* fields, int32_t n, const dmtz_flag
         ctx->det_voxels += w->det_voxels;
         w->det_launches = w->det_ms = w->det_bytes = w->det_voxels = 0;
     }
+    int first_err = 0;  // first error in field order (DMTZ_E_NOT_RUN only follows an error)
     for (int32_t i = 0; i < n; i++) {
         if (status_out) status_out[i] = rcs[i];
         if (rcs[i] < 0) {
-            for (dmtz_ctx* w : ctx->workers)
-                if (!w->err.empty()) ctx->err = w->err;
-            return rcs[i];
+            if (rcs[i] != DMTZ_E_NOT_RUN && first_err == 0) first_err = rcs[i];
+            continue;
         }
         if (counts_out) counts_out[i] = cnts[i];
         if (sweeps_out) sweeps_out[i] = sws[i];
         if (rcs[i] != 0) worst = 1;
+    }
+    if (first_err) {
+        for (dmtz_ctx* w : ctx->workers)
+            if (!w->err.empty()) ctx->err = w->err;
+        return first_err;
     }
     return worst;
 }

@@ -323,8 +323,9 @@ __global__ __launch_bounds__(BLOCK) void k_to_bf16(const float* p, int n, uint16
 
 // Warm start: vertices with a positive edit logit move down by size * eps (size clamped to
 // [0, 2]), never below the snap of f - eps.
+// warm[v] records which vertices it moved.
 __global__ __launch_bounds__(BLOCK) void k_warm(const float* f, float* g, const float* Z, long n, double eps,
-                                                unsigned long long* edits) {
+                                                unsigned long long* edits, uint8_t* warm) {
     const long v = (long)blockIdx.x * BLOCK + threadIdx.x;
     bool ed = false;
     if (v < n && Z[v * 2] > 0.f) {
@@ -337,8 +338,17 @@ __global__ __launch_bounds__(BLOCK) void k_warm(const float* f, float* g, const 
         ed = !(c == g[v]);
         g[v] = c;
     }
+    if (v < n) warm[v] = ed ? 1 : 0;
     const unsigned long long b = __ballot(ed ? 1 : 0);
     if (__lane_id() == 0 && b) atomicAdd(edits, (unsigned long long)__popcll(b));
+}
+
+// A warm edit moves a vertex by an amount that no step count of the edit stream expresses, so
+// such vertices are reported with counter 6, whose value the stream carries explicitly
+// (main_3d.cu:1502-1508); counters 6 and 7 (restored to f) already carry theirs.
+__global__ __launch_bounds__(BLOCK) void k_warm_counter(const uint8_t* warm, long n, uint8_t* counter) {
+    const long v = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (v < n && warm[v] && counter[v] < 6) counter[v] = 6;
 }
 
 inline unsigned blocks_for(long n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
@@ -362,6 +372,7 @@ struct dmtz_net {
     uint16_t* grad[2] = {nullptr, nullptr};
     uint16_t* feat = nullptr;
     float* z = nullptr;
+    uint8_t* warm = nullptr;  // vertices moved by the last warm start
 };
 
 namespace {
@@ -380,8 +391,10 @@ void free_acts(dmtz_net* net) {
     for (auto& p : net->grad) (void)hipFree(p), p = nullptr;
     (void)hipFree(net->feat);
     (void)hipFree(net->z);
+    (void)hipFree(net->warm);
     net->feat = nullptr;
     net->z = nullptr;
+    net->warm = nullptr;
     net->cap = 0;
 }
 
@@ -393,6 +406,7 @@ int ensure_acts(dmtz_ctx* ctx, dmtz_net* net, long n) {
     for (auto& p : net->grad) NK(hipMalloc(&p, b));
     NK(hipMalloc(&net->feat, b));
     NK(hipMalloc(&net->z, sizeof(float) * COUT_LAST * (size_t)n));
+    NK(hipMalloc(&net->warm, (size_t)n));
     net->cap = n;
     return 0;
 }
@@ -632,13 +646,18 @@ int dmtz_correct_learned_f32(dmtz_ctx* ctx, dmtz_net* net, const float* d_f, flo
     if (rc) return rc;
     hipStream_t s = dmtz_internal::stream(ctx);
     NK(hipMemsetAsync(net->edits, 0, sizeof(unsigned long long), s));
-    k_warm<<<blocks_for(n), BLOCK, 0, s>>>(d_f, d_g, net->z, n, eps, net->edits);
+    k_warm<<<blocks_for(n), BLOCK, 0, s>>>(d_f, d_g, net->z, n, eps, net->edits, net->warm);
     NK(hipGetLastError());
     unsigned long long ne = 0;
     NK(hipMemcpyAsync(&ne, net->edits, sizeof(ne), hipMemcpyDeviceToHost, s));
     NK(hipStreamSynchronize(s));
     if (warm_edits_out) *warm_edits_out = (int64_t)ne;
-    return dmtz_correct_dev_f32(ctx, d_f, d_g, nx, ny, nz, eps, range, flags, out, d_counter_out, sweeps_out);
+    const int st = dmtz_correct_dev_f32(ctx, d_f, d_g, nx, ny, nz, eps, range, flags, out, d_counter_out, sweeps_out);
+    if (st < 0 || !d_counter_out) return st;
+    k_warm_counter<<<blocks_for(n), BLOCK, 0, s>>>(net->warm, n, d_counter_out);
+    NK(hipGetLastError());
+    NK(hipStreamSynchronize(s));
+    return st;
 }
 
 }  // extern "C"

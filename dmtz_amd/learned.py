@@ -11,7 +11,7 @@ import ctypes
 import numpy as np
 
 from . import _native as N
-from .api import CorrectionFlags, CorrectionResult, correct_device, default_context
+from .api import CorrectionFlags, CorrectionResult, _device_arg, correct_device, default_context
 
 PARAMS = 4 * 6928
 CHANNELS = 16
@@ -123,22 +123,23 @@ def training_pair(net, f, g, eps, rng, flags=None, ctx=None):
 
 
 def correct_learned(net, f, g, eps, rng, flags=None, counter_out=None):
-    """Warm-started correction of cuda tensors f, g (g corrected in place). Returns
-    (CorrectionResult, number of warm-start edits)."""
-    import torch
-
+    """Warm-started correction of float32 device fields f, g (devmem.DeviceArray or cuda torch
+    tensors; g corrected in place). Returns (CorrectionResult, number of warm-start edits)."""
     ctx = net.ctx
     flags = flags or CorrectionFlags()
-    assert f.is_cuda and g.is_cuda and f.dtype == torch.float32 and g.dtype == torch.float32
-    nz, ny, nx = f.shape
+    pf, shape, tf = _device_arg(f)
+    pg, gshape, tg = _device_arg(g)
+    if tf != "f32" or tg != "f32" or gshape != shape or len(shape) != 3:
+        raise ValueError("correct_learned: f and g must be 3-D float32 device buffers of one shape")
+    pc = _device_arg(counter_out)[0] if counter_out is not None else None
+    nz, ny, nx = shape
     counts = N.Counts()
     sweeps = ctypes.c_int32(0)
     warm = ctypes.c_int64(0)
     fl = flags.c()
-    torch.cuda.synchronize(f.device)
-    rc = net._lib.dmtz_correct_learned_f32(ctx.handle, net.handle, _ptr(f), _ptr(g), nx, ny, nz, float(eps),
-                                           float(rng), ctypes.byref(fl), ctypes.byref(counts),
-                                           _ptr(counter_out) if counter_out is not None else None,
+    rc = net._lib.dmtz_correct_learned_f32(ctx.handle, net.handle, ctypes.c_void_p(pf), ctypes.c_void_p(pg), nx, ny,
+                                           nz, float(eps), float(rng), ctypes.byref(fl), ctypes.byref(counts),
+                                           ctypes.c_void_p(pc) if pc is not None else None,
                                            ctypes.byref(sweeps), ctypes.byref(warm))
     ctx.check(rc, "dmtz_correct_learned")
     res = CorrectionResult(g, counts.as_tuple(), int(sweeps.value), rc, None, ctx.trajectory(), ctx.phases(),

@@ -36,6 +36,8 @@ extern "C" {
 #define DMTZ_E_CAPACITY (-3)
 #define DMTZ_E_UNSUPPORTED (-4)
 #define DMTZ_E_WALK (-5)
+/* Batch status of a field that never ran because an earlier field failed with an error. */
+#define DMTZ_E_NOT_RUN (-6)
 
 /* The reference's positional flags (main_3d.cu:6264-6271) plus a sweep cap and a finisher.
  * max_sweeps <= 0 selects the default cap (100000). mode: 0 = plain loop.
@@ -63,7 +65,11 @@ typedef struct dmtz_flags {
 } dmtz_flags;
 
 /* Mismatch counters count_f_* (main_3d.cu:68-75) in the order min, 1-saddle, 2-saddle, max,
- * descending path, ascending path, descending wall, ascending wall. */
+ * descending path, ascending path, descending wall, ascending wall. As in the reference
+ * (main_3d.cu:6767-6770), ds_vpath, as_vpath and ds_wall start at 1 and keep that value until the
+ * first separatrix check runs, which happens only once the critical-cell counts reach zero: a
+ * run with preserve_vpath = 1 that stops earlier (max_sweeps or a stall) reports 1 there,
+ * meaning "not checked yet", not a measured count. Runs with preserve_vpath = 0 report 0. */
 typedef struct dmtz_counts {
     int64_t min, saddle1, saddle2, max, ds_vpath, as_vpath, ds_wall, as_wall;
 } dmtz_counts;
@@ -126,9 +132,11 @@ typedef struct dmtz_field_f64 {
  * 8 lanes: host threads with worker contexts of their own (stream and buffers, kept by ctx for
  * later batches), so that their sweeps overlap on the device. Each field's result is exactly
  * that of a single correction. counts_out, status_out and sweeps_out (each nullable) receive one
- * entry per field; status_out[i] is field i's return code. Returns 0 when every field converged,
- * 1 when one did not (not converged or stalled), or the first negative error in field order
- * (fields not yet started then stay untouched). */
+ * entry per field; status_out[i] is field i's return code, or DMTZ_E_NOT_RUN for a field that
+ * never started because another field failed with an error (lanes stop taking fields then).
+ * counts_out and sweeps_out are written only for fields that ran to a non-negative status; the
+ * field g of a field that never started stays untouched. Returns 0 when every field converged,
+ * 1 when one did not (not converged or stalled), or the first negative error in field order. */
 int dmtz_correct_batch_f32(dmtz_ctx* ctx, const dmtz_field_f32* fields, int32_t nfields, const dmtz_flags* flags,
                            dmtz_counts* counts_out, int32_t* status_out, int32_t* sweeps_out);
 int dmtz_correct_batch_f64(dmtz_ctx* ctx, const dmtz_field_f64* fields, int32_t nfields, const dmtz_flags* flags,
@@ -158,8 +166,9 @@ int dmtz_phase_times(const dmtz_ctx* ctx, double* out6);
 
 /* Live statistics of the dense mismatch-detection kernel (the per-sweep classification of
  * main_3d.cu:6864-6874 + 6903-6909) accumulated over calls on ctx, measured with HIP events on
- * the context stream: out4 = {launches, total milliseconds, algorithmic bytes (f and g read
- * once + 8 bytes per emitted mismatch record), voxels classified}. reset != 0 zeroes them. */
+ * the context stream: out4 = {launches, total milliseconds, bytes the kernel must move (the
+ * 16-byte mismatch mask of every voxel read + 8 bytes written per emitted mismatch record),
+ * voxels classified}. reset != 0 zeroes them. */
 int dmtz_kernel_stats(dmtz_ctx* ctx, double* out4, int reset);
 
 /* Outcome of the last correction on ctx beyond the counts: out4 = {stalls, vertices changed by

@@ -69,7 +69,9 @@ int dmtz_net_train_step(dmtz_ctx* ctx, dmtz_net* net, const uint16_t* d_feat, co
 /* Warm-started correction of device fields: features of g, forward pass, then every vertex with
  * a positive edit logit gets g = max(g - size * eps, snap(f - eps)) with size clamped to [0, 2]
  * (warm_edits_out, nullable, receives their number); then the plain loop of
- * dmtz_correct_dev_f32 with the given flags, which returns as there. */
+ * dmtz_correct_dev_f32 with the given flags, which returns as there. In d_counter_out the
+ * vertices the warm start moved carry counter 6, the edit-stream value whose edit is stored
+ * explicitly (main_3d.cu:1502-1508), unless the loop left them at 6 or 7 already. */
 int dmtz_correct_learned_f32(dmtz_ctx* ctx, dmtz_net* net, const float* d_f, float* d_g, int64_t nx, int64_t ny,
                              int64_t nz, double eps, double range, const dmtz_flags* flags, dmtz_counts* out,
                              uint8_t* d_counter_out, int32_t* sweeps_out, int64_t* warm_edits_out);

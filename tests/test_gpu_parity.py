@@ -336,3 +336,26 @@ def test_critical_cell_sets_small(oracle, dev):
         assert [tuple(r) for r in cells[2].tolist()] == want_t
         want_q = sorted(tuple(sorted(oracle.tet_verts(dims, int(q)))) for q in np.nonzero(pt == -1)[0])
         assert [tuple(r) for r in cells[3].tolist()] == want_q
+
+
+def test_batch_error_leaves_unrun_fields_marked(dev):
+    """A field that fails (nx = 2 is below the minimum grid) in the middle of a batch: its status
+    is the argument error, fields that ran report their own results, and fields that never
+    started report NOT_RUN with their g untouched."""
+    items = []
+    for seed, shape in [(0, (12, 11, 10)), (1, (9, 8, 2)), (2, (10, 9, 8)), (3, (11, 10, 9))]:
+        f = fields.gaussian_mixture(shape, 4, seed=seed, noise=0.02)
+        eps, rng = fields.eps_from_rel(f, 2e-2)
+        items.append((f, fields.quantize(f, eps), eps, rng))
+    fl = dmtz_amd.CorrectionFlags()
+    with pytest.raises(dmtz_amd.DmtzError) as ei:
+        dmtz_amd.correct_batch(items, fl, ctx=dev)
+    res = ei.value.results
+    assert res[1].status == -1  # DMTZ_E_ARG
+    for (f, g, eps, rng), r in zip(items, res):
+        assert r.status in (0, -1, dmtz_amd.NOT_RUN)
+        if r.status == dmtz_amd.NOT_RUN:
+            assert r.g.tobytes() == g.tobytes() and r.sweeps == 0
+        elif r.status == 0:
+            one = dmtz_amd.correct(f, g, eps, rng, fl, ctx=dev)
+            assert r.g.tobytes() == one.g.tobytes() and r.sweeps == one.sweeps
