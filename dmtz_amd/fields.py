@@ -9,6 +9,8 @@ float64 before the final rounding to float32.
 * ``quantize`` — the decompressed field g = 2*eps*round(f / (2*eps)), kept inside [f-eps, f+eps].
 * ``eps_from_rel`` — the absolute bound and range of main_3d.cu:6301-6309.
 """
+import os
+
 import numpy as np
 
 
@@ -45,7 +47,13 @@ def fbm(shape, seed=0, beta=3.0, device=None):
     D, H, W = shape
     if device is None:
         n = _rng(seed, 3).standard_normal(shape)
-        spec = np.fft.rfftn(n)
+        try:  # scipy's FFT runs on all host cores; numpy's on one
+            import scipy.fft as fft
+
+            kw = dict(workers=os.cpu_count() or 1)
+        except ImportError:
+            fft, kw = np.fft, {}
+        spec = fft.rfftn(n, **kw)
         kz = np.fft.fftfreq(D)[:, None, None]
         ky = np.fft.fftfreq(H)[None, :, None]
         kx = np.fft.rfftfreq(W)[None, None, :]
@@ -53,7 +61,7 @@ def fbm(shape, seed=0, beta=3.0, device=None):
         k[0, 0, 0] = 1.0
         amp = k ** (-beta / 2.0)
         amp[0, 0, 0] = 0.0
-        out = np.fft.irfftn(spec * amp, s=shape)
+        out = fft.irfftn(spec * amp, s=shape, axes=(0, 1, 2), **kw)
         lo, hi = out.min(), out.max()
         return ((out - lo) / (hi - lo)).astype(np.float32)
     import torch

@@ -61,6 +61,178 @@ typedef int64_t i64;
 
 namespace orc {
 
+// ----------------------------------------------------------------------------------------------
+// Branch-hit counters of the restated repair rules and separatrix checks (test infrastructure:
+// counting is off unless orc_coverage() switches it on). Each entry names the branch and the
+// main_3d.cu lines it restates.
+// ----------------------------------------------------------------------------------------------
+enum CovId {
+    C_MIN_F_PAIRED,
+    C_MIN_F_CRITICAL,
+    C_S1_FP_VERTEX,
+    C_S1_FP_VERTEX_G_UNPAIRED,
+    C_S1_FP_VERTEX_TARGET,
+    C_S1_FP_FACE,
+    C_S1_FP_FACE_A,
+    C_S1_FP_FACE_B,
+    C_S1_FP_FACE_C,
+    C_S1_FP_FACE_C_FPAIR,
+    C_S1_FP_FACE_C_DIFF,
+    C_S1_FP_FACE_D,
+    C_S1_FP_FACE_D1,
+    C_S1_FP_FACE_D2,
+    C_S1_FN_VERTEX,
+    C_S1_FN_FACE,
+    C_S1_FN_FACE_SAME,
+    C_S1_FN_FACE_FVERTEX,
+    C_S1_FN_FACE_SEARCH,
+    C_S1_FN_FACE_BETWEEN,
+    C_S1_FN_FACE_NO_BETWEEN,
+    C_S2_FP_EDGE,
+    C_S2_FP_EDGE_A,
+    C_S2_FP_EDGE_A_TARGET,
+    C_S2_FP_EDGE_B,
+    C_S2_FP_EDGE_B_GUNPAIRED,
+    C_S2_FP_EDGE_B_V2,
+    C_S2_FP_EDGE_B_V1,
+    C_S2_FP_EDGE_B_V3,
+    C_S2_FP_EDGE_C,
+    C_S2_FP_TET,
+    C_S2_FP_TET_MIXED,
+    C_S2_FP_TET_EDGE,
+    C_S2_FP_TET_TET,
+    C_S2_FP_TET_W,
+    C_S2_FP_TET_GUNPAIRED,
+    C_S2_FP_TET_ALL_ABOVE,
+    C_S2_FN_EDGE,
+    C_S2_FN_EDGE_FFACE,
+    C_S2_FN_EDGE_FVERTEX,
+    C_S2_FN_TET,
+    C_S2_FN_TET_ALL,
+    C_S2_FN_TET_TRUECELL,
+    C_S2_FN_TET_SMALLEST,
+    C_S2_FN_TET_NO_SMALLEST,
+    C_MAX_FP,
+    C_MAX_FP_ALL_BELOW,
+    C_MAX_FP_GEDGE,
+    C_MAX_FP_GTET,
+    C_MAX_FN,
+    C_MAX_FN_ALL,
+    C_MAX_FN_ALL_FTET,
+    C_MAX_FN_FEDGE,
+    C_MAX_FN_FTET,
+    C_MAX_FN_FCRIT,
+    C_ASW_F_CRITICAL,
+    C_ASW_DIVERGENT,
+    C_ASW_NOT_PAIRED_BACK,
+    C_ASW_BOUNDARY,
+    C_DSG_RECORD,
+    C_DSG_F_MIN,
+    C_DSC_FIRST,
+    C_DSC_NO_TROUBLE,
+    C_DSC_SECOND,
+    C_ASC_LABEL_DIFFERS,
+    C_ASC_RECORD,
+    C_FIXAS_FTET,
+    C_FIXAS_FACE,
+    C_FIXDS_NO_TROUBLE,
+    C_FIXDS_FMIN,
+    C_FIXDS_TARGET,
+    C_WALL_V3,
+    C_WALL_ENDS,
+    C_WALL_F_UNPAIRED,
+    C_WALL_G_UNPAIRED,
+    C_WALLDET_FOUND,
+    C_WALLDET_PUSH,
+    C_NCOV
+};
+static const char* const COV_NAMES[C_NCOV] = {
+    "MIN_F_PAIRED  :2560-2576 f pairs the vertex: target the paired edge's other end",
+    "MIN_F_CRITICAL  :2577-2585 f-critical vertex: target it",
+    "S1_FP_VERTEX  :2602-2641 false positive, f partner a vertex",
+    "S1_FP_VERTEX_G_UNPAIRED  :2617 g leaves the vertex unpaired: no target",
+    "S1_FP_VERTEX_TARGET  :2626-2640 other end above g partner: target it",
+    "S1_FP_FACE  :2643-2827 false positive, f partner a face",
+    "S1_FP_FACE_A  :2684-2699 both edge ends above the third vertex in g",
+    "S1_FP_FACE_B  :2700-2708 both above it in f",
+    "S1_FP_FACE_C  :2719-2800 g pairs the face with another edge",
+    "S1_FP_FACE_C_FPAIR  :2768-2786 that edge is f-paired with a face: its third vertex",
+    "S1_FP_FACE_C_DIFF  :2787-2797 nonzero delta: the edge's far vertex",
+    "S1_FP_FACE_D  :2801-2823 g face otherwise paired",
+    "S1_FP_FACE_D1  :2802-2812 first end",
+    "S1_FP_FACE_D2  :2813-2823 second end",
+    "S1_FN_VERTEX  :2839-2850 false negative, g partner a vertex",
+    "S1_FN_FACE  :2852-3035 false negative, g partner a face",
+    "S1_FN_FACE_SAME  :2891-2915 both ends on one side of the third vertex in g",
+    "S1_FN_FACE_FVERTEX  :2919-2937 f pairs the edge with a vertex",
+    "S1_FN_FACE_SEARCH  :2940-3003 f-critical edge: search a between vertex",
+    "S1_FN_FACE_BETWEEN  :3004-3016 lowest vertex between the ends",
+    "S1_FN_FACE_NO_BETWEEN  :3017-3029 none: both ends",
+    "S2_FP_EDGE  :3057-3223 false positive, f partner an edge",
+    "S2_FP_EDGE_A  :3097-3119 both above the third vertex in g",
+    "S2_FP_EDGE_A_TARGET  :3097-3119 third vertex of f's face of the edge",
+    "S2_FP_EDGE_B  :3120-3204 sides differ in f and in g",
+    "S2_FP_EDGE_B_GUNPAIRED  :3120-3134 edge not face-paired in g: third vertex",
+    "S2_FP_EDGE_B_V2  :3154-3169 g face of the edge: end v2",
+    "S2_FP_EDGE_B_V1  :3170-3180 g face of the edge: end v1",
+    "S2_FP_EDGE_B_V3  :3181-3203 otherwise the third vertex",
+    "S2_FP_EDGE_C  :3205-3222 third vertex not below both ends in g",
+    "S2_FP_TET  :3226-3477 false positive, f partner a tet",
+    "S2_FP_TET_MIXED  :3284-3323 g face of the tet not below in f nor g",
+    "S2_FP_TET_EDGE  :3288-3301 that face f-paired with an edge",
+    "S2_FP_TET_TET  :3302-3322 that face f-paired with a tet",
+    "S2_FP_TET_W  :3324-3337 the g face's vertices",
+    "S2_FP_TET_GUNPAIRED  :3338-3350 tet critical in g: fourth vertex",
+    "S2_FP_TET_ALL_ABOVE  :3351-3357 fourth vertex below the face in f: target it",
+    "S2_FN_EDGE  :3488-3610 false negative, g partner an edge",
+    "S2_FN_EDGE_FFACE  :3522-3543 f pairs the edge with a face: its third vertex",
+    "S2_FN_EDGE_FVERTEX  :3544-3602 f pairs the edge with a vertex: both ends",
+    "S2_FN_TET  :3613-3980 false negative, g partner a tet",
+    "S2_FN_TET_ALL  :3714-3930 fourth vertex below the face in g",
+    "S2_FN_TET_TRUECELL  :3937-3948 tet f-paired: the face's vertices",
+    "S2_FN_TET_SMALLEST  :3949-3961 get_smallest_vertex (:2115-2194)",
+    "S2_FN_TET_NO_SMALLEST  :3962-3975 none: the face's vertices",
+    "MAX_FP  :3995-4097 false positive",
+    "MAX_FP_ALL_BELOW  :4028-4039 face below the fourth vertex in f",
+    "MAX_FP_GEDGE  :4049-4063 g pairs the face with an edge: its higher end",
+    "MAX_FP_GTET  :4064-4095 g pairs the face with a tet: fourth vertex",
+    "MAX_FN  :4098-4298 false negative",
+    "MAX_FN_ALL  :4169-4224 face below the fourth vertex in g",
+    "MAX_FN_ALL_FTET  :4202-4223 and in f: the f tet's far vertex",
+    "MAX_FN_FEDGE  :4231-4243 f pairs the face with an edge: its ends",
+    "MAX_FN_FTET  :4244-4258 f pairs the face with a tet: its vertices",
+    "MAX_FN_FCRIT  :4259-4296 face f-critical: vertices below",
+    "ASW_F_CRITICAL  is_false :4464-4467 walk reaches an f-critical tet",
+    "ASW_DIVERGENT  is_false :4469-4479 f and g pairings differ: record",
+    "ASW_NOT_PAIRED_BACK  is_false :4486-4489 face not paired back",
+    "ASW_BOUNDARY  is_false :4491-4494 walk leaves through the boundary",
+    "DSG_RECORD  get_false_ds_vpath :4879-4887 start vertex paired differently: record",
+    "DSG_F_MIN  get_false_ds_vpath :4870-4873 endpoint is an f minimum",
+    "DSC_FIRST  get_false_ds_vpath_connectivity :5005-5013 first end reaches another minimum",
+    "DSC_NO_TROUBLE  get_false_ds_vpath_connectivity :5010 no differing vertex on the path (trouble_maker -1)",
+    "DSC_SECOND  get_false_ds_vpath_connectivity :5050-5058 second end reaches another minimum",
+    "ASC_LABEL_DIFFERS  get_false_as_vpath_connectivity :4822-4823 g reaches another maximum",
+    "ASC_RECORD  get_false_as_vpath_connectivity :4825 and the f path diverges: record",
+    "FIXAS_FTET  fix_as_vpath :5188-5217 g face f-paired with a tet: its far vertex",
+    "FIXAS_FACE  fix_as_vpath :5218-5243 g face's vertex off the f face",
+    "FIXDS_NO_TROUBLE  fix_ds_vpath :5274-5280 recorded vertex -1",
+    "FIXDS_FMIN  fix_ds_vpath :5274-5282 recorded vertex is an f minimum",
+    "FIXDS_TARGET  fix_ds_vpath :5283-5294 the f edge's other end",
+    "WALL_V3  fix_ds_wall :5311-5338 both faces paired back, nonzero delta: third vertex",
+    "WALL_ENDS  fix_ds_wall :5339-5353 zero delta: the edge's ends",
+    "WALL_F_UNPAIRED  fix_ds_wall :5355-5392 edge not face-paired in f: its ends",
+    "WALL_G_UNPAIRED  fix_ds_wall :5393-5411 edge not face-paired in g: f face's third vertex",
+    "WALLDET_FOUND  get_false_ds_wall :5114-5121 an edge of the wall paired differently",
+    "WALLDET_PUSH  get_false_ds_wall :5125 continue over the f-paired face"
+};
+static std::atomic<bool> cov_on{false};
+static std::atomic<long long> cov_hits[C_NCOV];
+#define COV(k)                                                                      \
+    do {                                                                            \
+        if (cov_on.load(std::memory_order_relaxed))                                  \
+            cov_hits[k].fetch_add(1, std::memory_order_relaxed);                     \
+    } while (0)
+
 // main_3d.cu:44-51 — the 14-neighbour stencil of the Freudenthal lattice.
 static const int DIRS[14][3] = {{1, 0, 0},  {-1, 0, 0}, {0, 1, 0},  {0, -1, 0}, {0, 0, 1},
                                 {0, 0, -1}, {-1, 1, 0}, {1, -1, 0}, {0, 1, 1},  {0, -1, -1},
@@ -774,8 +946,10 @@ template <class T> struct State {
             i64 vertex, v2;
             edge_verts(g, PV(F_FIELD, id), vertex, v2);
             if (vertex != id) v2 = vertex;
+            COV(C_MIN_F_PAIRED);
             tgt(v2);
         } else {
+            COV(C_MIN_F_CRITICAL);
             tgt(id);
         }
     }
@@ -786,6 +960,7 @@ template <class T> struct State {
         i64 p = PE(F_FIELD, id);
         if (p != -1) {
             if (p < g.N && p >= 0 && PV(F_FIELD, p) == id) {  // :2602-2641
+                COV(C_S1_FP_VERTEX);
                 i64 v1, v2;
                 edge_verts(g, edge, v1, v2);
                 if (v2 != p) v1 = p;
@@ -794,12 +969,19 @@ template <class T> struct State {
                     v1 = p;
                 }
                 i64 dv = PV(G_FIELD, v1);
-                if (dv == -1 || dv < 0 || dv >= g.E) return;
+                if (dv == -1 || dv < 0 || dv >= g.E) {
+                    COV(C_S1_FP_VERTEX_G_UNPAIRED);
+                    return;
+                }
                 i64 fv1, fv2;
                 edge_verts(g, dv, fv1, fv2);
                 i64 false_v2 = (v1 == fv1) ? fv2 : fv1;
-                if (cmp(v2, false_v2, G_FIELD)) tgt(v2);
+                if (cmp(v2, false_v2, G_FIELD)) {
+                    COV(C_S1_FP_VERTEX_TARGET);
+                    tgt(v2);
+                }
             } else if (p < g.F && p >= 0 && PF(F_FIELD, p) == id) {  // :2643-2827
+                COV(C_S1_FP_FACE);
                 i64 face = p;
                 i64 v1, v2;
                 edge_verts(g, edge, v1, v2);
@@ -812,12 +994,15 @@ template <class T> struct State {
                 bool v1_edge = cmp(v1, v3, G_FIELD), v2_edge = cmp(v2, v3, G_FIELD);
                 bool o_v1 = cmp(v1, v3, F_FIELD), o_v2 = cmp(v2, v3, F_FIELD);
                 if (v1_edge && v2_edge) {
+                    COV(C_S1_FP_FACE_A);
                     tgt(v3);  // :2684-2699
                 } else if (o_v1 && o_v2) {
+                    COV(C_S1_FP_FACE_B);
                     tgt(v3);  // :2700-2708
                 } else {
                     i64 q = PF(G_FIELD, p);
                     if (q != -1 && PE(G_FIELD, q) == p) {  // :2719
+                        COV(C_S1_FP_FACE_C);
                         i64 false_edge = q;
                         i64 a1, a2;
                         edge_verts(g, false_edge, a1, a2);
@@ -836,21 +1021,32 @@ template <class T> struct State {
                             tri_verts(g, pfe, c1, c2, c3);
                             if (c1 != w1 && c1 != w2) c3 = c1;
                             else if (c2 != w1 && c2 != w2) c3 = c2;
+                            COV(C_S1_FP_FACE_C_FPAIR);
                             tgt(c3);
                         } else if (std::fabs(diff) > 0) {  // :2787
+                            COV(C_S1_FP_FACE_C_DIFF);
                             tgt(false_v);
                         }
                     } else {  // :2801-2823
-                        if (o_v1 && !v1_edge) tgt(other1);
-                        else tgt(other1_1);
+                        COV(C_S1_FP_FACE_D);
+                        if (o_v1 && !v1_edge) {
+                            COV(C_S1_FP_FACE_D1);
+                            tgt(other1);
+                        }
+                        else {
+                            COV(C_S1_FP_FACE_D2);
+                            tgt(other1_1);
+                        }
                     }
                 }
             }
         } else {  // false negative, :2830-3036
             i64 q = PE(G_FIELD, id);
             if (q != -1 && q < g.N && q >= 0 && PV(G_FIELD, q) == id) {  // :2839-2850
+                COV(C_S1_FN_VERTEX);
                 tgt(q);
             } else if (q != -1 && q < g.F && q >= 0 && PF(G_FIELD, q) == id) {  // :2852
+                COV(C_S1_FN_FACE);
                 i64 face = q;
                 i64 v1, v2;
                 edge_verts(g, edge, v1, v2);
@@ -861,13 +1057,18 @@ template <class T> struct State {
                 bool v1_edge = cmp(v2, v3, G_FIELD), v2_edge = cmp(v1, v3, G_FIELD);  // :2886-2887
                 bool or1 = cmp(v2, v3, F_FIELD), or2 = cmp(v1, v3, F_FIELD);
                 if (v1_edge == v2_edge) {
+                    COV(C_S1_FN_FACE_SAME);
                     if (v1_edge != or1) tgt(v2);
                     if (v2_edge != or2) tgt(v1);
                 } else {
                     i64 t = PE(F_FIELD, id);
                     if (t != -1) {
-                        if (t >= 0 && t < g.N && PV(F_FIELD, t) == id) tgt(v1 == t ? v2 : v1);
+                        if (t >= 0 && t < g.N && PV(F_FIELD, t) == id) {
+                            COV(C_S1_FN_FACE_FVERTEX);
+                            tgt(v1 == t ? v2 : v1);
+                        }
                     } else {
+                        COV(C_S1_FN_FACE_SEARCH);
                         // :2942-3003 — lowest f-vertex strictly between the endpoints
                         i64 cn[14];
                         int n = edge_common(g, v1, v2, literal, cn);
@@ -886,8 +1087,12 @@ template <class T> struct State {
                                 }
                             }
                         }
-                        if (have) tgt(best_id);
+                        if (have) {
+                            COV(C_S1_FN_FACE_BETWEEN);
+                            tgt(best_id);
+                        }
                         else {
+                            COV(C_S1_FN_FACE_NO_BETWEEN);
                             tgt(v1);
                             tgt(v2);
                         }
@@ -934,6 +1139,7 @@ template <class T> struct State {
         i64 p = PF(F_FIELD, id);
         if (p != -1) {
             if (p < g.E && p >= 0 && PE(F_FIELD, p) == id) {  // :3057-3223
+                COV(C_S2_FP_EDGE);
                 i64 edge = p;
                 i64 v1, v2;
                 edge_verts(g, edge, v1, v2);
@@ -944,16 +1150,20 @@ template <class T> struct State {
                 bool v1_edge = cmp(v2, v3, G_FIELD), v2_edge = cmp(v1, v3, G_FIELD);
                 bool o_v1 = cmp(v2, v3, F_FIELD), o_v2 = cmp(v1, v3, F_FIELD);
                 if (v1_edge && v2_edge) {  // :3097
+                    COV(C_S2_FP_EDGE_A);
                     i64 f_id = PE(F_FIELD, edge);
                     if (f_id == -1 || PF(F_FIELD, f_id) != edge) return;
                     i64 c1, c2, c3;
                     tri_verts(g, f_id, c1, c2, c3);
                     if (c1 != v1 && c1 != v2) c3 = c1;
                     else if (c2 != v1 && c2 != v2) c3 = c2;
+                    COV(C_S2_FP_EDGE_A_TARGET);
                     tgt(c3);
                 } else if (o_v1 != o_v2 && v1_edge != v2_edge) {  // :3120
+                    COV(C_S2_FP_EDGE_B);
                     i64 f_id = PE(G_FIELD, edge);
                     if (f_id == -1 || PF(G_FIELD, f_id) != edge) {
+                        COV(C_S2_FP_EDGE_B_GUNPAIRED);
                         tgt(v3);
                     } else if (f_id < g.F && f_id >= 0) {  // :3135
                         i64 d1, d2, v3_1;
@@ -963,16 +1173,25 @@ template <class T> struct State {
                         bool e1 = cmp(v2, v3_1, G_FIELD), e2 = cmp(v1, v3_1, G_FIELD);
                         bool oe1 = cmp(v2, v3_1, F_FIELD), oe2 = cmp(v1, v3_1, F_FIELD);
                         if (e1 && e2 && ((!oe1 && oe2) || (oe1 && !oe2))) {
-                            if (!oe1) tgt(v2);
-                            else tgt(v1);
+                            if (!oe1) {
+                                COV(C_S2_FP_EDGE_B_V2);
+                                tgt(v2);
+                            }
+                            else {
+                                COV(C_S2_FP_EDGE_B_V1);
+                                tgt(v1);
+                            }
                         } else {
+                            COV(C_S2_FP_EDGE_B_V3);
                             tgt(v3);
                         }
                     }
                 } else if (!cmp(v1, v3, G_FIELD) || !cmp(v2, v3, G_FIELD)) {  // :3205
+                    COV(C_S2_FP_EDGE_C);
                     tgt(v3);
                 }
             } else if (p < g.T && p >= 0 && PT(F_FIELD, p) == id) {  // :3226-3477
+                COV(C_S2_FP_TET);
                 i64 tet = p;
                 i64 v1, v2, v3;
                 tri_verts(g, id, v1, v2, v3);
@@ -988,6 +1207,7 @@ template <class T> struct State {
                         bool g1 = cmp(w3, v4, G_FIELD), g2 = cmp(w2, v4, G_FIELD), g3 = cmp(w1, v4, G_FIELD);
                         bool h1 = cmp(w3, v4, F_FIELD), h2 = cmp(w2, v4, F_FIELD), h3 = cmp(w1, v4, F_FIELD);
                         if ((!g1 || !g2 || !g3) && (!h1 || !h2 || !h3)) {
+                            COV(C_S2_FP_TET_MIXED);
                             i64 idd = PF(F_FIELD, pff);
                             if (idd >= 0 && idd < g.E && PE(F_FIELD, idd) == pff) {  // :3288
                                 i64 e1, e2;
@@ -995,6 +1215,7 @@ template <class T> struct State {
                                 i64 vv3 = w3;
                                 if (w1 != e1 && e2 != w1) vv3 = w1;  // :3293
                                 else if (w2 != e1 && w2 != e2) vv3 = w2;
+                                COV(C_S2_FP_TET_EDGE);
                                 tgt(vv3);
                             } else if (idd >= 0 && idd < g.T && PT(F_FIELD, idd) == pff) {  // :3302
                                 i64 t1, t2, t3, t4;
@@ -1003,26 +1224,31 @@ template <class T> struct State {
                                 if (t1 != w1 && t1 != w2 && t1 != w3) t4 = w1;
                                 else if (t2 != w1 && t2 != w2 && t2 != w3) t4 = w2;
                                 else if (t3 != w1 && t3 != w2 && t3 != w3) t4 = w3;
+                                COV(C_S2_FP_TET_TET);
                                 tgt(t1);
                                 tgt(t2);
                                 tgt(t3);
                                 tgt(t4);
                             }
                         } else {  // :3324-3334
+                            COV(C_S2_FP_TET_W);
                             tgt(w1);
                             tgt(w2);
                             tgt(w3);
                         }
                     } else {
+                        COV(C_S2_FP_TET_GUNPAIRED);
                         tgt(v4);  // :3338-3345
                     }
                 } else {
+                    COV(C_S2_FP_TET_ALL_ABOVE);
                     tgt(v4);  // :3351-3357
                 }
             }
         } else {  // false negative :3480-3981
             i64 q = PF(G_FIELD, id);
             if (q != -1 && q < g.E && PE(G_FIELD, q) == id) {  // :3488-3610
+                COV(C_S2_FN_EDGE);
                 i64 edge_id2 = q;
                 i64 v1, v2;
                 edge_verts(g, edge_id2, v1, v2);
@@ -1034,12 +1260,15 @@ template <class T> struct State {
                     tri_verts(g, tf, a, b, v3);
                     if (s1 != a && s2 != a) v3 = a;
                     if (s1 != b && s2 != b) v3 = b;
+                    COV(C_S2_FN_EDGE_FFACE);
                     tgt(v3);
                 } else if (tf >= 0 && tf < g.N && PV(F_FIELD, tf) == edge_id2) {  // :3544
+                    COV(C_S2_FN_EDGE_FVERTEX);
                     tgt(v1);
                     tgt(v2);
                 }
             } else if (q != -1 && q < g.T && PT(G_FIELD, q) == id) {  // :3613-3980
+                COV(C_S2_FN_TET);
                 i64 tet = q;
                 i64 v1, v2, v3;
                 tri_verts(g, face, v1, v2, v3);
@@ -1049,6 +1278,7 @@ template <class T> struct State {
                 bool f1 = cmp(v3, v4, G_FIELD), f2 = cmp(v2, v4, G_FIELD), f3 = cmp(v1, v4, G_FIELD);
                 bool o1 = cmp(v3, v4, F_FIELD), o2 = cmp(v2, v4, F_FIELD), o3 = cmp(v1, v4, F_FIELD);
                 if (f1 && f2 && f3) {
+                    COV(C_S2_FN_TET_ALL);
                     // :3714-3927; the inner "else" branches are unreachable since f1,f2,f3 hold
                     if (f1 != o1) tgt(v3);
                     else if (f2 != o2) tgt(v2);
@@ -1056,13 +1286,18 @@ template <class T> struct State {
                 } else {
                     i64 true_cell = PT(F_FIELD, tet);
                     if (true_cell >= 0 && true_cell < g.T) {  // :3937 (face id compared to T)
+                        COV(C_S2_FN_TET_TRUECELL);
                         tgt(v1);
                         tgt(v2);
                         tgt(v3);
                     } else {
                         i64 m = smallest_vertex(v1, v2, v3);
-                        if (m != -1) tgt(m);
+                        if (m != -1) {
+                            COV(C_S2_FN_TET_SMALLEST);
+                            tgt(m);
+                        }
                         else {
+                            COV(C_S2_FN_TET_NO_SMALLEST);
                             tgt(v1);
                             tgt(v2);
                             tgt(v3);
@@ -1077,6 +1312,7 @@ template <class T> struct State {
     void fix_maximum(i64 id) {
         i64 p = PT(F_FIELD, id);
         if (p != -1) {  // false positive
+            COV(C_MAX_FP);
             i64 face1 = p;
             i64 v1, v2, v3;
             tri_verts(g, face1, v1, v2, v3);
@@ -1085,6 +1321,7 @@ template <class T> struct State {
             i64 v4 = other_of_tet(q1, q2, q3, q4, v1, v2, v3);
             bool o1 = cmp(v1, v4, F_FIELD), o2 = cmp(v2, v4, F_FIELD), o3 = cmp(v3, v4, F_FIELD);
             if (o1 && o2 && o3) {
+                COV(C_MAX_FP_ALL_BELOW);
                 tgt(v4);
             } else {
                 i64 fc = PF(G_FIELD, face1);
@@ -1092,13 +1329,16 @@ template <class T> struct State {
                     if (fc < g.E && PE(G_FIELD, fc) == face1) {
                         i64 a, b;
                         edge_verts(g, fc, a, b);
+                        COV(C_MAX_FP_GEDGE);
                         tgt(cmp(a, b, G_FIELD) ? a : b);
                     } else if (fc < g.T && PT(G_FIELD, fc) == face1) {
+                        COV(C_MAX_FP_GTET);
                         tgt(v4);
                     }
                 }
             }
         } else {  // false negative
+            COV(C_MAX_FN);
             i64 face1 = PT(G_FIELD, id);
             i64 v1, v2, v3;
             tri_verts(g, face1, v1, v2, v3);
@@ -1108,6 +1348,7 @@ template <class T> struct State {
             bool f1 = cmp(v1, v4, G_FIELD), f2 = cmp(v2, v4, G_FIELD), f3 = cmp(v3, v4, G_FIELD);
             bool o1 = cmp(v1, v4, F_FIELD), o2 = cmp(v2, v4, F_FIELD), o3 = cmp(v3, v4, F_FIELD);
             if (f1 && f2 && f3) {
+                COV(C_MAX_FN_ALL);
                 if (!o1) tgt(v1);
                 if (!o2) tgt(v2);
                 if (!o3) tgt(v3);
@@ -1116,6 +1357,7 @@ template <class T> struct State {
                     if (it != -1) {
                         i64 t1, t2, t3, t4;
                         tet_verts(g, it, t1, t2, t3, t4);
+                        COV(C_MAX_FN_ALL_FTET);
                         tgt(other_of_tet(t1, t2, t3, t4, v1, v2, v3));
                     }
                 }
@@ -1125,17 +1367,20 @@ template <class T> struct State {
                     if (pid >= 0 && pid < g.E && PE(F_FIELD, pid) == face1) {
                         i64 a, b;
                         edge_verts(g, pid, a, b);
+                        COV(C_MAX_FN_FEDGE);
                         tgt(a);
                         tgt(b);
                     } else {
                         i64 t1, t2, t3, t4;
                         tet_verts(g, pid, t1, t2, t3, t4);
+                        COV(C_MAX_FN_FTET);
                         tgt(t1);
                         tgt(t2);
                         tgt(t3);
                         tgt(t4);
                     }
                 } else {
+                    COV(C_MAX_FN_FCRIT);
                     if (!o1) tgt(v1);
                     if (!o2) tgt(v2);
                     if (!o3) tgt(v3);
@@ -1155,13 +1400,20 @@ template <class T> struct State {
             old = cur;
             i64 t = PT(F_FIELD, cur);
             i64 dt = PT(G_FIELD, cur);
-            if (t == -1) break;
+            if (t == -1) {
+                COV(C_ASW_F_CRITICAL);
+                break;
+            }
             if (t != dt) {
                 divergent = cur;
+                COV(C_ASW_DIVERGENT);
                 return true;
             }
             i64 pft = PF(F_FIELD, t);
-            if (pft == -1 || PT(F_FIELD, pft) != t) break;
+            if (pft == -1 || PT(F_FIELD, pft) != t) {
+                COV(C_ASW_NOT_PAIRED_BACK);
+                break;
+            }
             i64 t1, t2;
             cell_tetra(g, t, t1, t2, literal);
             cur = (t1 == cur) ? t2 : t1;
@@ -1170,6 +1422,7 @@ template <class T> struct State {
                 break;
             }
         } while (cur != old && cur != -1);
+        if (cur == -1) COV(C_ASW_BOUNDARY);
         return false;
     }
 
@@ -1254,9 +1507,13 @@ template <class T> struct State {
             i64 dec_v = dec_cur;
             if (v != dec_v) break;
             i64 ce = PV(F_FIELD, v);
-            if (ce == -1) break;
+            if (ce == -1) {
+                COV(C_DSG_F_MIN);
+                break;
+            }
             i64 dce = PV(G_FIELD, v);
             if (ce != dce) {
+                COV(C_DSG_RECORD);
                 rec = v;
                 return true;
             }
@@ -1373,6 +1630,8 @@ template <class T> struct State {
             i64 trouble = -1, end;
             ds_walk_conn(v1, trouble, end);
             if (end != min_label[2 * i]) {
+                COV(C_DSC_FIRST);
+                if (trouble == -1) COV(C_DSC_NO_TROUBLE);
                 l_ds.push_back(i);
                 ds_index[2 * i] = trouble;
                 ds_index[2 * i + 1] = 0;
@@ -1380,6 +1639,7 @@ template <class T> struct State {
             }
             ds_walk_conn(v2, trouble, end);
             if (end != min_label[2 * i + 1]) {
+                COV(C_DSC_SECOND);
                 l_ds.push_back(i);
                 ds_index[2 * i] = trouble;
                 ds_index[2 * i + 1] = 0;
@@ -1396,8 +1656,10 @@ template <class T> struct State {
             for (int k = 0; k < n; k++) {
                 i64 dec_id = paired_max(tets[k], G_FIELD);
                 if (dec_id != max_label[2 * i + k]) {
+                    COV(C_ASC_LABEL_DIFFERS);
                     i64 dv;
                     if (is_false(tets[k], dv)) {
+                        COV(C_ASC_RECORD);
                         l_as.push_back(tets[k]);
                         as_index[2 * tets[k]] = dv;
                         as_index[2 * tets[k] + 1] = 3;
@@ -1441,12 +1703,16 @@ template <class T> struct State {
                     i64 pj = PE(F_FIELD, j);
                     if (pj == -1) continue;
                     if (pj != PE(G_FIELD, j)) {
+                        COV(C_WALLDET_FOUND);
                         l_dsw.push_back(i);
                         dsw_index[i] = j;
                         found = true;
                         break;
                     }
-                    if (pj != tri && pj >= 0 && pj < g.F && PF(F_FIELD, pj) == j) stack.push_back(pj);
+                    if (pj != tri && pj >= 0 && pj < g.F && PF(F_FIELD, pj) == j) {
+                        COV(C_WALLDET_PUSH);
+                        stack.push_back(pj);
+                    }
                 }
             }
         }
@@ -1460,12 +1726,14 @@ template <class T> struct State {
         if (false_edge == -1 || true_edge == -1) return;  // unreachable with matching critical sets
         i64 pfe = PF(F_FIELD, false_edge);
         if (pfe >= 0 && pfe < g.T && PT(F_FIELD, pfe) == false_edge) {
+            COV(C_FIXAS_FTET);
             i64 v1, v2, v3;
             tri_verts(g, false_edge, v1, v2, v3);
             i64 q1, q2, q3, q4;
             tet_verts(g, pfe, q1, q2, q3, q4);
             tgt(other_of_tet(q1, q2, q3, q4, v1, v2, v3));
         } else {
+            COV(C_FIXAS_FACE);
             i64 v1, v2, v3;
             tri_verts(g, true_edge, v1, v2, v3);
             i64 a, b, tv3;
@@ -1479,9 +1747,16 @@ template <class T> struct State {
     // fix_ds_vpath main_3d.cu:5246-5295 (live else-branch)
     void fix_ds_vpath(i64 id) {
         i64 v_idx = ds_index[2 * id];
-        if (v_idx < 0) return;  // trouble_maker == -1: the reference reads index -1
+        if (v_idx < 0) {  // trouble_maker == -1: the reference reads index -1
+            COV(C_FIXDS_NO_TROUBLE);
+            return;
+        }
         i64 tpe = PV(F_FIELD, v_idx);
-        if (tpe == -1) return;
+        if (tpe == -1) {
+            COV(C_FIXDS_FMIN);
+            return;
+        }
+        COV(C_FIXDS_TARGET);
         i64 a, b;
         edge_verts(g, tpe, a, b);
         tgt(v_idx == a ? b : a);
@@ -1500,17 +1775,22 @@ template <class T> struct State {
             if (a != v1 && a != v2) v3 = a;
             if (b != v1 && b != v2) v3 = b;
             double diff = delta(v3);
-            if (std::fabs(diff) > 1e-16 * range) tgt(v3);
-            else {
+            if (std::fabs(diff) > 1e-16 * range) {
+                COV(C_WALL_V3);
+                tgt(v3);
+            } else {
+                COV(C_WALL_ENDS);
                 tgt(v1);
                 tgt(v2);
             }
         } else if (tf == -1 || tf >= g.F || (tf < g.F && tf >= 0 && PF(F_FIELD, tf) != e)) {
+            COV(C_WALL_F_UNPAIRED);
             i64 v1, v2;
             edge_verts(g, e, v1, v2);
             tgt(v1);
             tgt(v2);
         } else if (ff == -1 || ff >= g.F || (ff < g.F && ff >= 0 && PF(G_FIELD, ff) != e)) {
+            COV(C_WALL_G_UNPAIRED);
             i64 v1, v2;
             edge_verts(g, e, v1, v2);
             i64 a, b, v3;
@@ -1704,6 +1984,21 @@ typedef struct {
     int32_t preserve_saddles, preserve_vpath, preserve_geometry, filtration, preserve_connectors,
         preserve_type, edit_type, max_sweeps, finish;
 } orc_flags;
+
+// Branch-hit counters: enable = 1 clears them and starts counting, 0 stops. out (nullable,
+// cap entries) receives the counts; returns the number of counters.
+extern "C" int orc_coverage(int enable, long long* out, int cap) {
+    if (enable == 1) {
+        for (int k = 0; k < C_NCOV; k++) cov_hits[k].store(0);
+        cov_on.store(true);
+    } else if (enable == 0) {
+        cov_on.store(false);
+    }
+    if (out)
+        for (int k = 0; k < C_NCOV && k < cap; k++) out[k] = cov_hits[k].load();
+    return C_NCOV;
+}
+extern "C" const char* orc_coverage_name(int k) { return (k >= 0 && k < C_NCOV) ? COV_NAMES[k] : ""; }
 
 extern "C" int orc_set_threads(int n) {
 #ifdef _OPENMP

@@ -73,6 +73,9 @@ def lib():
             getattr(L, "orc_correct_" + t).argtypes = [P, P, I64, I64, I64, ctypes.c_double,
                                                       ctypes.c_double, P, P, P, P, P, I64, P, P]
         L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_coverage.argtypes = [ctypes.c_int, P, ctypes.c_int]
+        L.orc_coverage_name.argtypes = [ctypes.c_int]
+        L.orc_coverage_name.restype = ctypes.c_char_p
         _LIB = L
     return _LIB
 
@@ -87,6 +90,18 @@ def _tag(a):
     if a.dtype == np.float64:
         return "f64"
     raise TypeError("field must be float32 or float64")
+
+
+def coverage(enable=None):
+    """Branch-hit counters of the restated repair rules and separatrix checks: enable=True clears
+    and starts them, False stops them; returns {branch name and cited lines: hits}."""
+    L = lib()
+    n = L.orc_coverage(-1, None, 0)
+    if enable is not None:
+        L.orc_coverage(1 if enable else 0, None, 0)
+    out = np.zeros(n, np.int64)
+    L.orc_coverage(-1, _p(out), n)
+    return {L.orc_coverage_name(k).decode(): int(out[k]) for k in range(n)}
 
 
 def set_threads(n):

@@ -7,6 +7,7 @@ the same seeded inputs.
 import numpy as np
 import pytest
 
+import corpus
 import dmtz_amd
 from dmtz_amd import fields
 
@@ -38,34 +39,20 @@ def test_gradient_bit_exact(oracle, dev):
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("edit_type", [0, 1])
 def test_edit_set_bit_exact(oracle, dev, dtype, edit_type):
-    for seed in range(3):
-        for shape, rel in [((8, 9, 10), 1e-2), ((16, 16, 16), 3e-2), ((6, 7, 9), 5e-2)]:
-            f = fields.gaussian_mixture(shape, 4, seed=seed, noise=0.02).astype(dtype)
-            eps, rng = fields.eps_from_rel(f, rel)
-            g = fields.quantize(f, eps)
-            print("targets", shape, dtype.__name__, edit_type, seed, flush=True)
-            want, wc = oracle.targets(f, g, eps, rng, oracle.flags(edit_type=edit_type))
-            got, gc = dmtz_amd.targets(f, g, eps, rng, dmtz_amd.CorrectionFlags(edit_type=edit_type),
-                                       ctx=dev)
-            assert gc == wc
-            assert (got == want).all()
+    """One detection + repair phase: mismatch counts and edit sets, on Gaussian mixtures, fBm
+    crops and the inputs that reach the rarer repair branches (tests/corpus.py)."""
+    for name, f, g, eps, rng in corpus.edit_set_inputs():
+        f = f.astype(dtype)
+        g = g.astype(dtype)
+        print("targets", name, dtype.__name__, edit_type, flush=True)
+        want, wc = oracle.targets(f, g, eps, rng, oracle.flags(edit_type=edit_type))
+        got, gc = dmtz_amd.targets(f, g, eps, rng, dmtz_amd.CorrectionFlags(edit_type=edit_type),
+                                   ctx=dev)
+        assert gc == wc, name
+        assert (got == want).all(), name
 
 
-FLAG_CASES = [
-    ("cp", {}),
-    ("geometry", dict(preserve_vpath=1, preserve_geometry=1)),
-    ("connectivity", dict(preserve_vpath=1, preserve_geometry=0)),
-    ("edit_type1", dict(edit_type=1, preserve_vpath=1, preserve_geometry=1)),
-    ("filtration", dict(filtration=1)),
-    ("preserve_type", dict(preserve_type=1)),
-    # without saddle repairs these inputs reach a fixed point with maxima left (the reference
-    # loops forever there); both sides must report the stall in the same state
-    ("no_saddles", dict(preserve_saddles=0, max_sweeps=300)),
-    # the same stalls resolved by the finisher: restored boxes, frozen counters, zero mismatches
-    ("no_saddles_finish", dict(preserve_saddles=0, max_sweeps=300, finish=1)),
-    # runs that never stall are unchanged by the finisher
-    ("geometry_finish", dict(preserve_vpath=1, preserve_geometry=1, finish=1)),
-]
+FLAG_CASES = corpus.FLAG_CASES
 # expected status per case (0 converged, 2 stalled)
 EXPECT = {"no_saddles": 2}
 
@@ -85,13 +72,13 @@ def _compare_run(oracle, dev, f, g, eps, rng, fl):
 
 @pytest.mark.parametrize("name,fl", FLAG_CASES, ids=[c[0] for c in FLAG_CASES])
 def test_correction_bit_exact(oracle, dev, name, fl):
-    for seed, shape, rel in [(0, (16, 16, 16), 1e-2), (1, (24, 20, 12), 2e-2), (2, (13, 17, 11), 5e-2)]:
-        f = fields.gaussian_mixture(shape, 6, seed=seed, noise=0.02)
-        eps, rng = fields.eps_from_rel(f, rel)
-        g = fields.quantize(f, eps)
-        print("correct", name, shape, seed, flush=True)
+    """Full corrections on Gaussian mixtures and on fBm crops at eps = 1e-2, 1e-3 and 1e-4 of the
+    range (tests/corpus.py), compared with the oracle bit for bit."""
+    for inp, f, g, eps, rng in corpus.gm_inputs() + corpus.fbm_inputs():
+        print("correct", name, inp, f.shape, flush=True)
         got = _compare_run(oracle, dev, f, g, eps, rng, fl)
-        assert got.status == EXPECT.get(name, 0)
+        if inp.startswith("gm"):
+            assert got.status == EXPECT.get(name, 0)
         if got.status == 0:
             assert got.counts == (0,) * 8
             assert np.abs(got.g.astype(np.float64) - f).max() <= eps
@@ -99,21 +86,14 @@ def test_correction_bit_exact(oracle, dev, name, fl):
 
 # 2-saddle walls (preserve_connectors): expected status per input of test_correction_bit_exact;
 # without the finisher two of the inputs reach a fixed point with walls left
-WALL_CASES = [
-    ("walls_geometry", dict(preserve_vpath=1, preserve_geometry=1, preserve_connectors=1), (0, 2, 2)),
-    ("walls_connectivity", dict(preserve_vpath=1, preserve_geometry=0, preserve_connectors=1), (0, 2, 2)),
-    ("walls_finish", dict(preserve_vpath=1, preserve_geometry=1, preserve_connectors=1, finish=1), (0, 0, 0)),
-]
+WALL_EXPECT = {"walls_geometry": (0, 2, 2), "walls_connectivity": (0, 2, 2), "walls_finish": (0, 0, 0)}
+WALL_CASES = [(name, fl, WALL_EXPECT[name]) for name, fl in corpus.WALL_CASES]
 
 
 @pytest.mark.parametrize("name,fl,expect", WALL_CASES, ids=[c[0] for c in WALL_CASES])
 def test_correction_with_walls_bit_exact(oracle, dev, name, fl, expect):
-    inputs = [(0, (16, 16, 16), 1e-2), (1, (24, 20, 12), 2e-2), (2, (13, 17, 11), 5e-2)]
-    for (seed, shape, rel), status in zip(inputs, expect):
-        f = fields.gaussian_mixture(shape, 6, seed=seed, noise=0.02)
-        eps, rng = fields.eps_from_rel(f, rel)
-        g = fields.quantize(f, eps)
-        print("correct", name, shape, seed, flush=True)
+    for (inp, f, g, eps, rng), status in zip(corpus.gm_inputs(), expect):
+        print("correct", name, inp, f.shape, flush=True)
         got = _compare_run(oracle, dev, f, g, eps, rng, fl)
         assert got.status == status
         assert (got.trajectory.reshape(-1, 8)[:, 6] > 0).sum() > 1  # walls were found and repaired
@@ -126,24 +106,21 @@ def test_wall_lists_bit_exact(oracle, dev, dtype):
     """Wall lists (face, first differing edge) and the fix_ds_wall edit set, on quantised g and
     on g whose critical cells already match f's."""
     seen = 0
-    for seed, shape, rel in [(2, (16, 16, 16), 5e-3), (1, (24, 20, 12), 2e-2), (3, (13, 17, 11), 5e-2)]:
-        f = fields.gaussian_mixture(shape, 6, seed=seed, noise=0.02).astype(dtype)
-        eps, rng = fields.eps_from_rel(f, rel)
-        g0 = fields.quantize(f, eps)
-        g1 = oracle.correct(f, g0, eps, rng, oracle.flags())["g"]
-        for g in (g0, g1):
-            for edit_type in (0, 1):
-                print("walls", shape, dtype.__name__, seed, edit_type, flush=True)
-                ofl = oracle.flags(preserve_vpath=1, preserve_geometry=1, preserve_connectors=1, edit_type=edit_type)
-                want = oracle.paths(f, g, ofl)
-                got = dmtz_amd.walls(f, g, eps, rng, dmtz_amd.CorrectionFlags(edit_type=edit_type), ctx=dev,
-                                     with_targets=True)
-                assert sorted(zip(got["face"].tolist(), got["edge"].tolist())) == \
-                    sorted(zip(want["wall"].tolist(), want["wall_rec"].tolist()))
-                want_t, n = oracle.wall_targets(f, g, eps, rng, ofl)
-                assert n == len(got["face"])
-                assert (got["targets"] == want_t).all()
-                seen += n
+    for name, f, g, eps, rng in corpus.wall_inputs(oracle):
+        f = f.astype(dtype)
+        g = g.astype(dtype)
+        for edit_type in (0, 1):
+            print("walls", name, dtype.__name__, edit_type, flush=True)
+            ofl = oracle.flags(preserve_vpath=1, preserve_geometry=1, preserve_connectors=1, edit_type=edit_type)
+            want = oracle.paths(f, g, ofl)
+            got = dmtz_amd.walls(f, g, eps, rng, dmtz_amd.CorrectionFlags(edit_type=edit_type), ctx=dev,
+                                 with_targets=True)
+            assert sorted(zip(got["face"].tolist(), got["edge"].tolist())) == \
+                sorted(zip(want["wall"].tolist(), want["wall_rec"].tolist()))
+            want_t, n = oracle.wall_targets(f, g, eps, rng, ofl)
+            assert n == len(got["face"])
+            assert (got["targets"] == want_t).all()
+            seen += n
     assert seen > 0
 
 
@@ -190,26 +167,22 @@ def test_separatrix_lists_bit_exact(oracle, dev, geometry):
     P = ctypes.c_void_p
     I = ctypes.c_int64
     L.orc_path_targets_f32.argtypes = [P, P, I, I, I, ctypes.c_double, ctypes.c_double, P, P, P]
-    for seed in range(4):
-        for shape, rel in [((12, 12, 12), 1e-2), ((10, 14, 9), 3e-2), ((16, 16, 16), 5e-3)]:
-            f = fields.gaussian_mixture(shape, 5, seed=seed, noise=0.02)
-            eps, rng = fields.eps_from_rel(f, rel)
-            g = oracle.correct(f, fields.quantize(f, eps), eps, rng, oracle.flags())["g"]
-            fl = dict(preserve_vpath=1, preserve_geometry=geometry)
-            print("paths", shape, seed, geometry, flush=True)
-            want = oracle.paths(f, g, oracle.flags(**fl))
-            got = dmtz_amd.paths(f, g, dmtz_amd.CorrectionFlags(**fl), ctx=dev, with_targets=True)
-            assert sorted(zip(got["ds"].tolist(), got["ds_rec"].tolist())) == \
-                sorted(zip(want["ds"].tolist(), want["ds_rec"].tolist()))
-            assert sorted(zip(got["as_"].tolist(), got["as_rec"].tolist())) == \
-                sorted(zip(want["as_"].tolist(), want["as_rec"].tolist()))
-            D, H, W = shape
-            tg = np.zeros(W * H * D, np.uint8)
-            c2 = np.zeros(2, np.int64)
-            ofl = oracle.flags(**fl)
-            L.orc_path_targets_f32(f.ctypes.data_as(P), g.ctypes.data_as(P), W, H, D, eps, rng,
-                                   ctypes.byref(ofl), tg.ctypes.data_as(P), c2.ctypes.data_as(P))
-            assert (got["targets"].ravel() == tg.astype(bool)).all()
+    for name, f, g, eps, rng in corpus.path_inputs(oracle):
+        fl = dict(preserve_vpath=1, preserve_geometry=geometry)
+        print("paths", name, geometry, flush=True)
+        want = oracle.paths(f, g, oracle.flags(**fl))
+        got = dmtz_amd.paths(f, g, dmtz_amd.CorrectionFlags(**fl), ctx=dev, with_targets=True)
+        assert sorted(zip(got["ds"].tolist(), got["ds_rec"].tolist())) == \
+            sorted(zip(want["ds"].tolist(), want["ds_rec"].tolist()))
+        assert sorted(zip(got["as_"].tolist(), got["as_rec"].tolist())) == \
+            sorted(zip(want["as_"].tolist(), want["as_rec"].tolist()))
+        D, H, W = f.shape
+        tg = np.zeros(W * H * D, np.uint8)
+        c2 = np.zeros(2, np.int64)
+        ofl = oracle.flags(**fl)
+        L.orc_path_targets_f32(f.ctypes.data_as(P), g.ctypes.data_as(P), W, H, D, eps, rng,
+                               ctypes.byref(ofl), tg.ctypes.data_as(P), c2.ctypes.data_as(P))
+        assert (got["targets"].ravel() == tg.astype(bool)).all()
 
 
 def test_classify_matches_oracle(oracle, dev):
