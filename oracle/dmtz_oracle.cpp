@@ -2113,6 +2113,46 @@ static int targets_impl(const T* f, const T* gfield, i64 W, i64 H, i64 D, double
     std::memcpy(targets, s.target.data(), s.g.N);
     return 0;
 }
+// The repair rule of one mismatched cell on (f, g) as they stand: type 0..3 selects fix_minimum,
+// fix_saddle, fix_2saddle or fix_maximum and id the cell (reference simplex ID); targets[N]
+// receives its edit set. Returns 1 when the cell is in that type's mismatch list, 0 when not.
+static int fix_one_impl(const float* f, const float* gfield, i64 W, i64 H, i64 D, double eps, double range,
+                        int edit_type, int type, i64 id, uint8_t* targets) {
+    init_dirs();
+    State<float> s(W, H, D);
+    s.f = f;
+    std::vector<float> gcopy(gfield, gfield + s.g.N);
+    s.d = gcopy.data();
+    s.eps = eps;
+    s.range = range;
+    std::memset(&s.fl, 0, sizeof(orc_flags));
+    s.fl.preserve_saddles = 1;
+    s.fl.edit_type = edit_type;
+    s.literal = false;
+    s.alloc();
+    s.gradient(F_FIELD);
+    s.gradient(G_FIELD);
+    s.detect_min();
+    s.detect_s1();
+    s.detect_s2();
+    s.detect_max();
+    const std::vector<i64>* lists[4] = {&s.l_min, &s.l_s1, &s.l_s2, &s.l_max};
+    if (type < 0 || type > 3) return -1;
+    const std::vector<i64>& l = *lists[type];
+    const bool listed = std::find(l.begin(), l.end(), id) != l.end();
+    if (listed) {
+        if (type == 0) s.fix_minimum(id);
+        else if (type == 1) s.fix_saddle(id);
+        else if (type == 2) s.fix_2saddle(id);
+        else s.fix_maximum(id);
+    }
+    std::memcpy(targets, s.target.data(), s.g.N);
+    return listed ? 1 : 0;
+}
+extern "C" int orc_fix_one_f32(const float* f, const float* g, i64 W, i64 H, i64 D, double eps, double range,
+                               int edit_type, int type, i64 id, uint8_t* targets) {
+    return fix_one_impl(f, g, W, H, D, eps, range, edit_type, type, id, targets);
+}
 extern "C" int orc_targets_f32(const float* f, const float* g, i64 W, i64 H, i64 D, double eps, double range,
                     const orc_flags* fl, uint8_t* targets, i64* counts4) {
     return targets_impl<float>(f, g, W, H, D, eps, range, fl, targets, counts4);

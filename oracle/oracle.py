@@ -73,6 +73,8 @@ def lib():
             getattr(L, "orc_correct_" + t).argtypes = [P, P, I64, I64, I64, ctypes.c_double,
                                                       ctypes.c_double, P, P, P, P, P, I64, P, P]
         L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_fix_one_f32.argtypes = [P, P, I64, I64, I64, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                      ctypes.c_int, I64, P]
         L.orc_coverage.argtypes = [ctypes.c_int, P, ctypes.c_int]
         L.orc_coverage_name.argtypes = [ctypes.c_int]
         L.orc_coverage_name.restype = ctypes.c_char_p
@@ -204,6 +206,19 @@ def paths(f, g, fl):
     n_ds, n_as, n_w = (int(x) for x in c3)
     return dict(rc=rc, ds=ds_l[:n_ds], ds_rec=ds_r[:n_ds], as_=as_l[:n_as], as_rec=as_r[:n_as],
                 wall=w_l[:n_w], wall_rec=w_r[:n_w])
+
+
+def fix_one(f, g, eps, rng, cell_type, cell, edit_type=0):
+    """The repair rule of one mismatched cell (cell_type 0..3: minimum, 1-saddle, 2-saddle,
+    maximum; cell: its reference simplex ID) on float32 (f, g): the sorted vertices of its edit
+    set, or None when the cell is not mismatched."""
+    f = np.ascontiguousarray(f, dtype=np.float32)
+    g = np.ascontiguousarray(g, dtype=np.float32)
+    D, H, W = f.shape
+    tg = np.zeros(W * H * D, np.uint8)
+    rc = lib().orc_fix_one_f32(_p(f), _p(g), W, H, D, float(eps), float(rng), int(edit_type), int(cell_type),
+                               int(cell), _p(tg))
+    return sorted(np.nonzero(tg)[0].tolist()) if rc == 1 else None
 
 
 def wall_targets(f, g, eps, rng, fl=None):

@@ -94,6 +94,30 @@ def branch_inputs():
     return [("branch%d" % s,) + _branch_input(s) for s in (0, 1407, 543, 965, 8106)]
 
 
+# Tiny integer fields of the hand-worked repair-rule cases (test_oracle_kats.py): x fastest,
+# 3x3x2 and 3x2x2 grids, (D, H, W) = (2, 3, 3) and (2, 2, 3); g differs at the listed vertices.
+F_A = [0, 7, 9, 16, 5, 8, 1, 10, 17, 2, 13, 11, 15, 6, 4, 3, 12, 14]
+F_B = [9, 11, 1, 3, 2, 4, 6, 7, 0, 10, 5, 8]
+F_C = [8, 11, 4, 7, 5, 0, 1, 9, 2, 10, 6, 3]
+F_D = [6, 16, 7, 10, 2, 11, 0, 15, 17, 14, 9, 12, 5, 13, 3, 4, 8, 1]
+KAT_INPUTS = {
+    "A": ((2, 3, 3), F_A, {8: 0.5}),
+    "B": ((2, 2, 3), F_B, {1: 2.5}),
+    "C": ((2, 2, 3), F_C, {4: 1.5}),
+    "D": ((2, 3, 3), F_D, {2: 11.5, 4: 4.5}),
+}
+
+
+def kat_field(name):
+    """(f, g) of the hand-worked repair-rule cases of test_oracle_kats.py."""
+    shape, fv, changes = KAT_INPUTS[name]
+    f = np.array(fv, np.float32).reshape(shape)
+    g = f.copy()
+    for v, x in changes.items():
+        g.flat[v] = x
+    return f, g
+
+
 def edit_set_inputs():
     """(name, f, g, eps, range, dtype) of the one-phase edit-set comparisons."""
     out = []
@@ -104,6 +128,7 @@ def edit_set_inputs():
             out.append(("gm%d_%s" % (seed, shape), f, fields.quantize(f, eps), eps, rng))
     out += fbm_inputs()
     out += branch_inputs()
+    out += [("kat_" + k,) + kat_field(k) + (100.0, 100.0) for k in sorted(KAT_INPUTS)]
     return out
 
 
