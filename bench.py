@@ -370,11 +370,15 @@ def batch_fields(k, seed):
 
 
 def bench_batch(args):
-    """The batched form: one dmtz_correct_batch call per step over K device-resident fields."""
+    """The batched forms on K device-resident fields, timed against the plain form on the same
+    fields (one field after another): plain, batched (one dmtz_correct_batch call) and
+    batched+learned (dmtz_correct_batch_learned_f32: the warm start on every field, then the
+    batch). value is the batched+learned rate; the ratio to the plain form is the metric's."""
     import numpy as np
 
     import dmtz_amd
     from dmtz_amd import devmem
+    from dmtz_amd import learned as L
 
     log("generating %d batch fields" % args.batch)
     items = batch_fields(args.batch, args.seed)
@@ -386,49 +390,74 @@ def bench_batch(args):
                                      finish=args.finish)
     ctx = dmtz_amd.Context(0)
     batch = [(fd[i], gd[i], items[i][2], items[i][3]) for i in range(len(items))]
+    net = None
+    if args.learned and args.learned != "none" and os.path.exists(args.learned):
+        net = L.Net(seed=0, ctx=ctx)
+        net.set_params(np.load(args.learned))
 
-    def step():
+    def reset():
         for a, b in zip(gd, g0):
             a.copy_from(b)
+
+    def plain():
+        return [dmtz_amd.correct_device(f, g, eps, rng, flags, ctx=ctx) for f, g, eps, rng in batch]
+
+    def batched():
         return dmtz_amd.correct_batch(batch, flags, ctx=ctx)
 
-    for w in range(args.warmup):
-        rs = step()
-        log("warmup %d: sweeps %s" % (w, [r.sweeps for r in rs]))
-    devmem.synchronize()
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        rs = step()
-        log("step %d: total sweeps %d" % (s, sum(r.sweeps for r in rs)))
-    devmem.synchronize()
-    elapsed = time.perf_counter() - t0
-    edited = 0
-    for i, r in enumerate(rs):
-        assert r.converged, (i, r.counts)
-        gi = gd[i].numpy()
-        err = float(np.abs(gi.astype(np.float64) - items[i][0].astype(np.float64)).max())
-        assert err <= items[i][2], (i, err)
-        edited += int(np.count_nonzero(gi != items[i][1]))
+    def batched_learned():
+        return L.correct_batch_learned(net, batch, flags)[0]
+
     voxels = sum(int(f.size) for f, _, _, _, _ in items)
+    forms = [("plain", plain), ("batched", batched)] + ([("batched_learned", batched_learned)] if net else [])
+    out_forms = {}
+    for name, fn in forms:
+        for w in range(args.warmup):
+            reset()
+            fn()
+        t = 0.0
+        for s in range(args.steps):
+            reset()
+            devmem.synchronize()
+            t0 = time.perf_counter()
+            rs = fn()
+            devmem.synchronize()
+            t += time.perf_counter() - t0
+        edited = 0
+        for i, r in enumerate(rs):
+            assert r.converged, (name, i, r.counts)
+            gi = gd[i].numpy()
+            err = float(np.abs(gi.astype(np.float64) - items[i][0].astype(np.float64)).max())
+            assert err <= items[i][2], (name, i, err)
+            edited += int(np.count_nonzero(gi != items[i][1]))
+        out_forms[name] = dict(value=round(voxels * args.steps / t, 1), ms_per_step=round(t / args.steps * 1e3, 3),
+                               sweeps=int(sum(r.sweeps for r in rs)), edit_ratio=round(edited / voxels, 6))
+        log("%s: %.1f ms per batch, %d sweeps" % (name, t / args.steps * 1e3, out_forms[name]["sweeps"]))
+    if net:
+        net.close()
+    best = "batched_learned" if net else "batched"
+    plain_v = out_forms["plain"]["value"]
     workload = ("batch of %d float32 fields, sides 32..192, Gaussian mixture / fBm, eps log-uniform "
                 "[1e-4, 1e-2]*range, %s" % (args.batch, "critical cells + separatrix geometry"
                                              if args.geometry else "critical cells"))
     out = {
-        "metric": "voxels/s to zero topology mismatch within eps (batched form, 1 GPU)",
-        "value": round(voxels * args.steps / elapsed, 1),
+        "metric": "voxels/s to zero topology mismatch within eps (%s vs plain, 1 GPU)" % best.replace("_", "+"),
+        "value": out_forms[best]["value"],
         "unit": "voxels/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "ms_per_step": out_forms[best]["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": workload, "fields": args.batch, "voxels": voxels, "seed": args.seed,
-                   "sweeps": [r.sweeps for r in rs], "form": "batched", "finish": args.finish},
-        "edit_ratio": round(edited / voxels, 6),
+                   "form": best, "finish": args.finish},
+        "forms": out_forms,
+        "speedup_vs_plain": {k: round(v["value"] / plain_v, 3) for k, v in out_forms.items()},
+        "edit_ratio": out_forms[best]["edit_ratio"],
         "roofline": None,
         "cpu_baseline": None if args.cpu == "none" else cpu_baseline(one_thread=False),
     }

@@ -37,6 +37,11 @@ class Counts(ctypes.Structure):
         return tuple(int(getattr(self, n)) for n, _ in self._fields_)
 
 
+class EditStream(ctypes.Structure):
+    """dmtz_edit_stream: sizes of an edit stream (cost / original_cost, main_3d.cu:1494-1757)."""
+    _fields_ = [("edited", I64), ("index_bytes", I64), ("n_edits", I64)]
+
+
 def _field_struct(name, ctype):
     return type(name, (ctypes.Structure,), {"_fields_": [
         ("f", P), ("g", P), ("nx", I64), ("ny", I64), ("nz", I64), ("eps", D), ("range", D),
@@ -81,6 +86,14 @@ def lib():
         getattr(L, "dmtz_walls_" + t).argtypes = [P, P, P, I64, I64, I64, D, D, ctypes.POINTER(Flags), P, P, P,
                                                   I64, P]
         getattr(L, "dmtz_correct_batch_" + t).argtypes = [P, P, I32, ctypes.POINTER(Flags), P, P, P]
+    for t in ("f32", "f64"):
+        getattr(L, "dmtz_edit_stream_" + t).argtypes = [P, P, P, P, I64, D, I32, P, I64, P, I64,
+                                                        ctypes.POINTER(EditStream)]
+        getattr(L, "dmtz_edit_apply_" + t).argtypes = [P, P, I64, D, I32, P, I64, P, I64]
+    L.dmtz_bitmap_rle.argtypes = [P, P, I64, P, I64, ctypes.POINTER(I64)]
+    L.dmtz_read_raw_f64.argtypes = [ctypes.c_char_p, P, I64, ctypes.POINTER(I64)]
+    L.dmtz_write_bin_f64.argtypes = [ctypes.c_char_p, P, I64]
+    L.dmtz_read_bin_f64.argtypes = [ctypes.c_char_p, P, I64, ctypes.POINTER(I64)]
     L.dmtz_trajectory.argtypes = [P, P, I64]
     L.dmtz_trajectory.restype = I64
     L.dmtz_phase_times.argtypes = [P, P]
@@ -100,6 +113,7 @@ def lib():
     L.dmtz_net_train_step.argtypes = [P, P, P, P, I64, I64, I64, F, F, F, P]
     L.dmtz_correct_learned_f32.argtypes = [P, P, P, P, I64, I64, I64, D, D, ctypes.POINTER(Flags),
                                            ctypes.POINTER(Counts), P, ctypes.POINTER(I32), P]
+    L.dmtz_correct_batch_learned_f32.argtypes = [P, P, P, I32, ctypes.POINTER(Flags), P, P, P, P]
     _LIB = L
     return L
 
@@ -115,8 +129,10 @@ EXPORTS = (
     "dmtz_paths_f32", "dmtz_paths_f64",
     "dmtz_classify_f32", "dmtz_classify_f64",
     "dmtz_correct_batch_f32", "dmtz_correct_batch_f64",
+    "dmtz_edit_stream_f32", "dmtz_edit_stream_f64", "dmtz_edit_apply_f32", "dmtz_edit_apply_f64", "dmtz_bitmap_rle",
+    "dmtz_read_raw_f64", "dmtz_write_bin_f64", "dmtz_read_bin_f64",
     "dmtz_walls_f32", "dmtz_walls_f64",
     "dmtz_net_create", "dmtz_net_destroy", "dmtz_net_params", "dmtz_net_set_params",
     "dmtz_net_features_f32", "dmtz_net_forward", "dmtz_net_gradient", "dmtz_net_train_step",
-    "dmtz_correct_learned_f32",
+    "dmtz_correct_learned_f32", "dmtz_correct_batch_learned_f32",
 )

@@ -1,4 +1,6 @@
 // Context management and the type-independent entry points of include/dmtz.h.
+#include <cstdio>
+
 #include "driver.h"
 
 extern "C" {
@@ -12,6 +14,7 @@ int dmtz_ctx_create(dmtz_ctx** out, int device) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&ctx->d_ctr, sizeof(ull) * K_NCTR);
     if (e == hipSuccess) e = hipHostMalloc(&ctx->h_ctr, sizeof(ull) * K_NCTR);
+    if (e == hipSuccess) e = hipHostMalloc(&ctx->h_as_total, sizeof(ull));
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
     if (e != hipSuccess) {
@@ -29,6 +32,7 @@ void dmtz_ctx_destroy(dmtz_ctx* ctx) {
     ctx->workers.clear();
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->as_stream) (void)hipStreamSynchronize(ctx->as_stream);
     free_bufs(ctx);
     free_rec(ctx);
     free_fc(ctx);
@@ -46,7 +50,14 @@ void dmtz_ctx_destroy(dmtz_ctx* ctx) {
         if (ctx->fix_stream[k]) (void)hipStreamDestroy(ctx->fix_stream[k]);
     }
     if (ctx->fix_fork) (void)hipEventDestroy(ctx->fix_fork);
+    if (ctx->as_stream) {
+        (void)hipStreamSynchronize(ctx->as_stream);
+        (void)hipStreamDestroy(ctx->as_stream);
+    }
+    for (hipEvent_t ev : {ctx->as_ev_fork, ctx->as_ev_count, ctx->as_ev_done})
+        if (ev) (void)hipEventDestroy(ev);
     (void)hipHostFree(ctx->h_ctr);
+    (void)hipHostFree(ctx->h_as_total);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -106,3 +117,62 @@ int device(dmtz_ctx* ctx) { return ctx->device; }
 void set_error(dmtz_ctx* ctx, const char* msg) { ctx->err = msg; }
 }  // namespace dmtz_internal
 
+
+// Field files (getdata2 main_3d.cu:153-181, saveVectorToBin main_3d.cu:1315-1327). Host code.
+namespace {
+int read_doubles(FILE* fp, int64_t count, double* out, int64_t cap) {
+    if (!out) return 0;
+    if (cap < count) return DMTZ_E_CAPACITY;
+    return (int64_t)std::fread(out, sizeof(double), (size_t)count, fp) == count ? 0 : DMTZ_E_IO;
+}
+}  // namespace
+
+extern "C" {
+
+int dmtz_read_raw_f64(const char* path, double* out, int64_t cap, int64_t* n_out) {
+    if (!path || !n_out) return DMTZ_E_ARG;
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) return DMTZ_E_IO;
+    int rc = 0;
+    long size = -1;
+    if (std::fseek(fp, 0, SEEK_END) == 0) size = std::ftell(fp);
+    if (size < 0 || std::fseek(fp, 0, SEEK_SET) != 0) rc = DMTZ_E_IO;
+    if (rc == 0) {
+        *n_out = (int64_t)size / (int64_t)sizeof(double);  // a trailing partial value is ignored, as there
+        rc = read_doubles(fp, *n_out, out, cap);
+    }
+    std::fclose(fp);
+    return rc;
+}
+
+int dmtz_write_bin_f64(const char* path, const double* data, int64_t n) {
+    if (!path || n < 0 || (n > 0 && !data)) return DMTZ_E_ARG;
+    FILE* fp = std::fopen(path, "wb");
+    if (!fp) return DMTZ_E_IO;
+    const uint64_t count = (uint64_t)n;
+    bool ok = std::fwrite(&count, sizeof(count), 1, fp) == 1;
+    ok = ok && (n == 0 || (int64_t)std::fwrite(data, sizeof(double), (size_t)n, fp) == n);
+    ok = (std::fclose(fp) == 0) && ok;
+    return ok ? 0 : DMTZ_E_IO;
+}
+
+int dmtz_read_bin_f64(const char* path, double* out, int64_t cap, int64_t* n_out) {
+    if (!path || !n_out) return DMTZ_E_ARG;
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) return DMTZ_E_IO;
+    uint64_t count = 0;
+    int rc = std::fread(&count, sizeof(count), 1, fp) == 1 ? 0 : DMTZ_E_IO;
+    long size = -1;
+    if (rc == 0 && std::fseek(fp, 0, SEEK_END) == 0) size = std::ftell(fp);
+    if (rc == 0 && (size < 0 || (uint64_t)(size - (long)sizeof(count)) != count * sizeof(double) ||
+                    std::fseek(fp, (long)sizeof(count), SEEK_SET) != 0))
+        rc = DMTZ_E_IO;  // the count must match the file's length
+    if (rc == 0) {
+        *n_out = (int64_t)count;
+        rc = read_doubles(fp, *n_out, out, cap);
+    }
+    std::fclose(fp);
+    return rc;
+}
+
+}  // extern "C"

@@ -65,7 +65,8 @@ enum : int {
     K_DSW = 17,                                 // 2-saddle wall mismatch list
     K_WOVF = 18,                                // wall walks that outgrew their visited set
     K_ASNEXT = 19,                              // next branch of the work-stealing path walk
-    K_NCTR = 20
+    K_DTILES = 20,                              // tiles whose masks the last detection read
+    K_NCTR = 21
 };
 
 constexpr uint8_t FROZEN = 7;  // counter value of a vertex restored to f by the finisher
@@ -209,9 +210,10 @@ __global__ __launch_bounds__(BLOCK, CODE_BLOCKS_PER_CU) void k_code(const T* fld
 
 // g star code of vertex v and its mismatch mask: the cells v owns in g whose critical status in
 // g differs from f's (owner_mismatch, main_3d.cu:2492-2550). Returns whether the code changed.
+// nz_old / nz_new (nullable) receive whether the mask was nonzero before and after.
 template <class T>
 __device__ __forceinline__ bool code_g_at(const T* g, Dims d, i64 v, StarCode* code, const CritMask* crit_f,
-                                          CritMask* mm) {
+                                          CritMask* mm, bool* nz_old = nullptr, bool* nz_new = nullptr) {
     LowerStar ls;
     star_of(g, d, v, ls);
     StarCode sc;
@@ -227,6 +229,11 @@ __device__ __forceinline__ bool code_g_at(const T* g, Dims d, i64 v, StarCode* c
     m.lo = (c_lo ^ cf.lo) & i_lo;
     m.hi = (c_hi ^ cf.hi) & i_hi;
     m.pad = 0;
+    if (nz_old) {
+        const CritMask o = mm[v];
+        *nz_old = (o.lo | (uint64_t)o.hi) != 0;
+    }
+    if (nz_new) *nz_new = (m.lo | (uint64_t)m.hi) != 0;
     mm[v] = m;
     return ((old.w[0] ^ sc.w[0]) | (old.w[1] ^ sc.w[1]) | (old.w[2] ^ sc.w[2]) | (old.w[3] ^ sc.w[3])) != 0;
 }
@@ -245,28 +252,36 @@ __device__ __forceinline__ void mark_vertex(Dims d, uint32_t v, uint32_t* cubes,
     }
 }
 
+// Dense re-evaluation; tile_nz (zeroed before) receives, per EMIT_TILE vertices, the number with
+// a nonzero mismatch mask (a wave's 64 vertices lie in one tile).
 template <class T>
 __global__ __launch_bounds__(BLOCK, CODE_BLOCKS_PER_CU) void k_code_g(const T* g, Dims d, Counts c, StarCode* code,
-                                                  const CritMask* crit_f, CritMask* mm) {
+                                                  const CritMask* crit_f, CritMask* mm, uint32_t* tile_nz) {
     const i64 v = (i64)blockIdx.x * BLOCK + threadIdx.x;
-    if (v < c.N) code_g_at(g, d, v, code, crit_f, mm);
+    bool nz = false;
+    if (v < c.N) code_g_at(g, d, v, code, crit_f, mm, nullptr, &nz);
+    const ull b = __ballot(nz ? 1 : 0);
+    if (__lane_id() == 0 && b) atomicAdd(tile_nz + (v / EMIT_TILE), (uint32_t)__popcll(b));
 }
 
 // The list length is read on the device (*n_ptr, written by the kernel that filled the list), so
 // the host launches it without waiting; the grid strides over the list. Block 0 adds the length
 // to *stars, the count of star codes re-evaluated in the run. With cubes != null, every vertex
 // whose code changed is marked for the path checks (their results read nothing but star codes).
+// tile_nz follows the masks whose nonzero-ness changes.
 template <class T>
 __global__ __launch_bounds__(BLOCK, CODE_BLOCKS_PER_CU) void k_code_list_g(const T* g, Dims d, StarCode* code,
                                                                          const CritMask* crit_f, CritMask* mm,
                                                                          const uint32_t* list, const ull* n_ptr,
                                                                          ull* stars, uint32_t* cubes,
-                                                                         uint32_t* vbits) {
+                                                                         uint32_t* vbits, uint32_t* tile_nz) {
     const ull n = *n_ptr;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(stars, n);
     for (ull i = (ull)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (ull)gridDim.x * BLOCK) {
         const uint32_t v = list[i];
-        const bool changed = code_g_at(g, d, (i64)v, code, crit_f, mm);
+        bool nz_old = false, nz_new = false;
+        const bool changed = code_g_at(g, d, (i64)v, code, crit_f, mm, &nz_old, &nz_new);
+        if (nz_old != nz_new) atomicAdd(tile_nz + (v / EMIT_TILE), nz_new ? 1u : 0xFFFFFFFFu);
         if (changed && cubes) mark_vertex(d, v, cubes, vbits);
     }
 }
@@ -321,8 +336,13 @@ __global__ __launch_bounds__(BLOCK) void k_critf_spread(const StarCode* cf, Dims
 // Dense mismatch detection: every vertex lists the mismatched cells it owns in g, from the
 // per-vertex mismatch masks kept with the g star codes (16 bytes per vertex). Waves without a
 // mismatch skip the list appends.
-__global__ __launch_bounds__(BLOCK) void k_detect(const CritMask* mm, i64 n, Lists4 out, ull* ctr) {
+// A tile whose count of nonzero masks is zero is skipped without reading its masks; tiles_read
+// counts the tiles whose masks were read.
+__global__ __launch_bounds__(BLOCK) void k_detect(const CritMask* mm, i64 n, const uint32_t* tile_nz, Lists4 out,
+                                                  ull* ctr, ull* tiles_read) {
     const i64 base = (i64)blockIdx.x * EMIT_TILE;
+    const bool any = tile_nz[blockIdx.x] != 0;
+    if (any && threadIdx.x == 0) atomicAdd(tiles_read, 1ull);
     uint64_t lo[EMIT_V];
     uint32_t hi[EMIT_V];
 #pragma unroll
@@ -330,7 +350,7 @@ __global__ __launch_bounds__(BLOCK) void k_detect(const CritMask* mm, i64 n, Lis
         const i64 v = base + (i64)k * BLOCK + threadIdx.x;
         lo[k] = 0;
         hi[k] = 0;
-        if (v < n) {
+        if (any && v < n) {
             const CritMask m = mm[v];
             lo[k] = m.lo;
             hi[k] = m.hi;
@@ -833,10 +853,15 @@ __global__ __launch_bounds__(BLOCK) void k_ds_check(ull n, const uint4* dsv, con
 // Connectivity mode (maxlab != null) first compares g's reached maximum with f's label
 // (main_3d.cu:4821-4826). res[2*j] = start tet if the branch fails else -1, res[2*j+1] = the
 // divergent tet.
+// Connectivity mode compares g's maximum with f's label (get_false_as_vpath_connectivity
+// main_3d.cu:4761-4835): maxlab holds f's labels per start face; with lab_from_f the start faces
+// are g's critical faces and f's label is computed here (the maximum f reaches from the tet when
+// f leaves the face critical, -1 otherwise: labels exist only for f's critical faces,
+// main_3d.cu:4689-4759).
 template <class T>
-__global__ __launch_bounds__(BLOCK) void k_as_branch(const uint64_t* fcf, ull n, const int64_t* maxlab, const T* f,
-                                                     const StarCode* cf, const T* g, const StarCode* cg, Dims d,
-                                                     Counts c, int64_t* res, ull* walk_ctr) {
+__global__ __launch_bounds__(BLOCK) void k_as_branch(const uint64_t* fcf, ull n, const int64_t* maxlab, int lab_from_f,
+                                                     const T* f, const StarCode* cf, const T* g, const StarCode* cg,
+                                                     Dims d, Counts c, int64_t* res, ull* walk_ctr) {
     const ull j = (ull)blockIdx.x * BLOCK + threadIdx.x;
     if (j >= 2 * n) return;
     const ull i = j >> 1;
@@ -851,6 +876,10 @@ __global__ __launch_bounds__(BLOCK) void k_as_branch(const uint64_t* fcf, ull n,
     i64 dv = -1;
     bool check = q >= 0;
     if (check && maxlab) check = R.paired_max(q, 1, c.N, ovf) != maxlab[2 * i + k];
+    if (check && lab_from_f) {
+        const i64 lab = R.F.pf(t) == -1 ? R.paired_max(q, 0, c.N, ovf) : -1;
+        check = R.paired_max(q, 1, c.N, ovf) != lab;
+    }
     if (check) R.as_walk(q, c.N, found, dv, ovf);
     res[2 * j] = found ? q : -1;
     res[2 * j + 1] = dv;
@@ -1296,6 +1325,7 @@ struct dmtz_ctx {
     StarCode* d_code_g = nullptr;
     CritMask* d_critf = nullptr;    // f-critical status of every vertex's local cells
     CritMask* d_mm = nullptr;       // mismatched cells each vertex owns in g
+    uint32_t* d_tile_nz = nullptr;  // per EMIT_TILE vertices: how many have a nonzero d_mm
     uint64_t* d_rec = nullptr;    // 4 mismatch record lists of rec_cap entries
     uint64_t* d_fc = nullptr;     // 4 f-critical lists of fc_cap entries
     int64_t* d_path = nullptr;    // ds id/rec, as id/rec, wall id/rec (fc_cap each)
@@ -1320,6 +1350,9 @@ struct dmtz_ctx {
     uint64_t* d_as_sd = nullptr;     // per-step dirty bits
     uint32_t* d_as_fails = nullptr;  // per-branch bit: the last check found a mismatch
     size_t as_off32_cap = 0, as_cube_cap = 0, as_order_cap = 0, as_sd_cap = 0, as_fails_cap = 0;
+    hipStream_t as_stream = nullptr;  // the ascending-path precompute, overlapping the first sweeps
+    hipEvent_t as_ev_fork = nullptr, as_ev_count = nullptr, as_ev_done = nullptr;
+    ull* h_as_total = nullptr;        // pinned: total steps of the precomputed paths
     uint32_t* d_wall_scr = nullptr;  // wall walks: per-thread visited sets and stacks
     size_t wall_scr_cap = 0;
     uint32_t* d_wall_ovf = nullptr;  // wall walks to repeat with more slots (two lists)
@@ -1369,6 +1402,8 @@ void free_bufs(dmtz_ctx* ctx) {
     (void)hipFree(ctx->d_code_g);
     (void)hipFree(ctx->d_critf);
     (void)hipFree(ctx->d_mm);
+    (void)hipFree(ctx->d_tile_nz);
+    ctx->d_tile_nz = nullptr;
     ctx->d_f = ctx->d_g = nullptr;
     ctx->d_counter = nullptr;
     ctx->d_stamp = ctx->d_dstamp = ctx->d_targets = ctx->d_changed = ctx->d_dirty = nullptr;
@@ -1454,6 +1489,7 @@ int ensure(dmtz_ctx* ctx, i64 N, size_t elem, bool want_fields) {
         CK(hipMalloc(&ctx->d_code_g, sizeof(StarCode) * n));
         CK(hipMalloc(&ctx->d_critf, sizeof(CritMask) * n));
         CK(hipMalloc(&ctx->d_mm, sizeof(CritMask) * n));
+        CK(hipMalloc(&ctx->d_tile_nz, sizeof(uint32_t) * (n / EMIT_TILE + 2)));
         ctx->cap_n = N;
         ctx->epoch = 0;
     }
@@ -1591,22 +1627,24 @@ struct Driver {
 
     int detect_once() {
         CK(hipMemsetAsync(ctx->d_ctr + K_MIN, 0, sizeof(ull) * 4, ctx->stream));
+        CK(hipMemsetAsync(ctx->d_ctr + K_DTILES, 0, sizeof(ull), ctx->stream));
         CK(hipEventRecord(ctx->ev0, ctx->stream));
-        k_detect<<<(unsigned)((c.N + EMIT_TILE - 1) / EMIT_TILE), BLOCK, 0, ctx->stream>>>(ctx->d_mm, c.N,
-                                                                                         rec_lists(ctx),
-                                                                                         ctx->d_ctr + K_MIN);
+        k_detect<<<(unsigned)((c.N + EMIT_TILE - 1) / EMIT_TILE), BLOCK, 0, ctx->stream>>>(
+            ctx->d_mm, c.N, ctx->d_tile_nz, rec_lists(ctx), ctx->d_ctr + K_MIN, ctx->d_ctr + K_DTILES);
         CK(hipGetLastError());
         CK(hipEventRecord(ctx->ev1, ctx->stream));
         int rc = sync_ctr(ctx);
         if (rc) return rc;
         float ms = 0;
         CK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-        // bytes the kernel must move: one mismatch mask per voxel read + one 8-byte record
-        // written per mismatch emitted
+        // bytes the kernel must move: the tile counts, the masks of the tiles with a mismatch,
+        // and one 8-byte record written per mismatch emitted
         ull m = ctx->h_ctr[0] + ctx->h_ctr[1] + ctx->h_ctr[2] + ctx->h_ctr[3];
+        const double ntiles = (double)((c.N + EMIT_TILE - 1) / EMIT_TILE);
         ctx->det_launches += 1;
         ctx->det_ms += ms;
-        ctx->det_bytes += (double)c.N * (double)sizeof(CritMask) + 8.0 * (double)m;
+        ctx->det_bytes += 4.0 * ntiles + (double)ctx->h_ctr[K_DTILES] * (double)EMIT_TILE * sizeof(CritMask) +
+                          8.0 * (double)m;
         ctx->det_voxels += (double)c.N;
         return 0;
     }
@@ -1769,7 +1807,7 @@ struct Driver {
         CK(hipGetLastError());
         k_code_list_g<T><<<stride_blocks(15 * est), BLOCK, 0, ctx->stream>>>(
             g, d, ctx->d_code_g, ctx->d_critf, ctx->d_mm, ctx->d_dirty, ctx->d_ctr + K_DIRTY, ctx->d_ctr + K_STARS,
-            as_ready ? ctx->d_cubes : nullptr, as_ready ? ctx->d_vbits : nullptr);
+            as_ready ? ctx->d_cubes : nullptr, as_ready ? ctx->d_vbits : nullptr, ctx->d_tile_nz);
         CK(hipGetLastError());
         return 0;
     }
@@ -1861,7 +1899,10 @@ struct Driver {
     // g star codes and mismatch masks of every vertex (the g gradient, main_3d.cu:6624-6638);
     // needs the f-critical masks.
     int g_codes() {
-        k_code_g<T><<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(g, d, c, ctx->d_code_g, ctx->d_critf, ctx->d_mm);
+        CK(hipMemsetAsync(ctx->d_tile_nz, 0, sizeof(uint32_t) * (size_t)((c.N + EMIT_TILE - 1) / EMIT_TILE),
+                          ctx->stream));
+        k_code_g<T><<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(g, d, c, ctx->d_code_g, ctx->d_critf, ctx->d_mm,
+                                                             ctx->d_tile_nz);
         CK(hipGetLastError());
         return 0;
     }
@@ -1884,9 +1925,17 @@ struct Driver {
     }
 
     ull nfce = 0, nfcf = 0;
+    // Connectivity-mode ascending checks from g's critical faces instead of f's (dmtz_paths on
+    // fields whose critical cells differ; in the correction loop the two sets are equal).
+    const uint64_t* as_gfaces = nullptr;
+    ull n_gfaces = 0;
 
     // f-critical cells of the types in type_mask into the f-critical lists (grown on demand).
-    int fcrit(int type_mask) {
+    int fcrit(int type_mask) { return fcrit_of(ctx->d_code_f, type_mask); }
+
+    // Critical cells of the types in type_mask of the field whose star codes are given, into the
+    // f-critical lists (grown on demand).
+    int fcrit_of(const StarCode* codes, int type_mask) {
         if (ctx->fc_cap == 0) {
             int rc0 = grow_fc(ctx, (ull)(c.N / 16));
             if (rc0) return rc0;
@@ -1895,7 +1944,7 @@ struct Driver {
             Lists4 FL = fcrit_lists(ctx);
             CK(hipMemsetAsync(ctx->d_ctr + K_FC0, 0, sizeof(ull) * 4, ctx->stream));
             k_fcrit<<<(unsigned)((c.N + EMIT_TILE - 1) / EMIT_TILE), BLOCK, 0, ctx->stream>>>(
-                ctx->d_code_f, c, FL, ctx->d_ctr + K_FC0, type_mask);
+                codes, c, FL, ctx->d_ctr + K_FC0, type_mask);
             CK(hipGetLastError());
             int rc = sync_ctr(ctx);
             if (rc) return rc;
@@ -1928,6 +1977,10 @@ struct Driver {
         Lists4 FL = fcrit_lists(ctx);
         const bool conn = fl.preserve_geometry != 1;
         const bool pre = as_precomputable();
+        if (pre && !as_ready && as_stage) {
+            int rc = as_join();
+            if (rc) return rc;
+        }
         if (pre && !as_ready) {
             int rc = as_prepare();
             if (rc) return rc;
@@ -1935,16 +1988,24 @@ struct Driver {
         if (nfcf && pre) {
             int rc = as_check();
             if (rc) return rc;
+        } else if (conn && as_gfaces) {
+            if (n_gfaces) {
+                k_as_branch<T><<<nblocks(2 * n_gfaces), BLOCK, 0, ctx->stream>>>(
+                    as_gfaces, n_gfaces, nullptr, 1, f, ctx->d_code_f, g, ctx->d_code_g, d, c, ctx->d_walk,
+                    ctx->d_ctr + K_WALK);
+                CK(hipGetLastError());
+            }
         } else if (nfcf) {
             const int64_t* lab = conn ? ctx->d_labels + 2 * ctx->fc_cap : nullptr;
-            k_as_branch<T><<<nblocks(2 * nfcf), BLOCK, 0, ctx->stream>>>(FL.l[2], nfcf, lab, f, ctx->d_code_f, g,
+            k_as_branch<T><<<nblocks(2 * nfcf), BLOCK, 0, ctx->stream>>>(FL.l[2], nfcf, lab, 0, f, ctx->d_code_f, g,
                                                                          ctx->d_code_g, d, c, ctx->d_walk,
                                                                          ctx->d_ctr + K_WALK);
             CK(hipGetLastError());
         }
-        if (nfcf) {
-            k_as_combine<<<nblocks(nfcf), BLOCK, 0, ctx->stream>>>(ctx->d_walk, nfcf, path_list(1),
-                                                                   pre && as_compact ? ctx->d_as_fails : nullptr);
+        const ull nstarts = conn && as_gfaces ? n_gfaces : nfcf;
+        if (nstarts) {
+            k_as_combine<<<nblocks(nstarts), BLOCK, 0, ctx->stream>>>(ctx->d_walk, nstarts, path_list(1),
+                                                                      pre && as_compact ? ctx->d_as_fails : nullptr);
             CK(hipGetLastError());
         }
         if (nfce && pre) {
@@ -2040,14 +2101,80 @@ struct Driver {
         Lists4 FL = fcrit_lists(ctx);
         int rc = nb ? as_paths(FL, nb) : 0;
         if (rc) return rc;
-        rc = as_bitmaps_and_ds(FL, (size_t)(c.N >> 5) + 1);
+        rc = as_bitmaps_and_ds(FL, (size_t)(c.N >> 5) + 1, ctx->stream);
         if (rc) return rc;
         as_ready = true;
         as_full = true;
         return 0;
     }
 
+    // The precompute depends on f alone, so single-field runs start it on a stream of its own
+    // right after the first detection and let it overlap the critical-point sweeps; the first
+    // separatrix check waits for it (as_join). as_stage: 0 none in flight, 1 counting walk
+    // enqueued, 2 fill enqueued.
+    int as_stage = 0;
+
+    int as_start_async() {
+        if (!ctx->fix_concurrent || !as_precomputable() || as_ready || !nfcf) return 0;
+        if (!ctx->as_stream) {
+            CK(hipStreamCreateWithFlags(&ctx->as_stream, hipStreamNonBlocking));
+            CK(hipEventCreateWithFlags(&ctx->as_ev_fork, hipEventDisableTiming));
+            CK(hipEventCreateWithFlags(&ctx->as_ev_count, hipEventDisableTiming));
+            CK(hipEventCreateWithFlags(&ctx->as_ev_done, hipEventDisableTiming));
+        }
+        CK(hipEventRecord(ctx->as_ev_fork, ctx->stream));
+        CK(hipStreamWaitEvent(ctx->as_stream, ctx->as_ev_fork, 0));
+        int rc = as_paths_count(fcrit_lists(ctx), 2 * nfcf, ctx->as_stream, ctx->h_as_total);
+        if (rc) return rc;
+        CK(hipEventRecord(ctx->as_ev_count, ctx->as_stream));
+        as_stage = 1;
+        return 0;
+    }
+
+    // Enqueue the fill once the counting walk is done (wait = false: only if it already is).
+    int as_advance(bool wait) {
+        if (as_stage != 1) return 0;
+        if (!wait) {
+            const hipError_t q = hipEventQuery(ctx->as_ev_count);
+            if (q == hipErrorNotReady) return 0;
+            CK(q);
+        } else {
+            CK(hipEventSynchronize(ctx->as_ev_count));
+        }
+        Lists4 FL = fcrit_lists(ctx);
+        int rc = as_paths_fill(FL, 2 * nfcf, ctx->as_stream, *ctx->h_as_total);
+        if (rc) return rc;
+        rc = as_bitmaps_and_ds(FL, (size_t)(c.N >> 5) + 1, ctx->as_stream);
+        if (rc) return rc;
+        CK(hipEventRecord(ctx->as_ev_done, ctx->as_stream));
+        as_stage = 2;
+        return 0;
+    }
+
+    // The context's stream waits for the precompute; the checks can then use it.
+    int as_join() {
+        int rc = as_advance(true);
+        if (rc) return rc;
+        if (as_stage == 2) {
+            CK(hipStreamWaitEvent(ctx->stream, ctx->as_ev_done, 0));
+            as_stage = 0;
+            as_ready = true;
+            as_full = true;
+        }
+        return 0;
+    }
+
+    // The ascending-path precompute in two phases on stream st: the counting walk and the scan of
+    // the step counts (the total lands in *h_total when st gets there), then, with the total on
+    // the host, the ordering, the fill and the compact copies.
     int as_paths(const Lists4& FL, ull nb) {
+        int rc = as_paths_count(FL, nb, ctx->stream, ctx->h_as_total);
+        if (rc) return rc;
+        CK(hipStreamSynchronize(ctx->stream));
+        return as_paths_fill(FL, nb, ctx->stream, *ctx->h_as_total);
+    }
+
+    int as_paths_count(const Lists4& FL, ull nb, hipStream_t st, ull* h_total) {
         int rc = grow_buf(ctx, ctx->d_as_cnt, ctx->as_cnt_cap, nb + 1);
         if (rc) return rc;
         rc = grow_buf(ctx, ctx->d_as_off, ctx->as_off_cap, nb + 1);
@@ -2055,25 +2182,30 @@ struct Driver {
         const int keep = as_keep_steps(nb);
         rc = grow_buf(ctx, ctx->d_as_first, ctx->as_first_cap, nb * (ull)keep);
         if (rc) return rc;
-        CK(hipMemsetAsync(ctx->d_as_cnt + nb, 0, sizeof(ull), ctx->stream));
-        CK(hipMemsetAsync(ctx->d_ctr + K_ASNEXT, 0, sizeof(ull), ctx->stream));
+        CK(hipMemsetAsync(ctx->d_as_cnt + nb, 0, sizeof(ull), st));
+        CK(hipMemsetAsync(ctx->d_ctr + K_ASNEXT, 0, sizeof(ull), st));
         const unsigned wsb = nblocks(nb) < AS_WS_BLOCKS ? nblocks(nb) : AS_WS_BLOCKS;
-        k_as_count_ws<T><<<wsb, BLOCK, 0, ctx->stream>>>(FL.l[2], nb, f, ctx->d_code_f, d, c, ctx->d_as_cnt,
+        k_as_count_ws<T><<<wsb, BLOCK, 0, st>>>(FL.l[2], nb, f, ctx->d_code_f, d, c, ctx->d_as_cnt,
                                                           ctx->d_as_first, keep, ctx->d_ctr + K_ASNEXT,
                                                           ctx->d_ctr + K_WALK);
         CK(hipGetLastError());
         size_t tmp = 0;
-        CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, ctx->d_as_cnt, ctx->d_as_off, (int)(nb + 1), ctx->stream));
+        CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, ctx->d_as_cnt, ctx->d_as_off, (int)(nb + 1), st));
         uint8_t* tp = (uint8_t*)ctx->d_scan_tmp;
         rc = grow_buf(ctx, tp, ctx->scan_tmp_cap, tmp);
         ctx->d_scan_tmp = tp;
         if (rc) return rc;
         CK(hipcub::DeviceScan::ExclusiveSum(ctx->d_scan_tmp, tmp, ctx->d_as_cnt, ctx->d_as_off, (int)(nb + 1),
-                                            ctx->stream));
-        ull total = 0;
-        CK(hipMemcpyAsync(&total, ctx->d_as_off + nb, sizeof(ull), hipMemcpyDeviceToHost, ctx->stream));
-        CK(hipStreamSynchronize(ctx->stream));
-        rc = grow_buf(ctx, ctx->d_as_steps, ctx->as_steps_cap, total);
+                                            st));
+        CK(hipMemcpyAsync(h_total, ctx->d_as_off + nb, sizeof(ull), hipMemcpyDeviceToHost, st));
+        return 0;
+    }
+
+    int as_paths_fill(const Lists4& FL, ull nb, hipStream_t st, ull total) {
+        const int keep = as_keep_steps(nb);
+        size_t tmp = 0;
+        CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, ctx->d_as_cnt, ctx->d_as_off, (int)(nb + 1), st));
+        int rc = grow_buf(ctx, ctx->d_as_steps, ctx->as_steps_cap, total);
         if (rc) return rc;
         as_compact = total < ((ull)1 << 32) && nb < ((ull)1 << 31);
         if (as_compact) {  // branch order by path length (for the fill and the checks)
@@ -2083,28 +2215,28 @@ struct Driver {
             if (rc) return rc;
             rc = grow_buf(ctx, ctx->d_fix_key2, ctx->fix_key2_cap, nb);
             if (rc) return rc;
-            k_as_len_keys<<<nblocks(nb), BLOCK, 0, ctx->stream>>>(nb, ctx->d_as_cnt, ctx->d_fix_key,
+            k_as_len_keys<<<nblocks(nb), BLOCK, 0, st>>>(nb, ctx->d_as_cnt, ctx->d_fix_key,
                                                                    ctx->d_as_order + nb);
             CK(hipGetLastError());
             size_t tmp2 = 0;
             CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp2, ctx->d_fix_key, ctx->d_fix_key2, ctx->d_as_order + nb,
-                                                  ctx->d_as_order, (int)nb, 0, 8, ctx->stream));
+                                                  ctx->d_as_order, (int)nb, 0, 8, st));
             uint8_t* tp2 = (uint8_t*)ctx->d_scan_tmp;
             rc = grow_buf(ctx, tp2, ctx->scan_tmp_cap, tmp2);
             ctx->d_scan_tmp = tp2;
             if (rc) return rc;
             CK(hipcub::DeviceRadixSort::SortPairs(ctx->d_scan_tmp, tmp2, ctx->d_fix_key, ctx->d_fix_key2,
-                                                  ctx->d_as_order + nb, ctx->d_as_order, (int)nb, 0, 8, ctx->stream));
+                                                  ctx->d_as_order + nb, ctx->d_as_order, (int)nb, 0, 8, st));
             // steps in check order: offsets from the counts taken in that order
-            k_as_gather_cnt<<<nblocks(nb + 1), BLOCK, 0, ctx->stream>>>(nb, ctx->d_as_order, ctx->d_as_cnt,
+            k_as_gather_cnt<<<nblocks(nb + 1), BLOCK, 0, st>>>(nb, ctx->d_as_order, ctx->d_as_cnt,
                                                                        ctx->d_as_off);
             CK(hipGetLastError());
             CK(hipcub::DeviceScan::ExclusiveSum(ctx->d_scan_tmp, tmp, ctx->d_as_off, ctx->d_as_cnt, (int)(nb + 1),
-                                                ctx->stream));
+                                                st));
             std::swap(ctx->d_as_cnt, ctx->d_as_off);
             std::swap(ctx->as_cnt_cap, ctx->as_off_cap);
         }
-        k_as_fill<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(FL.l[2], nb, as_compact ? ctx->d_as_order : nullptr, f,
+        k_as_fill<T><<<nblocks(nb), BLOCK, 0, st>>>(FL.l[2], nb, as_compact ? ctx->d_as_order : nullptr, f,
                                                               ctx->d_code_f, d, c, ctx->d_as_off, ctx->d_as_first,
                                                               keep, ctx->d_as_steps);
         CK(hipGetLastError());
@@ -2113,7 +2245,7 @@ struct Driver {
             if (rc) return rc;
             rc = grow_buf(ctx, ctx->d_as_cube, ctx->as_cube_cap, total + 1);
             if (rc) return rc;
-            k_as_compact<<<nblocks(nb), BLOCK, 0, ctx->stream>>>(nb, ctx->d_as_off, ctx->d_as_steps, ctx->d_as_off32,
+            k_as_compact<<<nblocks(nb), BLOCK, 0, st>>>(nb, ctx->d_as_off, ctx->d_as_steps, ctx->d_as_off32,
                                                                   ctx->d_as_cube);
             CK(hipGetLastError());
             rc = grow_buf(ctx, ctx->d_as_sd, ctx->as_sd_cap, (total + 63) / 64 + 1);
@@ -2125,13 +2257,13 @@ struct Driver {
         return 0;
     }
 
-    int as_bitmaps_and_ds(const Lists4& FL, size_t words) {
+    int as_bitmaps_and_ds(const Lists4& FL, size_t words, hipStream_t st) {
         int rc = grow_buf(ctx, ctx->d_cubes, ctx->cubes_cap, words);
         if (rc) return rc;
         rc = grow_buf(ctx, ctx->d_vbits, ctx->vbits_cap, words);
         if (rc) return rc;
-        CK(hipMemsetAsync(ctx->d_cubes, 0, sizeof(uint32_t) * words, ctx->stream));
-        CK(hipMemsetAsync(ctx->d_vbits, 0, sizeof(uint32_t) * words, ctx->stream));
+        CK(hipMemsetAsync(ctx->d_cubes, 0, sizeof(uint32_t) * words, st));
+        CK(hipMemsetAsync(ctx->d_vbits, 0, sizeof(uint32_t) * words, st));
         if (nfce) {
             rc = grow_buf(ctx, ctx->d_dsv, ctx->dsv_cap, nfce);
             if (rc) return rc;
@@ -2139,7 +2271,7 @@ struct Driver {
             if (rc) return rc;
             rc = grow_buf(ctx, ctx->d_ds_res, ctx->ds_res_cap, nfce);
             if (rc) return rc;
-            k_ds_prepare<T><<<nblocks(nfce), BLOCK, 0, ctx->stream>>>(FL.l[1], nfce, f, ctx->d_code_f, d, c,
+            k_ds_prepare<T><<<nblocks(nfce), BLOCK, 0, st>>>(FL.l[1], nfce, f, ctx->d_code_f, d, c,
                                                                        ctx->d_dsv, ctx->d_dse);
             CK(hipGetLastError());
         }
@@ -2221,6 +2353,7 @@ struct Driver {
     // leaves. Without the finisher the run returns DMTZ_STALLED there; with it, finish_step() is
     // applied and the sweep's counts are re-evaluated.
     int run(dmtz_counts* out, int32_t* sweeps_out) {
+        if (ctx->as_stream) CK(hipStreamSynchronize(ctx->as_stream));  // after a failed earlier run
         ctx->traj.clear();
         for (double& p : ctx->phase) p = 0;
         ctx->stalls = ctx->finished = ctx->radius = 0;
@@ -2251,6 +2384,10 @@ struct Driver {
         if (rc) return rc;
         ctx->phase[1] += secs(t1, Clock::now());
         i64 cnt[8] = {(i64)ctx->h_ctr[0], (i64)ctx->h_ctr[1], (i64)ctx->h_ctr[2], (i64)ctx->h_ctr[3], 1, 1, 1, 0};
+        if (paths) {
+            rc = as_start_async();
+            if (rc) return rc;
+        }
         i64 sweeps = 0;
         int status = 0;
         const char* lg = std::getenv("DMTZ_LOG");
@@ -2354,6 +2491,8 @@ struct Driver {
                     status = DMTZ_STALLED;
                     goto done;
                 }
+                rc = as_advance(false);
+                if (rc) return rc;
             }
             if (!paths) break;
             if (sweeps >= maxs) {
@@ -2384,6 +2523,10 @@ struct Driver {
         }
     done:
         if (!paths) cnt[4] = cnt[5] = cnt[6] = cnt[7] = 0;
+        if (as_stage) {  // a precompute the run did not need: let it finish, it is not used
+            CK(hipStreamSynchronize(ctx->as_stream));
+            as_stage = 0;
+        }
         CK(hipStreamSynchronize(ctx->stream));
         ctx->phase[5] = secs(t_start, Clock::now());
         if (out) {
@@ -2697,9 +2840,28 @@ int paths_host(dmtz_ctx* ctx, const T* f, const T* g, i64 nx, i64 ny, i64 nz, co
     if (rc) return rc;
     rc = drv.g_codes();
     if (rc) return rc;
+    uint64_t* gfaces = nullptr;
+    if (fl.preserve_geometry != 1) {
+        // get_false_as_vpath_connectivity starts from g's critical faces (main_3d.cu:4766)
+        rc = drv.fcrit_of(ctx->d_code_g, 4);
+        if (rc) return rc;
+        drv.n_gfaces = ctx->h_ctr[K_FC0 + 2];
+        CK(hipMalloc(&gfaces, sizeof(uint64_t) * (drv.n_gfaces + 1)));
+        hipError_t e = hipMemcpyAsync(gfaces, fcrit_lists(ctx).l[2], sizeof(uint64_t) * drv.n_gfaces,
+                                      hipMemcpyDeviceToDevice, ctx->stream);
+        if (e != hipSuccess) {
+            (void)hipFree(gfaces);
+            ctx->err = hipGetErrorString(e);
+            return DMTZ_E_HIP;
+        }
+        drv.as_gfaces = gfaces;
+    }
     rc = drv.prepare_paths();
-    if (rc) return rc;
-    rc = drv.detect_paths();
+    if (rc == 0) rc = drv.detect_paths();
+    if (gfaces) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(gfaces);
+    }
     if (rc) return rc;
     ull nds = ctx->h_ctr[K_DS], nas = ctx->h_ctr[K_AS];
     counts2[0] = (int64_t)nds;

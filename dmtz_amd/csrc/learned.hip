@@ -631,6 +631,37 @@ int dmtz_net_train_step(dmtz_ctx* ctx, dmtz_net* net, const uint16_t* d_feat, co
     return read_loss(ctx, net, (long)(nx * ny * nz), loss_out);
 }
 
+int dmtz_correct_batch_learned_f32(dmtz_ctx* ctx, dmtz_net* net, const dmtz_field_f32* fields, int32_t nfields,
+                                   const dmtz_flags* flags, dmtz_counts* counts_out, int32_t* status_out,
+                                   int32_t* sweeps_out, int64_t* warm_edits_out) {
+    if (!ctx || !net || !flags || nfields < 0 || (nfields > 0 && !fields)) return DMTZ_E_ARG;
+    for (int32_t i = 0; i < nfields; i++) {
+        const dmtz_field_f32& fd = fields[i];
+        if (!fd.on_device || !fd.f || !fd.g) {
+            dmtz_internal::set_error(ctx, "batched learned correction takes device fields only");
+            return DMTZ_E_ARG;
+        }
+        int rc = check_grid(ctx, fd.nx, fd.ny, fd.nz);
+        if (rc) return rc;
+        const long n = (long)(fd.nx * fd.ny * fd.nz);
+        rc = ensure_acts(ctx, net, n);
+        if (rc) return rc;
+        rc = dmtz_net_features_f32(ctx, fd.g, fd.nx, fd.ny, fd.nz, fd.eps, fd.range, net->feat);
+        if (rc) return rc;
+        rc = forward(ctx, net, net->feat, fd.nx, fd.ny, fd.nz, net->z);
+        if (rc) return rc;
+        hipStream_t s = dmtz_internal::stream(ctx);
+        NK(hipMemsetAsync(net->edits, 0, sizeof(unsigned long long), s));
+        k_warm<<<blocks_for(n), BLOCK, 0, s>>>(fd.f, fd.g, net->z, n, fd.eps, net->edits, net->warm);
+        NK(hipGetLastError());
+        unsigned long long ne = 0;
+        NK(hipMemcpyAsync(&ne, net->edits, sizeof(ne), hipMemcpyDeviceToHost, s));
+        NK(hipStreamSynchronize(s));
+        if (warm_edits_out) warm_edits_out[i] = (int64_t)ne;
+    }
+    return dmtz_correct_batch_f32(ctx, fields, nfields, flags, counts_out, status_out, sweeps_out);
+}
+
 int dmtz_correct_learned_f32(dmtz_ctx* ctx, dmtz_net* net, const float* d_f, float* d_g, int64_t nx, int64_t ny,
                              int64_t nz, double eps, double range, const dmtz_flags* flags, dmtz_counts* out,
                              uint8_t* d_counter_out, int32_t* sweeps_out, int64_t* warm_edits_out) {

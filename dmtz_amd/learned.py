@@ -145,3 +145,32 @@ def correct_learned(net, f, g, eps, rng, flags=None, counter_out=None):
     res = CorrectionResult(g, counts.as_tuple(), int(sweeps.value), rc, None, ctx.trajectory(), ctx.phases(),
                            ctx.run_info())
     return res, int(warm.value)
+
+
+def correct_batch_learned(net, items, flags=None):
+    """The batched form with the warm start: items is a list of (f, g, eps, rng) float32 device
+    buffers (g corrected in place). Returns (list of CorrectionResult, warm edits per field)."""
+    from .api import _batch_results
+
+    ctx = net.ctx
+    flags = flags or CorrectionFlags()
+    n = len(items)
+    arr = (N.FieldF32 * n)()
+    for i, (f, g, eps, rng) in enumerate(items):
+        pf, shape, tf = _device_arg(f)
+        pg, gshape, tg = _device_arg(g)
+        if tf != "f32" or tg != "f32" or gshape != shape or len(shape) != 3:
+            raise ValueError("correct_batch_learned: field %d must be 3-D float32 device buffers of one shape" % i)
+        nz, ny, nx = shape
+        arr[i] = N.FieldF32(pf, pg, nx, ny, nz, float(eps), float(rng), 1, 0)
+    counts = (N.Counts * n)()
+    status = np.zeros(n, np.int32)
+    sweeps = np.zeros(n, np.int32)
+    warm = np.zeros(n, np.int64)
+    fl = flags.c()
+    rc = net._lib.dmtz_correct_batch_learned_f32(ctx.handle, net.handle, arr, n, ctypes.byref(fl),
+                                                 ctypes.cast(counts, ctypes.c_void_p),
+                                                 status.ctypes.data_as(ctypes.c_void_p),
+                                                 sweeps.ctypes.data_as(ctypes.c_void_p),
+                                                 warm.ctypes.data_as(ctypes.c_void_p))
+    return _batch_results(ctx, rc, [it[1] for it in items], counts, status, sweeps), warm

@@ -15,6 +15,10 @@ import time
 
 import numpy as np
 
+# index of the edit logit's bias in the parameters: 4 layers of (W[27][16][16], b[16]); the last
+# layer's output channel 0 is the edit logit
+LOGIT_BIAS = 3 * (27 * 16 * 16 + 16) + 27 * 16 * 16
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -37,6 +41,12 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=12)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--lam", type=float, default=0.5)
+    ap.add_argument("--pos-weight", type=float, default=0.0,
+                    help="weight of the edited class in the loss; below 1 a false edit costs more than a missed one "
+                         "(0: sqrt(1 / edited fraction), capped to [1, 20])")
+    ap.add_argument("--logit-shift", type=float, default=0.0,
+                    help="subtract this from the edit logit's bias of the saved parameters: the warm start then "
+                         "edits only vertices with logit above it")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default="")
     ap.add_argument("--init", default="", help="start from these parameters (.npy)")
@@ -72,7 +82,7 @@ def main(argv=None):
         if i % 10 == 9:
             log("pairs %d/%d (%.1f s)" % (i + 1, args.fields, time.perf_counter() - t0))
     ypos = float(np.mean([lab[..., 0].mean().item() for _, lab in data]))
-    pos = float(min(20.0, max(1.0, np.sqrt(1.0 / max(ypos, 1e-4)))))
+    pos = args.pos_weight if args.pos_weight > 0 else float(min(20.0, max(1.0, np.sqrt(1.0 / max(ypos, 1e-4)))))
     log("edited fraction %.4f, pos_weight %.2f" % (ypos, pos))
     steps = args.epochs * len(data)
     k = 0
@@ -85,6 +95,10 @@ def main(argv=None):
             k += 1
         log("epoch %d: bce %.4f size %.4f (%.1f s)" % (e, tot[0] / len(data), tot[1] / len(data),
                                                        time.perf_counter() - t0))
+    if args.logit_shift:
+        p = net.params()
+        p[LOGIT_BIAS] -= args.logit_shift
+        net.set_params(p)
     if args.out:
         np.save(args.out, net.params())
         log("saved %s" % args.out)
@@ -112,9 +126,12 @@ def main(argv=None):
         warm, edits = L.correct_learned(net, f, g2, eps, rng, flags)
         c = time.perf_counter()
         ok = plain.converged and warm.converged and (g2.double() - f.double()).abs().max().item() <= eps
+        n = g.numel()
         row = dict(field=kind, size=args.eval_size, rel_eps=rel, plain_sweeps=plain.sweeps,
                    plain_ms=round((b - a) * 1e3, 2), warm_sweeps=warm.sweeps, warm_ms=round((c - b) * 1e3, 2),
-                   warm_edits=edits, speedup=round((b - a) / (c - b), 3), converged=bool(ok))
+                   warm_edits=edits, speedup=round((b - a) / (c - b), 3), converged=bool(ok),
+                   plain_edit_ratio=round((g1 != g).sum().item() / n, 4),
+                   warm_edit_ratio=round((g2 != g).sum().item() / n, 4))
         print(json.dumps(row), flush=True)
         summary.append(row)
     if summary:

@@ -38,6 +38,7 @@ extern "C" {
 #define DMTZ_E_WALK (-5)
 /* Batch status of a field that never ran because an earlier field failed with an error. */
 #define DMTZ_E_NOT_RUN (-6)
+#define DMTZ_E_IO (-7)
 
 /* The reference's positional flags (main_3d.cu:6264-6271) plus a sweep cap and a finisher.
  * max_sweeps <= 0 selects the default cap (100000). mode: 0 = plain loop.
@@ -153,6 +154,57 @@ int dmtz_classify_f32(dmtz_ctx* ctx, const float* field, int64_t nx, int64_t ny,
                       uint64_t* crit_lo, uint32_t* crit_hi);
 int dmtz_classify_f64(dmtz_ctx* ctx, const double* field, int64_t nx, int64_t ny, int64_t nz, int64_t* counts4,
                       uint64_t* crit_lo, uint32_t* crit_hi);
+
+/* Edit stream of a correction: the reference's encodings of what a correction changed, built on
+ * the device from the decompressed field d_g0, the corrected field d_g1 and (edit_type 1) the
+ * counter output d_counter, n values each (device pointers on ctx's device).
+ *   edit_type 1, cost (main_3d.cu:1494-1646): d_index receives a 3-bit index per vertex packed
+ *     least significant bit first (encodeTo3BitBitmap, main_3d.cu:1446-1476; (3n + 7) / 8 bytes):
+ *     0 = unchanged, k = 1..5 = the k - 1 steps -2^-s eps of the counter-driven edit, 6 = value
+ *     carried in d_edits; d_edits receives g1 - g0 (double) per index 6 in vertex order (the
+ *     reference stores (f - eps) - g0 there, the same double for its snap, main_3d.cu:1507).
+ *   edit_type 0, original_cost (main_3d.cu:1649-1757): d_index receives the changed vertices'
+ *     indices as int32 differences (the first index, then the gap to the previous), 4 bytes each;
+ *     d_edits receives g1 - g0 per changed vertex.
+ * A changed vertex whose counter does not replay to its value (counters reset by filtration,
+ * main_3d.cu:6782; learned warm edits; the finisher's counter 7) is written as index 6 with its
+ * value, so every stream decodes back to d_g1 with dmtz_edit_apply_*. out receives the sizes:
+ * edited = changed vertices (the reference's edit_ratio is edited / n, main_3d.cu:1553),
+ * index_bytes and n_edits. When index_cap or edits_cap is below them (or a buffer is null) the
+ * sizes are still returned with DMTZ_E_CAPACITY. */
+typedef struct dmtz_edit_stream {
+    int64_t edited, index_bytes, n_edits;
+} dmtz_edit_stream;
+int dmtz_edit_stream_f32(dmtz_ctx* ctx, const float* d_g0, const float* d_g1, const uint8_t* d_counter, int64_t n,
+                         double eps, int32_t edit_type, uint8_t* d_index, int64_t index_cap, double* d_edits,
+                         int64_t edits_cap, dmtz_edit_stream* out);
+int dmtz_edit_stream_f64(dmtz_ctx* ctx, const double* d_g0, const double* d_g1, const uint8_t* d_counter, int64_t n,
+                         double eps, int32_t edit_type, uint8_t* d_index, int64_t index_cap, double* d_edits,
+                         int64_t edits_cap, dmtz_edit_stream* out);
+/* Decoding of an edit stream: d_g holds the decompressed field on entry and the corrected field on
+ * return (no counterpart in the reference, which only writes the streams). */
+int dmtz_edit_apply_f32(dmtz_ctx* ctx, float* d_g, int64_t n, double eps, int32_t edit_type, const uint8_t* d_index,
+                        int64_t index_bytes, const double* d_edits, int64_t n_edits);
+int dmtz_edit_apply_f64(dmtz_ctx* ctx, double* d_g, int64_t n, double eps, int32_t edit_type, const uint8_t* d_index,
+                        int64_t index_bytes, const double* d_edits, int64_t n_edits);
+
+/* Run-length code of a bitmap on the device (runLengthEncodeBitmap main_3d.cu:1399-1426): the
+ * bits of every byte from the most significant down, as (bit, run length) pairs of int32 in
+ * d_pairs (2 * cap_runs ints); an empty bitmap gives no runs. n_runs receives the run count;
+ * DMTZ_E_CAPACITY when cap_runs is smaller (call with d_pairs = NULL to size it). At most 2^28 - 1
+ * bytes, so that every run length fits the reference's int. */
+int dmtz_bitmap_rle(dmtz_ctx* ctx, const uint8_t* d_bitmap, int64_t nbytes, int32_t* d_pairs, int64_t cap_runs,
+                    int64_t* n_runs);
+
+/* Field files. getdata2 (main_3d.cu:153-181): a raw float64 file without header, in host byte
+ * order (little endian here); n_out receives the number of doubles, and out (nullable, cap
+ * doubles) the values. saveVectorToBin (main_3d.cu:1315-1327): an 8-byte count (size_t) followed
+ * by the doubles; dmtz_read_bin_f64 reads such a file back. DMTZ_E_IO when a file cannot be
+ * opened, read or written or is malformed; DMTZ_E_CAPACITY when cap is too small (n_out is still
+ * set). */
+int dmtz_read_raw_f64(const char* path, double* out, int64_t cap, int64_t* n_out);
+int dmtz_write_bin_f64(const char* path, const double* data, int64_t n);
+int dmtz_read_bin_f64(const char* path, double* out, int64_t cap, int64_t* n_out);
 
 /* Per-sweep counts of the last correction on ctx: 8 int64 per sweep (dmtz_counts order), up to
  * cap sweeps; returns the number of sweeps recorded. Mirrors the per-sweep prints of

@@ -76,6 +76,14 @@ int dmtz_correct_learned_f32(dmtz_ctx* ctx, dmtz_net* net, const float* d_f, flo
                              int64_t nz, double eps, double range, const dmtz_flags* flags, dmtz_counts* out,
                              uint8_t* d_counter_out, int32_t* sweeps_out, int64_t* warm_edits_out);
 
+/* The batched form with the learned warm start: each field of the array (device pointers only,
+ * on_device != 0) gets the warm edit of dmtz_correct_learned_f32, one field after another on the
+ * context's stream, and then all fields run through dmtz_correct_batch_f32 with the given flags,
+ * which returns as there. warm_edits_out (nullable) receives one count per field. */
+int dmtz_correct_batch_learned_f32(dmtz_ctx* ctx, dmtz_net* net, const dmtz_field_f32* fields, int32_t nfields,
+                                   const dmtz_flags* flags, dmtz_counts* counts_out, int32_t* status_out,
+                                   int32_t* sweeps_out, int64_t* warm_edits_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -78,6 +78,14 @@ def lib():
         L.orc_coverage.argtypes = [ctypes.c_int, P, ctypes.c_int]
         L.orc_coverage_name.argtypes = [ctypes.c_int]
         L.orc_coverage_name.restype = ctypes.c_char_p
+        L.orc_encode3.argtypes = [P, I64, P, I64]
+        L.orc_encode3.restype = I64
+        L.orc_rle_bitmap.argtypes = [P, I64, P, I64]
+        L.orc_rle_bitmap.restype = I64
+        L.orc_cost_streams.argtypes = [P, P, P, P, I64, ctypes.c_double, P, P, I64, P]
+        L.orc_cost_streams.restype = I64
+        L.orc_original_cost_streams.argtypes = [P, P, I64, P, P, I64]
+        L.orc_original_cost_streams.restype = I64
         _LIB = L
     return _LIB
 
@@ -258,3 +266,44 @@ def correct(f, g, eps, rng, fl=None, traj_cap=4096):
                 phases=dict(zip(("gradient_pairing", "get_f_cp", "get_f_path", "fix_f_cp",
                                  "fix_f_vpath"), phases[:5].tolist())),
                 stalls=int(info[0]), finished=int(info[1]), radius=int(info[2]))
+
+
+def encode3(values):
+    """encodeTo3BitBitmap (main_3d.cu:1446-1476) of int values 0..6; None when one is outside."""
+    v = np.ascontiguousarray(values, dtype=np.int32)
+    out = np.zeros((3 * v.size + 7) // 8 + 1, np.uint8)
+    k = lib().orc_encode3(_p(v), v.size, _p(out), out.size)
+    return None if k < 0 else out[:k].tobytes()
+
+
+def rle_bitmap(bitmap):
+    """runLengthEncodeBitmap (main_3d.cu:1399-1426): (bit, run length) pairs, int32, shape (runs, 2)."""
+    b = np.frombuffer(bytes(bitmap), np.uint8).copy()
+    L = lib()
+    runs = L.orc_rle_bitmap(_p(b), b.size, None, 0)
+    out = np.zeros((max(runs, 1), 2), np.int32)
+    L.orc_rle_bitmap(_p(b), b.size, _p(out), runs)
+    return out[:runs]
+
+
+def cost_streams(f, g0, g1, counter, bound):
+    """The index and value streams of cost (main_3d.cu:1494-1510, float64 fields): the per-vertex
+    indices (counter of changed vertices), their bitmap, and the index-6 values; plus the changed
+    vertex count."""
+    f, g0, g1 = (np.ascontiguousarray(a, np.float64).ravel() for a in (f, g0, g1))
+    c = np.ascontiguousarray(counter, np.uint8).ravel()
+    idx = np.zeros(f.size, np.int32)
+    edits = np.zeros(f.size, np.float64)
+    m = ctypes.c_int64(0)
+    cnt = lib().orc_cost_streams(_p(f), _p(g0), _p(g1), _p(c), f.size, float(bound), _p(idx), _p(edits), edits.size,
+                                 ctypes.byref(m))
+    return dict(index=idx, bitmap=encode3(idx), edits=edits[: m.value].copy(), edited=int(cnt))
+
+
+def original_cost_streams(g0, g1):
+    """The index differences and values of original_cost (main_3d.cu:1649-1671)."""
+    g0, g1 = (np.ascontiguousarray(a, np.float64).ravel() for a in (g0, g1))
+    diffs = np.zeros(g0.size, np.int32)
+    edits = np.zeros(g0.size, np.float64)
+    m = lib().orc_original_cost_streams(_p(g0), _p(g1), g0.size, _p(diffs), _p(edits), g0.size)
+    return dict(diffs=diffs[:m].copy(), edits=edits[:m].copy(), edited=int(m))
