@@ -86,6 +86,7 @@ def lib():
         getattr(L, "dmtz_walls_" + t).argtypes = [P, P, P, I64, I64, I64, D, D, ctypes.POINTER(Flags), P, P, P,
                                                   I64, P]
         getattr(L, "dmtz_correct_batch_" + t).argtypes = [P, P, I32, ctypes.POINTER(Flags), P, P, P]
+    L.dmtz_classify_noalloc_f32.argtypes = [P, P, I64, I64, I64, P, P, P]
     for t in ("f32", "f64"):
         getattr(L, "dmtz_edit_stream_" + t).argtypes = [P, P, P, P, I64, D, I32, P, I64, P, I64,
                                                         ctypes.POINTER(EditStream)]
@@ -127,7 +128,7 @@ EXPORTS = (
     "dmtz_gradient_f32", "dmtz_gradient_f64",
     "dmtz_targets_f32", "dmtz_targets_f64",
     "dmtz_paths_f32", "dmtz_paths_f64",
-    "dmtz_classify_f32", "dmtz_classify_f64",
+    "dmtz_classify_f32", "dmtz_classify_f64", "dmtz_classify_noalloc_f32",
     "dmtz_correct_batch_f32", "dmtz_correct_batch_f64",
     "dmtz_edit_stream_f32", "dmtz_edit_stream_f64", "dmtz_edit_apply_f32", "dmtz_edit_apply_f64", "dmtz_bitmap_rle",
     "dmtz_read_raw_f64", "dmtz_write_bin_f64", "dmtz_read_bin_f64",

@@ -28,7 +28,7 @@ HIPCC_FLAGS = [
 ]
 
 # one translation unit per field type for the correction path, so they compile in parallel
-SOURCES = ["dmtz_core.hip", "dmtz_f32.hip", "dmtz_f64.hip", "dmtz_codec.hip", "learned.hip"]
+SOURCES = ["dmtz_core.hip", "dmtz_f32.hip", "dmtz_f64.hip", "dmtz_codec.hip", "dmtz_noalloc.hip", "learned.hip"]
 
 
 def hipcc():

@@ -3,6 +3,10 @@
 
 #include "driver.h"
 
+namespace dmtz_internal {
+int noalloc_setup(int device);  // dmtz_noalloc.hip: device address of its scratch array
+}
+
 extern "C" {
 
 int dmtz_ctx_create(dmtz_ctx** out, int device) {
@@ -17,6 +21,7 @@ int dmtz_ctx_create(dmtz_ctx** out, int device) {
     if (e == hipSuccess) e = hipHostMalloc(&ctx->h_as_total, sizeof(ull));
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
+    if (e == hipSuccess && dmtz_internal::noalloc_setup(device) != 0) e = hipErrorInvalidSymbol;
     if (e != hipSuccess) {
         std::fprintf(stderr, "dmtz_ctx_create: %s\n", hipGetErrorString(e));
         delete ctx;

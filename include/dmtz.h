@@ -155,6 +155,17 @@ int dmtz_classify_f32(dmtz_ctx* ctx, const float* field, int64_t nx, int64_t ny,
 int dmtz_classify_f64(dmtz_ctx* ctx, const double* field, int64_t nx, int64_t ny, int64_t nz, int64_t* counts4,
                       uint64_t* crit_lo, uint32_t* crit_hi);
 
+/* No counterpart in the reference: dmtz_classify_f32 for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * d_field, d_crit_lo and d_crit_hi (both nullable) are device pointers on ctx's device; counts4
+ * is host memory; the outputs mean what they mean for dmtz_classify_f32. The call allocates
+ * nothing: its scratch (star codes and counters) is a fixed device array of the library, 64 MiB
+ * per GPU for at most 2^21 vertices (128^3), held until the process ends; larger grids return
+ * DMTZ_E_CAPACITY. All calls on one GPU share that array, so calls that overlap on one GPU
+ * (from different contexts or threads) race. */
+int dmtz_classify_noalloc_f32(dmtz_ctx* ctx, const float* d_field, int64_t nx, int64_t ny, int64_t nz,
+                              int64_t* counts4, uint64_t* d_crit_lo, uint32_t* d_crit_hi);
+
 /* Edit stream of a correction: the reference's encodings of what a correction changed, built on
  * the device from the decompressed field d_g0, the corrected field d_g1 and (edit_type 1) the
  * counter output d_counter, n values each (device pointers on ctx's device).
