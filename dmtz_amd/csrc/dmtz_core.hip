@@ -46,15 +46,16 @@ void dmtz_ctx_destroy(dmtz_ctx* ctx) {
     (void)hipFree(ctx->d_fix_key);
     (void)hipFree(ctx->d_fix_key2);
     for (int k = 0; k < 3; k++) {
-        if (ctx->fix_stream[k]) (void)hipStreamSynchronize(ctx->fix_stream[k]);
         (void)hipFree(ctx->d_fk[k]);
-        (void)hipFree(ctx->d_fk2[k]);
         (void)hipFree(ctx->d_fsorted[k]);
         (void)hipFree(ctx->d_ftmp[k]);
-        if (ctx->fix_join[k]) (void)hipEventDestroy(ctx->fix_join[k]);
-        if (ctx->fix_stream[k]) (void)hipStreamDestroy(ctx->fix_stream[k]);
+    }
+    if (ctx->fix_stream) {
+        (void)hipStreamSynchronize(ctx->fix_stream);
+        (void)hipStreamDestroy(ctx->fix_stream);
     }
     if (ctx->fix_fork) (void)hipEventDestroy(ctx->fix_fork);
+    if (ctx->fix_join) (void)hipEventDestroy(ctx->fix_join);
     if (ctx->as_stream) {
         (void)hipStreamSynchronize(ctx->as_stream);
         (void)hipStreamDestroy(ctx->as_stream);

@@ -517,7 +517,63 @@ __global__ __launch_bounds__(BLOCK) void k_fix_class(const uint64_t* recs, ull n
     key[i] = (uint16_t)k;
 }
 
-// One kernel per cell type, so each holds only its own repair rule.
+// Grouping of one kind's records by branch class: a counting sort over the 2^FIX_CLASS_BITS
+// classes. Each block of CS_PER_BLOCK records counts its classes (k_class_hist, class-major
+// counts, so that their exclusive scan is every (class, block)'s first output slot), then
+// scatters its records there (k_class_scatter). Lanes of a wave that share a class claim their
+// slots with one LDS atomic. No pass waits on another block's progress, so the sort runs beside
+// other kernels without stalling.
+constexpr int CS_BINS = 1 << FIX_CLASS_BITS;
+constexpr int CS_V = 8;
+constexpr ull CS_PER_BLOCK = (ull)BLOCK * CS_V;
+
+// Slot of this lane among the lanes of its wave with the same bin, from a per-bin LDS counter:
+// one LDS atomic per distinct bin in the wave. Lanes with bin < 0 take no slot.
+__device__ __forceinline__ uint32_t wave_bin_claim(int bin, uint32_t* h) {
+    uint32_t slot = 0;
+    ull todo = __ballot(bin >= 0);
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const int b = __shfl(bin, leader, 64);
+        const ull same = __ballot(bin == b);
+        uint32_t base = 0;
+        if (__lane_id() == leader) base = atomicAdd(h + b, (uint32_t)__popcll(same));
+        base = __shfl(base, leader, 64);
+        if (bin == b) slot = base + (uint32_t)__popcll(same & ((1ull << __lane_id()) - 1ull));
+        todo &= ~same;
+    }
+    return slot;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_class_hist(const uint16_t* key, ull n, ull nblk, uint32_t* hist) {
+    __shared__ uint32_t h[CS_BINS];
+    if (threadIdx.x < CS_BINS) h[threadIdx.x] = 0;
+    __syncthreads();
+    const ull base = (ull)blockIdx.x * CS_PER_BLOCK;
+    for (int k = 0; k < CS_V; k++) {
+        const ull i = base + (ull)k * BLOCK + threadIdx.x;
+        (void)wave_bin_claim(i < n ? (int)(key[i] & (CS_BINS - 1)) : -1, h);
+    }
+    __syncthreads();
+    if (threadIdx.x < CS_BINS) hist[(ull)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_class_scatter(const uint16_t* key, const uint64_t* recs, ull n, ull nblk,
+                                                         const uint32_t* first, uint64_t* out) {
+    __shared__ uint32_t h[CS_BINS];
+    if (threadIdx.x < CS_BINS) h[threadIdx.x] = first[(ull)threadIdx.x * nblk + blockIdx.x];
+    __syncthreads();
+    const ull base = (ull)blockIdx.x * CS_PER_BLOCK;
+    for (int k = 0; k < CS_V; k++) {
+        const ull i = base + (ull)k * BLOCK + threadIdx.x;
+        const int bin = i < n ? (int)(key[i] & (CS_BINS - 1)) : -1;
+        const uint32_t slot = wave_bin_claim(bin, h);
+        if (bin >= 0) out[slot] = recs[i];
+    }
+}
+
+// One kernel per cell type, so each holds only its own repair rule (one kernel over all types
+// needs twice the registers of the largest rule, and half the waves).
 template <class T, int TYPE>
 __global__ __launch_bounds__(BLOCK) void k_fix_cp(const uint64_t* recs, ull n, const T* f, const StarCode* cf,
                                                   const T* g, const StarCode* cg, Dims d, Counts c,
@@ -739,6 +795,70 @@ __global__ __launch_bounds__(BLOCK) void k_dirty(const uint32_t* changed, const 
     }
 }
 
+// Dense form of k_dirty for long change lists, where one returning atomic per (changed vertex,
+// star slot) costs more than a pass over all stamps: every pair stores the epoch into its star
+// vertex's stamp with a plain store (all writers store the same value), then k_dirty_collect
+// lists the vertices whose stamp holds the epoch, in vertex order. Same set as k_dirty.
+__global__ __launch_bounds__(BLOCK) void k_dirty_mark(const uint32_t* changed, const ull* n_ptr, Dims d,
+                                                      uint32_t* stamp, uint32_t epoch) {
+    const ull n = *n_ptr;
+    for (ull j = (ull)blockIdx.x * BLOCK + threadIdx.x; j < 15 * n; j += (ull)gridDim.x * BLOCK) {
+        const i64 v = changed[j / 15];
+        const int slot = (int)(j % 15);
+        if (slot == 14) {
+            stamp[v] = epoch;
+            continue;
+        }
+        i64 x, y, z;
+        vxyz(d, v, x, y, z);
+        const i64 nx = x + DIR[slot][0], ny = y + DIR[slot][1], nz = z + DIR[slot][2];
+        if (inb(d, nx, ny, nz)) stamp[vidx(d, nx, ny, nz)] = epoch;
+    }
+}
+
+// DIRTY_QUADS groups of four vertices per thread, one 16-byte load of their stamps each; the
+// loads of a wave are contiguous. Space in the list is claimed with one atomic per block (a wave
+// per block would make the claims on the one counter the cost of the pass).
+constexpr int DIRTY_QUADS = 16;
+__global__ __launch_bounds__(BLOCK) void k_dirty_collect(const uint32_t* stamp, i64 n, uint32_t epoch,
+                                                         uint32_t* dirty, ull* dirty_ctr) {
+    __shared__ int wsum[BLOCK / 64];
+    __shared__ ull bbase;
+    const i64 base = (i64)blockIdx.x * BLOCK * 4 * DIRTY_QUADS;
+    uint64_t m = 0;  // bit 4k + j: vertex base + 4 (k BLOCK + threadIdx.x) + j
+    for (int k = 0; k < DIRTY_QUADS; k++) {
+        const i64 v0 = base + 4 * ((i64)k * BLOCK + threadIdx.x);
+        uint64_t q = 0;
+        if (v0 + 3 < n) {
+            const uint4 u = *reinterpret_cast<const uint4*>(stamp + v0);
+            q = (uint64_t)(u.x == epoch) | (uint64_t)(u.y == epoch) << 1 | (uint64_t)(u.z == epoch) << 2 |
+                (uint64_t)(u.w == epoch) << 3;
+        } else {
+            for (int j = 0; j < 4; j++)
+                if (v0 + j < n && stamp[v0 + j] == epoch) q |= (uint64_t)1 << j;
+        }
+        m |= q << (4 * k);
+    }
+    int wtotal;
+    const int pre = wave_excl_scan(__popcll(m), wtotal);
+    const int w = (int)(threadIdx.x >> 6);
+    if (__lane_id() == 0) wsum[w] = wtotal;
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int i = 0; i < BLOCK / 64; i++) {
+        off += i < w ? wsum[i] : 0;
+        tot += wsum[i];
+    }
+    if (threadIdx.x == 0) bbase = tot ? atomicAdd(dirty_ctr, (ull)tot) : 0;
+    __syncthreads();
+    ull at = bbase + (ull)(off + pre);
+    while (m) {
+        const int b = __ffsll((unsigned long long)m) - 1;
+        m &= m - 1;
+        dirty[at++] = (uint32_t)(base + 4 * ((i64)(b >> 2) * BLOCK + threadIdx.x) + (b & 3));
+    }
+}
+
 // Filtration main_3d.cu:2405-2490: every vertex of an f-critical cell is pinned to f - eps.
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_pin(const uint64_t* recs, ull n, const T* f, T* g, Dims d, Counts c,
@@ -893,6 +1013,9 @@ __global__ __launch_bounds__(BLOCK) void k_as_branch(const uint64_t* fcf, ull n,
 // (as_keep_steps), between AS_K_MIN and AS_K_MAX.
 constexpr int AS_K_MIN = 8, AS_K_MAX = 64;
 constexpr unsigned AS_WS_BLOCKS = 2048;  // blocks of the work-stealing walk (eight per CU)
+// Beside the repair sweeps (on a second stream) the walk takes half the wave slots: its waves
+// run until every branch is walked, so with all slots the sweeps' kernels would wait for it.
+constexpr unsigned AS_WS_BLOCKS_SIDE = 1024;
 constexpr size_t AS_KEEP_BYTES = (size_t)4 << 30;
 struct StoreFirst {
     uint2* p;
@@ -933,10 +1056,12 @@ __global__ __launch_bounds__(BLOCK) void k_as_count(const uint64_t* fcf, ull nb,
 }
 
 // k_as_count with work stealing: path lengths are heavy-tailed, so instead of one branch per
-// thread, every lane takes a new branch from a shared counter (*next, one atomic per wave and
-// refill) whenever its walk ends, and the waves run until the branches are exhausted. Each loop
-// round every busy lane advances its walk by one step, so a wave's lanes stay busy until the
-// last branches. Same per-branch results as k_as_count.
+// thread, every lane takes a new branch whenever its walk ends, and the waves run until the
+// branches are exhausted. Each loop round every busy lane advances its walk by one step, so a
+// wave's lanes stay busy until the last branches. A wave claims AS_WS_CHUNK branches at a time
+// from the shared counter (*next) and hands them to its idle lanes, so the one counter sees an
+// atomic per chunk rather than one per round. Same per-branch results as k_as_count.
+constexpr ull AS_WS_CHUNK = 1024;
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_as_count_ws(const uint64_t* fcf, ull nb, const T* f, const StarCode* cf,
                                                        Dims d, Counts c, ull* cnt, uint2* first, int keep, ull* next,
@@ -946,28 +1071,46 @@ __global__ __launch_bounds__(BLOCK) void k_as_count_ws(const uint64_t* fcf, ull 
     bool busy = false;
     ull j = 0;
     i64 cur = -1, n = 0;
+    ull cb = 0, ce = 0;  // the wave's unclaimed branches [cb, ce) (wave-uniform)
+    bool dry = false;    // the shared counter has passed nb (wave-uniform)
     for (;;) {
         const ull idle = __ballot(!busy);
-        bool exhausted = false;
-        if (idle) {
-            const int leader = __ffsll((unsigned long long)idle) - 1;
-            ull base = 0;
-            if (lane == leader) base = atomicAdd(next, (ull)__popcll(idle));
-            const unsigned lo = __shfl((unsigned)(base & 0xffffffffu), leader, 64);
-            const unsigned hi = __shfl((unsigned)(base >> 32), leader, 64);
-            base = ((ull)hi << 32) | lo;
-            exhausted = base >= nb;
-            if (!busy) {
-                j = base + (ull)__popcll(idle & ((1ull << lane) - 1ull));
-                if (j < nb) {
-                    cur = branch_tet(fcf, j, f, cf, d, c);
-                    n = 0;
-                    if (cur < 0) cnt[j] = 0;  // no cofacet on this side: an empty path
-                    else busy = true;
+        if (idle && !(dry && cb >= ce)) {
+            const ull nid = (ull)__popcll(idle);
+            const ull rank = (ull)__popcll(idle & ((1ull << lane) - 1ull));
+            const ull avail = ce - cb;
+            ull jj = nb;  // this lane's new branch, if idle (nb: none)
+            if (nid <= avail) {
+                jj = cb + rank;
+                cb += nid;
+            } else {
+                ull nc = 0;
+                if (!dry) {
+                    const int leader = __ffsll((unsigned long long)idle) - 1;
+                    if (lane == leader) nc = atomicAdd(next, AS_WS_CHUNK);
+                    const unsigned lo = __shfl((unsigned)(nc & 0xffffffffu), leader, 64);
+                    const unsigned hi = __shfl((unsigned)(nc >> 32), leader, 64);
+                    nc = ((ull)hi << 32) | lo;
+                    dry = nc >= nb;
+                }
+                if (rank < avail) jj = cb + rank;
+                else if (!dry) jj = nc + (rank - avail);
+                if (!dry) {
+                    ce = nc + AS_WS_CHUNK < nb ? nc + AS_WS_CHUNK : nb;
+                    cb = nc + (nid - avail) < ce ? nc + (nid - avail) : ce;
+                } else {
+                    cb = ce;
                 }
             }
+            if (!busy && jj < nb) {
+                j = jj;
+                cur = branch_tet(fcf, j, f, cf, d, c);
+                n = 0;
+                if (cur < 0) cnt[j] = 0;  // no cofacet on this side: an empty path
+                else busy = true;
+            }
         }
-        if (__ballot(busy) == 0 && exhausted) break;
+        if (__ballot(busy) == 0 && dry && cb >= ce) break;
         if (busy) {
             bool fin = R.as_fpath_step(cur, n, StoreFirst{first + j * (ull)keep, keep});
             if (!fin && n > c.N) {  // as_fpath's cap
@@ -1302,6 +1445,11 @@ inline unsigned stride_blocks(ull estimate) {
     return (unsigned)(b < 1 ? 1 : (b > STRIDE_BLOCKS ? STRIDE_BLOCKS : b));
 }
 
+// The dirty stars are collected by one pass over all N stamps (k_dirty_mark + k_dirty_collect)
+// once the (changed vertex, star slot) pairs reach N / DIRTY_DENSE, and by one returning atomic
+// per pair (k_dirty) below that.
+constexpr ull DIRTY_DENSE = 256;
+
 }  // namespace
 
 // ================================================================================================
@@ -1332,16 +1480,15 @@ struct dmtz_ctx {
     uint16_t* d_fix_key = nullptr;   // sort keys (branch class per mismatch record, path length per
     uint16_t* d_fix_key2 = nullptr;  // ascending branch) and their sorted copy
     size_t fix_key_cap = 0, fix_key2_cap = 0;
-    // Repairs of record types 1..3: class keys, sorted records and sort scratch per type, and one
-    // stream per type for sweeps with few records, when the types run concurrently.
-    hipStream_t fix_stream[3] = {nullptr, nullptr, nullptr};
-    bool fix_concurrent = true;  // off in batch lanes: their own streams already overlap
-    hipEvent_t fix_fork = nullptr, fix_join[3] = {nullptr, nullptr, nullptr};
+    // Repairs of record types 1..3: class keys, grouped records and counting-sort scratch per type,
+    // and the second stream of the repair phase.
+    bool fix_concurrent = true;  // off in batch lanes: no second stream (repairs, path precompute)
+    hipStream_t fix_stream = nullptr;
+    hipEvent_t fix_fork = nullptr, fix_join = nullptr;
     uint16_t* d_fk[3] = {nullptr, nullptr, nullptr};
-    uint16_t* d_fk2[3] = {nullptr, nullptr, nullptr};
     uint64_t* d_fsorted[3] = {nullptr, nullptr, nullptr};
     uint8_t* d_ftmp[3] = {nullptr, nullptr, nullptr};
-    size_t fk_cap[3] = {0, 0, 0}, fk2_cap[3] = {0, 0, 0}, fsorted_cap[3] = {0, 0, 0}, ftmp_cap[3] = {0, 0, 0};
+    size_t fk_cap[3] = {0, 0, 0}, fsorted_cap[3] = {0, 0, 0}, ftmp_cap[3] = {0, 0, 0};
     uint2* d_as_first = nullptr;     // first steps per branch from the counting walk
     size_t as_first_cap = 0;
     uint32_t* d_as_off32 = nullptr;  // 32-bit branch offsets and per-step cubes for the dirty test
@@ -1674,35 +1821,40 @@ struct Driver {
         return 0;
     }
 
-    // Repair phase of one critical-point sweep (main_3d.cu:6801-6833) + apply (:6837).
-    // The repair kinds run concurrently, one stream each (the edits form a set: their order does
-    // not matter), so that one kind's longest chains of dependent reads overlap the others' work.
-    // With few records per kind each kind classifies and sorts on its own stream too; with many,
-    // the sorts run first on the context's stream, because the radix sort's look-back waits on
-    // blocks that would compete with the other kinds' repairs.
-    static constexpr ull FIX_CONCURRENT_MAX = 1ull << 15;
+    // Repair phase of one critical-point sweep (main_3d.cu:6801-6833) + apply (:6837). The
+    // records of each kind are grouped by branch class, then the kinds are repaired concurrently
+    // (the edits form a set: their order does not matter): minima and 2-saddles on the context's
+    // stream, saddles then maxima on a second one, so that one kind's longest chains of dependent
+    // reads overlap another's work. Two streams rather than one per kind: the process's streams
+    // share four hardware queues, and two streams on one queue run one after the other.
 
-    // Group the records of kind ty (1..3) by branch class on stream st; recs then points at the
-    // sorted copy.
-    int sort_class(int ty, hipStream_t st, const uint64_t*& recs, ull n) {
+    // Group the n records of kind ty (1..3) by branch class; recs then points at the grouped copy.
+    // Grouping only orders the records, so lists beyond 32-bit slots are left as they are.
+    int sort_class(int ty, const uint64_t*& recs, ull n) {
+        if (n < 2 || n >= (1ull << 32)) return 0;
         const int k = ty - 1;
+        hipStream_t st = ctx->stream;
         int rc = grow_buf(ctx, ctx->d_fk[k], ctx->fk_cap[k], n);
         if (rc) return rc;
         rc = grow_buf(ctx, ctx->d_fsorted[k], ctx->fsorted_cap[k], n);
-        if (rc) return rc;
-        rc = grow_buf(ctx, ctx->d_fk2[k], ctx->fk2_cap[k], n);
         if (rc) return rc;
         auto cls = ty == 1 ? k_fix_class<T, 1> : ty == 2 ? k_fix_class<T, 2> : k_fix_class<T, 3>;
         cls<<<nblocks(n), BLOCK, 0, st>>>(recs, n, f, ctx->d_code_f, g, ctx->d_code_g, ctx->d_critf, d, c,
                                           ctx->d_fk[k]);
         CK(hipGetLastError());
+        const ull nblk = (n + CS_PER_BLOCK - 1) / CS_PER_BLOCK, m = nblk * CS_BINS;
+        const size_t hist_bytes = (sizeof(uint32_t) * m + 255) / 256 * 256;
         size_t tmp = 0;
-        CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, ctx->d_fk[k], ctx->d_fk2[k], recs, ctx->d_fsorted[k],
-                                              (int)n, 0, FIX_CLASS_BITS, st));
-        rc = grow_buf(ctx, ctx->d_ftmp[k], ctx->ftmp_cap[k], tmp);
+        CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)m, st));
+        rc = grow_buf(ctx, ctx->d_ftmp[k], ctx->ftmp_cap[k], 2 * hist_bytes + tmp);
         if (rc) return rc;
-        CK(hipcub::DeviceRadixSort::SortPairs(ctx->d_ftmp[k], tmp, ctx->d_fk[k], ctx->d_fk2[k], recs,
-                                              ctx->d_fsorted[k], (int)n, 0, FIX_CLASS_BITS, st));
+        uint32_t* hist = reinterpret_cast<uint32_t*>(ctx->d_ftmp[k]);
+        uint32_t* first = reinterpret_cast<uint32_t*>(ctx->d_ftmp[k] + hist_bytes);
+        k_class_hist<<<(unsigned)nblk, BLOCK, 0, st>>>(ctx->d_fk[k], n, nblk, hist);
+        CK(hipGetLastError());
+        CK(hipcub::DeviceScan::ExclusiveSum(ctx->d_ftmp[k] + 2 * hist_bytes, tmp, hist, first, (int)m, st));
+        k_class_scatter<<<(unsigned)nblk, BLOCK, 0, st>>>(ctx->d_fk[k], recs, n, nblk, first, ctx->d_fsorted[k]);
+        CK(hipGetLastError());
         recs = ctx->d_fsorted[k];
         return 0;
     }
@@ -1710,56 +1862,54 @@ struct Driver {
     int fix_cp(const ull counts[4]) {
         int rc = begin_edits();
         if (rc) return rc;
+        ull nrec = 0;
+        rc = repair_records(counts, nrec);
+        if (rc) return rc;
+        return apply(nrec);
+    }
+
+    // The repair rules over the current mismatch lists (counts[ty] records of type ty) into the
+    // edit set of the current epoch; nrec receives the number of records repaired.
+    int repair_records(const ull counts[4], ull& nrec) {
         Lists4 L = rec_lists(ctx);
-        bool small = true;
-        for (int ty = 1; ty < 4; ty++) small = small && counts[ty] < FIX_CONCURRENT_MAX;
-        const bool side = ctx->fix_concurrent;
-        if (side && !ctx->fix_stream[2]) {
-            // created on first use: streams take the process's hardware queues in turn, so
-            // contexts that never run concurrent repairs (batch lanes) must not create them
-            CK(hipEventCreateWithFlags(&ctx->fix_fork, hipEventDisableTiming));
-            for (int k = 0; k < 3; k++) {
-                CK(hipEventCreateWithFlags(&ctx->fix_join[k], hipEventDisableTiming));
-                CK(hipStreamCreateWithFlags(&ctx->fix_stream[k], hipStreamNonBlocking));
-            }
-        }
         bool active[4];
         const uint64_t* recs[4];
-        ull nrec = 0;
+        nrec = 0;
+        int rc = 0;
         for (int ty = 0; ty < 4; ty++) {
             active[ty] = counts[ty] != 0 && !((ty == 1 || ty == 2) && fl.preserve_saddles != 1);
             recs[ty] = L.l[ty];
-            if (active[ty]) nrec += counts[ty];
-        }
-        if (!small)
-            for (int ty = 1; ty < 4; ty++)
-                if (active[ty]) {
-                    rc = sort_class(ty, ctx->stream, recs[ty], counts[ty]);
-                    if (rc) return rc;
-                }
-        if (side) CK(hipEventRecord(ctx->fix_fork, ctx->stream));
-        for (int ty = 0; ty < 4; ty++) {
             if (!active[ty]) continue;
-            const int k = ty - 1;
-            hipStream_t st = ctx->stream;
-            if (ty != 0 && side) {
-                st = ctx->fix_stream[k];
-                CK(hipStreamWaitEvent(st, ctx->fix_fork, 0));
-            }
-            if (ty != 0 && small) {
-                rc = sort_class(ty, st, recs[ty], counts[ty]);
+            nrec += counts[ty];
+            if (ty != 0) {
+                rc = sort_class(ty, recs[ty], counts[ty]);
                 if (rc) return rc;
             }
+        }
+        const bool side = ctx->fix_concurrent && (active[1] || active[3]) && (active[0] || active[2]);
+        if (side && !ctx->fix_stream) {
+            // created on first use: batch lanes never run a second stream
+            CK(hipStreamCreateWithFlags(&ctx->fix_stream, hipStreamNonBlocking));
+            CK(hipEventCreateWithFlags(&ctx->fix_fork, hipEventDisableTiming));
+            CK(hipEventCreateWithFlags(&ctx->fix_join, hipEventDisableTiming));
+        }
+        if (side) {
+            CK(hipEventRecord(ctx->fix_fork, ctx->stream));
+            CK(hipStreamWaitEvent(ctx->fix_stream, ctx->fix_fork, 0));
+        }
+        for (int ty : {2, 0, 1, 3}) {
+            if (!active[ty]) continue;
+            hipStream_t st = side && (ty == 1 || ty == 3) ? ctx->fix_stream : ctx->stream;
             auto kern = ty == 0 ? k_fix_cp<T, 0> : ty == 1 ? k_fix_cp<T, 1> : ty == 2 ? k_fix_cp<T, 2> : k_fix_cp<T, 3>;
             kern<<<nblocks(counts[ty]), BLOCK, 0, st>>>(recs[ty], counts[ty], f, ctx->d_code_f, g, ctx->d_code_g, d,
                                                         c, ctx->d_counter, eps, range, fl.edit_type, sink());
             CK(hipGetLastError());
-            if (st != ctx->stream) {  // the context's stream waits for this kind before applying the edits
-                CK(hipEventRecord(ctx->fix_join[k], st));
-                CK(hipStreamWaitEvent(ctx->stream, ctx->fix_join[k], 0));
-            }
         }
-        return apply(nrec);
+        if (side) {  // the context's stream waits for the second stream's kinds before the edits apply
+            CK(hipEventRecord(ctx->fix_join, ctx->fix_stream));
+            CK(hipStreamWaitEvent(ctx->stream, ctx->fix_join, 0));
+        }
+        return 0;
     }
 
     // applyDeltaBuffer1 (main_3d.cu:1122-1164) over the edit set, then re-evaluation of the g star
@@ -1801,9 +1951,18 @@ struct Driver {
             ctx->dirty_total += (double)c.N;
             return 0;
         }
-        k_dirty<<<stride_blocks(15 * est), BLOCK, 0, ctx->stream>>>(ctx->d_changed, ctx->d_ctr + K_CHG, d,
-                                                                   ctx->d_dstamp, ctx->epoch, ctx->d_dirty,
-                                                                   ctx->d_ctr + K_DIRTY);
+        if (DIRTY_DENSE * 15 * est >= (ull)c.N) {  // long list: plain stores, then one pass over the stamps
+            k_dirty_mark<<<stride_blocks(15 * est), BLOCK, 0, ctx->stream>>>(ctx->d_changed, ctx->d_ctr + K_CHG, d,
+                                                                            ctx->d_dstamp, ctx->epoch);
+            CK(hipGetLastError());
+            const ull per_block = (ull)BLOCK * 4 * DIRTY_QUADS;
+            k_dirty_collect<<<(unsigned)(((ull)c.N + per_block - 1) / per_block), BLOCK, 0, ctx->stream>>>(
+                ctx->d_dstamp, c.N, ctx->epoch, ctx->d_dirty, ctx->d_ctr + K_DIRTY);
+        } else {
+            k_dirty<<<stride_blocks(15 * est), BLOCK, 0, ctx->stream>>>(ctx->d_changed, ctx->d_ctr + K_CHG, d,
+                                                                       ctx->d_dstamp, ctx->epoch, ctx->d_dirty,
+                                                                       ctx->d_ctr + K_DIRTY);
+        }
         CK(hipGetLastError());
         k_code_list_g<T><<<stride_blocks(15 * est), BLOCK, 0, ctx->stream>>>(
             g, d, ctx->d_code_g, ctx->d_critf, ctx->d_mm, ctx->d_dirty, ctx->d_ctr + K_DIRTY, ctx->d_ctr + K_STARS,
@@ -2184,7 +2343,8 @@ struct Driver {
         if (rc) return rc;
         CK(hipMemsetAsync(ctx->d_as_cnt + nb, 0, sizeof(ull), st));
         CK(hipMemsetAsync(ctx->d_ctr + K_ASNEXT, 0, sizeof(ull), st));
-        const unsigned wsb = nblocks(nb) < AS_WS_BLOCKS ? nblocks(nb) : AS_WS_BLOCKS;
+        const unsigned cap_blocks = st == ctx->stream ? AS_WS_BLOCKS : AS_WS_BLOCKS_SIDE;
+        const unsigned wsb = nblocks(nb) < cap_blocks ? nblocks(nb) : cap_blocks;
         k_as_count_ws<T><<<wsb, BLOCK, 0, st>>>(FL.l[2], nb, f, ctx->d_code_f, d, c, ctx->d_as_cnt,
                                                           ctx->d_as_first, keep, ctx->d_ctr + K_ASNEXT,
                                                           ctx->d_ctr + K_WALK);
@@ -2676,7 +2836,7 @@ int correct_batch(dmtz_ctx* ctx, const Field* fields, int32_t n, const dmtz_flag
             ctx->err = "batch lane context creation failed";
             return rc;
         }
-        w->fix_concurrent = false;  // eight lanes with four streams each would share four hardware queues
+        w->fix_concurrent = false;  // one stream per lane: more would share the four hardware queues
         ctx->workers.push_back(w);
     }
     for (dmtz_ctx* w : ctx->workers) w->err.clear();
@@ -2793,16 +2953,9 @@ int targets_host(dmtz_ctx* ctx, const T* f, const T* g, i64 nx, i64 ny, i64 nz, 
     for (int k = 0; k < 4; k++) counts4[k] = (int64_t)cc[k];
     rc = drv.begin_edits();
     if (rc) return rc;
-    Lists4 L = rec_lists(ctx);
-    for (int ty = 0; ty < 4; ty++) {
-        if (cc[ty] == 0) continue;
-        if ((ty == 1 || ty == 2) && flags->preserve_saddles != 1) continue;
-        auto kern = ty == 0 ? k_fix_cp<T, 0> : ty == 1 ? k_fix_cp<T, 1> : ty == 2 ? k_fix_cp<T, 2> : k_fix_cp<T, 3>;
-        kern<<<nblocks(cc[ty]), BLOCK, 0, ctx->stream>>>(L.l[ty], cc[ty], (const T*)ctx->d_f, ctx->d_code_f,
-                                                         (const T*)ctx->d_g, ctx->d_code_g, d, c, ctx->d_counter, eps,
-                                                         range, flags->edit_type, drv.sink());
-        CK(hipGetLastError());
-    }
+    ull nrec = 0;
+    rc = drv.repair_records(cc, nrec);
+    if (rc) return rc;
     rc = sync_ctr(ctx);
     if (rc) return rc;
     ull nt = ctx->h_ctr[K_TGT];

@@ -631,22 +631,23 @@ struct Repair {
     }
 
     // One comparison step of as_fpath from tet cur (n steps so far): records (n, cur, t), moves
-    // cur to the next tet and returns true when the path ends there.
+    // cur to the next tet and returns true when the path ends there. as_walk's chain test
+    // (F.pf(t) names a tet whose F.pt is t again) holds whenever t = F.pt(cur) != -1: a tet and
+    // the triangle it pairs with are paired in one step of the gradient, each with the other
+    // (main_3d.cu:2310-2311), so F.pf(t) = cur. The step therefore reads one pairing instead of
+    // three.
     template <class Out>
     DMTZ_HD bool as_fpath_step(i64& cur, i64& n, Out out) const {
         const i64 t = F.pt(cur);
-        bool chain = false;
         i64 nxt = -1;
         if (t != -1) {
             out(n, cur, t);
             n++;
-            const i64 pft = F.pf(t);
-            chain = pft != -1 && F.pt(pft) == t;
             i64 t1 = -1, t2 = -1;
-            if (chain) cofacets(t, t1, t2);
+            cofacets(t, t1, t2);
             nxt = (t1 == cur) ? t2 : t1;
         }
-        const bool done = t == -1 || !chain || nxt == cur || nxt == -1;
+        const bool done = t == -1 || nxt == cur || nxt == -1;
         cur = nxt;
         return done;
     }

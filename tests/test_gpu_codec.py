@@ -79,14 +79,15 @@ def test_stream_round_trip_of_corrections(dev, dtype, fl):
 
 @pytest.mark.parametrize("geometry", [0, 1], ids=["critical_cells", "geometry"])
 def test_streams_match_the_oracle_restatement(dev, oracle, geometry):
-    """float64 corrections without filtration or finisher, so every changed vertex's counter
-    replays to its value: the edit_type 1 stream is the reference's cost stream byte for byte
-    (bitmap and index-6 values, oracle.cost_streams), the edit_type 0 stream is original_cost's
-    (oracle.original_cost_streams), and the run-length code of the bitmap is
+    """float64 corrections with the counter-driven edit (edit_type 1, the one the reference
+    reports through cost, main_3d.cu:7175-7178) and without filtration or finisher, so every
+    changed vertex's counter replays to its value: the edit_type 1 stream is the reference's cost
+    stream byte for byte (bitmap and index-6 values, oracle.cost_streams), the edit_type 0 stream
+    is original_cost's (oracle.original_cost_streams), and the run-length code of the bitmap is
     runLengthEncodeBitmap's (oracle.rle_bitmap)."""
-    fl = dmtz_amd.CorrectionFlags(preserve_vpath=geometry, preserve_geometry=geometry)
+    fl = dmtz_amd.CorrectionFlags(preserve_vpath=geometry, preserve_geometry=geometry, edit_type=1)
     for seed, shape, rel in [(0, (24, 20, 16), 1e-2), (2, (28, 28, 28), 1e-3), (3, (40, 17, 23), 3e-3)]:
-        f = fields.gaussian_mixture(shape, 6, seed=seed, noise=0.02)
+        f = fields.gaussian_mixture(shape, 6, seed=seed, noise=0.02).astype(np.float64)
         eps, rng = fields.eps_from_rel(f, rel)
         g0 = fields.quantize(f, eps)
         r = dmtz_amd.correct(f, g0, eps, rng, fl, ctx=dev, return_counter=True)

@@ -137,6 +137,10 @@ def _check_matching(pv, pe, pf, pt, dims, oracle):
             continue
         assert pt[p] == t
         assert set(oracle.tri_verts(dims, t)) <= set(oracle.tet_verts(dims, p))
+    # and from the tets: a tet's triangle names the tet back (the ascending path precompute
+    # reads only the tet's side of each pair)
+    for q in np.nonzero(pt >= 0)[0]:
+        assert pf[pt[q]] == q
 
 
 @pytest.mark.parametrize("k", range(6))
