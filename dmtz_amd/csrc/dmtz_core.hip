@@ -18,7 +18,7 @@ int dmtz_ctx_create(dmtz_ctx** out, int device) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&ctx->d_ctr, sizeof(ull) * K_NCTR);
     if (e == hipSuccess) e = hipHostMalloc(&ctx->h_ctr, sizeof(ull) * K_NCTR);
-    if (e == hipSuccess) e = hipHostMalloc(&ctx->h_as_total, sizeof(ull));
+    if (e == hipSuccess) e = hipHostMalloc(&ctx->h_as_total, 2 * sizeof(ull));
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
     if (e == hipSuccess && dmtz_internal::noalloc_setup(device) != 0) e = hipErrorInvalidSymbol;
@@ -43,8 +43,6 @@ void dmtz_ctx_destroy(dmtz_ctx* ctx) {
     free_fc(ctx);
     free_paths(ctx);
     (void)hipFree(ctx->d_ctr);
-    (void)hipFree(ctx->d_fix_key);
-    (void)hipFree(ctx->d_fix_key2);
     for (int k = 0; k < 3; k++) {
         (void)hipFree(ctx->d_fk[k]);
         (void)hipFree(ctx->d_fsorted[k]);

@@ -99,6 +99,44 @@ __device__ __forceinline__ ull wave_alloc(ull* ctr, int n) {
     return (((ull)hi << 32) | lo) + (ull)pre;
 }
 
+// Every thread of the block must call this (block-uniform control flow). Returns the thread's
+// first slot in the list, claimed with one atomic per block: appends of every wave to one
+// counter are serialised at the memory side, so a wave-level claim over a list of millions of
+// entries costs more than the work that fills it.
+__device__ __forceinline__ ull block_alloc(ull* ctr, int n) {
+    __shared__ int wsum[BLOCK / 64];
+    __shared__ ull bbase;
+    int wt;
+    const int pre = wave_excl_scan(n, wt);
+    const int w = (int)(threadIdx.x >> 6);
+    if (__lane_id() == 0) wsum[w] = wt;
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int i = 0; i < BLOCK / 64; i++) {
+        off += i < w ? wsum[i] : 0;
+        tot += wsum[i];
+    }
+    if (threadIdx.x == 0) bbase = tot ? atomicAdd(ctr, (ull)tot) : 0;
+    __syncthreads();
+    const ull r = bbase + (ull)(off + pre);
+    __syncthreads();  // wsum and bbase are reused by the block's next call
+    return r;
+}
+
+// Every thread of the block must call this. Adds the number of threads with x set to *ctr.
+__device__ __forceinline__ void block_add(ull* ctr, bool x) {
+    __shared__ int wcnt[BLOCK / 64];
+    const ull b = __ballot(x ? 1 : 0);
+    if (__lane_id() == 0) wcnt[threadIdx.x >> 6] = __popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int i = 0; i < BLOCK / 64; i++) tot += wcnt[i];
+        if (tot) atomicAdd(ctr, (ull)tot);
+    }
+    __syncthreads();
+}
+
 // Every lane of the wave must call this. Adds the number of lanes with x set to *ctr.
 __device__ __forceinline__ void wave_add(ull* ctr, bool x) {
     const ull b = __ballot(x ? 1 : 0);
@@ -630,9 +668,9 @@ __global__ __launch_bounds__(BLOCK) void k_apply(const uint32_t* list, const ull
                 mv = ch || k != k0;
             }
         }
-        ull at = wave_alloc(chg_ctr, ch ? 1 : 0);
+        ull at = block_alloc(chg_ctr, ch ? 1 : 0);
         if (ch) changed[at] = (uint32_t)v;
-        wave_add(moved, mv);
+        block_add(moved, mv);
     }
 }
 
@@ -790,7 +828,7 @@ __global__ __launch_bounds__(BLOCK) void k_dirty(const uint32_t* changed, const 
             }
             if (ok) fresh = atomicExch(stamp + w, epoch) != epoch;
         }
-        ull at = wave_alloc(dirty_ctr, fresh ? 1 : 0);
+        ull at = block_alloc(dirty_ctr, fresh ? 1 : 0);
         if (fresh) dirty[at] = (uint32_t)w;
     }
 }
@@ -891,9 +929,9 @@ struct PathList {
     int64_t* rec;  // recorded vertex / divergent tet
     ull* ctr;
     ull cap;
-    // Every lane of the wave must call this; lanes with has = false add nothing.
-    __device__ void push_wave(bool has, i64 a, i64 b) const {
-        ull at = wave_alloc(ctr, has ? 1 : 0);
+    // Every thread of the block must call this; threads with has = false add nothing.
+    __device__ void push_block(bool has, i64 a, i64 b) const {
+        ull at = block_alloc(ctr, has ? 1 : 0);
         if (has && at < cap) {
             id[at] = a;
             rec[at] = b;
@@ -922,7 +960,7 @@ __global__ __launch_bounds__(BLOCK) void k_ds_geo(const uint64_t* fce, ull n, co
         Repair<T, NoSink> R(f, cf, g, cg, d, c, nullptr, 0, 0, 0, NoSink{});
         has = R.ds_geometry(e, rec);
     }
-    out.push_wave(has, e, rec);
+    out.push_block(has, e, rec);
 }
 
 // Descending geometry checks over per-edge data computed once per run: endpoints and their f
@@ -966,7 +1004,7 @@ __global__ __launch_bounds__(BLOCK) void k_ds_check(ull n, const uint4* dsv, con
             rec = res[i];
         }
     }
-    out.push_wave(rec >= 0, e, rec);
+    out.push_block(rec >= 0, e, rec);
 }
 
 // Ascending branch walks (is_false main_3d.cu:4437-4497), thread = (f-critical face, cofacet k).
@@ -1198,14 +1236,39 @@ __global__ __launch_bounds__(BLOCK) void k_as_gather_cnt(ull nb, const uint32_t*
     else if (t == nb) out[nb] = 0;
 }
 
-// Branch order of the checks: branches sorted by step count (capped at 255), so that a wave's
-// walks have similar lengths. Keys and identity order for the sort.
-__global__ __launch_bounds__(BLOCK) void k_as_len_keys(ull nb, const ull* cnt, uint16_t* key, uint32_t* idx) {
-    const ull j = (ull)blockIdx.x * BLOCK + threadIdx.x;
-    if (j >= nb) return;
-    key[j] = (uint16_t)(cnt[j] < 255 ? cnt[j] : 255);
-    idx[j] = (uint32_t)j;
+// Branch order of the checks: branches grouped by step count (capped at 255), so that a wave's
+// walks have similar lengths and the branches without a step come first (the checks skip them).
+// A counting sort like k_class_hist / k_class_scatter, with the key read from the counts.
+constexpr int LEN_BINS = 256;
+__device__ __forceinline__ int len_bin(const ull* cnt, ull j) { return (int)(cnt[j] < 255 ? cnt[j] : 255); }
+
+__global__ __launch_bounds__(BLOCK) void k_len_hist(ull nb, const ull* cnt, ull nblk, uint32_t* hist) {
+    __shared__ uint32_t h[LEN_BINS];
+    h[threadIdx.x] = 0;  // BLOCK == LEN_BINS
+    __syncthreads();
+    const ull base = (ull)blockIdx.x * CS_PER_BLOCK;
+    for (int k = 0; k < CS_V; k++) {
+        const ull j = base + (ull)k * BLOCK + threadIdx.x;
+        (void)wave_bin_claim(j < nb ? len_bin(cnt, j) : -1, h);
+    }
+    __syncthreads();
+    hist[(ull)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
 }
+
+__global__ __launch_bounds__(BLOCK) void k_len_scatter(ull nb, const ull* cnt, ull nblk, const uint32_t* first,
+                                                       uint32_t* order) {
+    __shared__ uint32_t h[LEN_BINS];
+    h[threadIdx.x] = first[(ull)threadIdx.x * nblk + blockIdx.x];
+    __syncthreads();
+    const ull base = (ull)blockIdx.x * CS_PER_BLOCK;
+    for (int k = 0; k < CS_V; k++) {
+        const ull j = base + (ull)k * BLOCK + threadIdx.x;
+        const int bin = j < nb ? len_bin(cnt, j) : -1;
+        const uint32_t slot = wave_bin_claim(bin, h);
+        if (bin >= 0) order[slot] = (uint32_t)j;
+    }
+}
+static_assert(BLOCK == LEN_BINS, "k_len_hist keeps one bin per thread");
 
 // Per-step dirty bits of the compact path copies: bit s of sd is set when the tet of step s
 // lies in a marked cube. Each wave covers 4 x 64 consecutive steps, lane l the steps
@@ -1248,16 +1311,18 @@ __device__ __forceinline__ uint32_t next_dirty(const uint64_t* sd, uint32_t s, u
 
 // k_as_check over the compact copies. Thread t checks branch order[t], whose steps are at
 // [off32[t], off32[t + 1]) (the copies are laid out in check order, so a wave's offsets and dirty
-// words are adjacent); results stay indexed by branch. Unless full, a branch with no dirty step keeps its last result, and a branch whose
+// words are adjacent); results stay indexed by branch. Threads start at check position t0: the
+// positions before it hold the branches without a step, whose result stays empty. Unless full, a
+// branch with no dirty step keeps its last result, and a branch whose
 // last result was "no mismatch" compares only its dirty steps, in order: the others matched then
 // and read unchanged star codes now. A branch that had a mismatch and has a dirty step is walked
 // again from its start.
 template <class T>
-__global__ __launch_bounds__(BLOCK) void k_as_check32(ull nb, const uint32_t* order, const uint32_t* off32,
+__global__ __launch_bounds__(BLOCK) void k_as_check32(ull t0, ull nb, const uint32_t* order, const uint32_t* off32,
                                                       const uint64_t* sd, const uint2* steps, const T* g,
                                                       const StarCode* cg, Dims d, Counts c, int full, int64_t* res,
                                                       uint32_t* fails) {
-    const ull t = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    const ull t = t0 + (ull)blockIdx.x * BLOCK + threadIdx.x;
     if (t >= nb) return;
     const uint32_t a = off32[t], b = off32[t + 1];
     uint32_t s = a;
@@ -1300,7 +1365,7 @@ __global__ __launch_bounds__(BLOCK) void k_as_combine(const int64_t* res, ull n,
         a = p[0] >= 0 ? p[0] : p[2];
         b = p[0] >= 0 ? p[1] : p[3];
     }
-    out.push_wave(has, a, b);
+    out.push_block(has, a, b);
 }
 
 // Endpoint labels of f (get_orginal_saddle_min_pairs / _max_pairs, main_3d.cu:4613-4759), one
@@ -1367,7 +1432,7 @@ __global__ __launch_bounds__(BLOCK) void k_ds_combine(const uint64_t* fce, ull n
         has = bad1 || bad2;
         rec = bad1 ? p[1] : (p[1] != -1 ? p[1] : p[3]);
     }
-    out.push_wave(has, e, rec);
+    out.push_block(has, e, rec);
 }
 
 // 2-saddle walls, get_false_ds_wall main_3d.cu:5064-5133, made deterministic: from each
@@ -1477,9 +1542,6 @@ struct dmtz_ctx {
     uint64_t* d_rec = nullptr;    // 4 mismatch record lists of rec_cap entries
     uint64_t* d_fc = nullptr;     // 4 f-critical lists of fc_cap entries
     int64_t* d_path = nullptr;    // ds id/rec, as id/rec, wall id/rec (fc_cap each)
-    uint16_t* d_fix_key = nullptr;   // sort keys (branch class per mismatch record, path length per
-    uint16_t* d_fix_key2 = nullptr;  // ascending branch) and their sorted copy
-    size_t fix_key_cap = 0, fix_key2_cap = 0;
     // Repairs of record types 1..3: class keys, grouped records and counting-sort scratch per type,
     // and the second stream of the repair phase.
     bool fix_concurrent = true;  // off in batch lanes: no second stream (repairs, path precompute)
@@ -1499,7 +1561,8 @@ struct dmtz_ctx {
     size_t as_off32_cap = 0, as_cube_cap = 0, as_order_cap = 0, as_sd_cap = 0, as_fails_cap = 0;
     hipStream_t as_stream = nullptr;  // the ascending-path precompute, overlapping the first sweeps
     hipEvent_t as_ev_fork = nullptr, as_ev_count = nullptr, as_ev_done = nullptr;
-    ull* h_as_total = nullptr;        // pinned: total steps of the precomputed paths
+    ull* h_as_total = nullptr;        // pinned: total steps of the precomputed paths, and (low
+                                      // half of [1]) the branches without a step
     uint32_t* d_wall_scr = nullptr;  // wall walks: per-thread visited sets and stacks
     size_t wall_scr_cap = 0;
     uint32_t* d_wall_ovf = nullptr;  // wall walks to repeat with more slots (two lists)
@@ -2262,15 +2325,27 @@ struct Driver {
         if (rc) return rc;
         rc = as_bitmaps_and_ds(FL, (size_t)(c.N >> 5) + 1, ctx->stream);
         if (rc) return rc;
+        CK(hipStreamSynchronize(ctx->stream));
+        return as_begin();
+    }
+
+    // The precomputed paths are in place: the checks start with a full one, and the branches
+    // without a step, which the compact checks skip, keep an empty result.
+    ull as_n0 = 0;  // branches without a step (the first as_n0 of the check order)
+    int as_begin() {
         as_ready = true;
         as_full = true;
+        as_n0 = as_compact ? (ctx->h_as_total[1] & 0xffffffffull) : 0;
+        if (as_compact && nfcf) CK(hipMemsetAsync(ctx->d_walk, 0xFF, sizeof(int64_t) * 4 * nfcf, ctx->stream));
         return 0;
     }
 
     // The precompute depends on f alone, so single-field runs start it on a stream of its own
     // right after the first detection and let it overlap the critical-point sweeps; the first
-    // separatrix check waits for it (as_join). as_stage: 0 none in flight, 1 counting walk
-    // enqueued, 2 fill enqueued.
+    // separatrix check waits for it (as_join). Started after the first repair phase instead, it
+    // slows the dense re-classification that follows (that kernel needs the wave slots the walk
+    // holds) more than it spares the rules. as_stage: 0 none in flight, 1 counting walk enqueued,
+    // 2 fill enqueued.
     int as_stage = 0;
 
     int as_start_async() {
@@ -2315,10 +2390,10 @@ struct Driver {
         int rc = as_advance(true);
         if (rc) return rc;
         if (as_stage == 2) {
+            CK(hipEventSynchronize(ctx->as_ev_done));  // usually long done: the sweeps take longer
             CK(hipStreamWaitEvent(ctx->stream, ctx->as_ev_done, 0));
             as_stage = 0;
-            as_ready = true;
-            as_full = true;
+            return as_begin();
         }
         return 0;
     }
@@ -2369,30 +2444,32 @@ struct Driver {
         if (rc) return rc;
         as_compact = total < ((ull)1 << 32) && nb < ((ull)1 << 31);
         if (as_compact) {  // branch order by path length (for the fill and the checks)
-            rc = grow_buf(ctx, ctx->d_as_order, ctx->as_order_cap, 2 * nb);
+            rc = grow_buf(ctx, ctx->d_as_order, ctx->as_order_cap, nb);
             if (rc) return rc;
-            rc = grow_buf(ctx, ctx->d_fix_key, ctx->fix_key_cap, nb);
-            if (rc) return rc;
-            rc = grow_buf(ctx, ctx->d_fix_key2, ctx->fix_key2_cap, nb);
-            if (rc) return rc;
-            k_as_len_keys<<<nblocks(nb), BLOCK, 0, st>>>(nb, ctx->d_as_cnt, ctx->d_fix_key,
-                                                                   ctx->d_as_order + nb);
-            CK(hipGetLastError());
+            const ull nblk = (nb + CS_PER_BLOCK - 1) / CS_PER_BLOCK, m = nblk * LEN_BINS;
+            const size_t hist_bytes = (sizeof(uint32_t) * m + 255) / 256 * 256;
             size_t tmp2 = 0;
-            CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp2, ctx->d_fix_key, ctx->d_fix_key2, ctx->d_as_order + nb,
-                                                  ctx->d_as_order, (int)nb, 0, 8, st));
+            CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp2, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)m, st));
             uint8_t* tp2 = (uint8_t*)ctx->d_scan_tmp;
-            rc = grow_buf(ctx, tp2, ctx->scan_tmp_cap, tmp2);
+            rc = grow_buf(ctx, tp2, ctx->scan_tmp_cap, 2 * hist_bytes + (tmp > tmp2 ? tmp : tmp2));
             ctx->d_scan_tmp = tp2;
             if (rc) return rc;
-            CK(hipcub::DeviceRadixSort::SortPairs(ctx->d_scan_tmp, tmp2, ctx->d_fix_key, ctx->d_fix_key2,
-                                                  ctx->d_as_order + nb, ctx->d_as_order, (int)nb, 0, 8, st));
+            uint32_t* hist = reinterpret_cast<uint32_t*>(tp2);
+            uint32_t* first = reinterpret_cast<uint32_t*>(tp2 + hist_bytes);
+            k_len_hist<<<(unsigned)nblk, BLOCK, 0, st>>>(nb, ctx->d_as_cnt, nblk, hist);
+            CK(hipGetLastError());
+            CK(hipcub::DeviceScan::ExclusiveSum(tp2 + 2 * hist_bytes, tmp2, hist, first, (int)m, st));
+            k_len_scatter<<<(unsigned)nblk, BLOCK, 0, st>>>(nb, ctx->d_as_cnt, nblk, first, ctx->d_as_order);
+            CK(hipGetLastError());
+            // branches without a step: the first slot of length 1 (low half of h_as_total[1])
+            ctx->h_as_total[1] = 0;
+            CK(hipMemcpyAsync(ctx->h_as_total + 1, first + nblk, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
             // steps in check order: offsets from the counts taken in that order
             k_as_gather_cnt<<<nblocks(nb + 1), BLOCK, 0, st>>>(nb, ctx->d_as_order, ctx->d_as_cnt,
                                                                        ctx->d_as_off);
             CK(hipGetLastError());
-            CK(hipcub::DeviceScan::ExclusiveSum(ctx->d_scan_tmp, tmp, ctx->d_as_off, ctx->d_as_cnt, (int)(nb + 1),
-                                                st));
+            CK(hipcub::DeviceScan::ExclusiveSum(tp2 + 2 * hist_bytes, tmp, ctx->d_as_off, ctx->d_as_cnt,
+                                                (int)(nb + 1), st));
             std::swap(ctx->d_as_cnt, ctx->d_as_off);
             std::swap(ctx->as_cnt_cap, ctx->as_off_cap);
         }
@@ -2452,10 +2529,10 @@ struct Driver {
                                   ctx->stream>>>(as_total, ctx->d_as_cube, ctx->d_cubes, ctx->d_as_sd);
                 CK(hipGetLastError());
             }
-            k_as_check32<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(nb, ctx->d_as_order, ctx->d_as_off32,
-                                                                     ctx->d_as_sd, ctx->d_as_steps, g,
-                                                                     ctx->d_code_g, d, c, as_full ? 1 : 0,
-                                                                     ctx->d_walk, ctx->d_as_fails);
+            if (nb > as_n0)
+                k_as_check32<T><<<nblocks(nb - as_n0), BLOCK, 0, ctx->stream>>>(
+                    as_n0, nb, ctx->d_as_order, ctx->d_as_off32, ctx->d_as_sd, ctx->d_as_steps, g, ctx->d_code_g, d, c,
+                    as_full ? 1 : 0, ctx->d_walk, ctx->d_as_fails);
         } else
             k_as_check<T><<<nblocks(nb), BLOCK, 0, ctx->stream>>>(nb, ctx->d_as_off, ctx->d_as_steps, g,
                                                                    ctx->d_code_g, d, c, ctx->d_cubes, as_full ? 1 : 0,
