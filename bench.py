@@ -325,25 +325,33 @@ def bench_learned(args, ctx, f, f_d, main_run, g0_d, g_d, flags, plain_value):
         g_d.copy_from(g0_d)
         return L.correct_learned(net, f_d, g_d, eps, rng, flags)
 
-    for _ in range(max(1, args.warmup)):
-        step()
-    devmem.synchronize()
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        res, edits = step()
-        log("learned step %d: %d sweeps, %d warm edits, counts %s" % (s, res.sweeps, edits, res.counts))
-    devmem.synchronize()
-    elapsed = time.perf_counter() - t0
-    assert res.converged, res.counts
-    gout = g_d.numpy()
-    assert float(np.abs(gout.astype(np.float64) - f.astype(np.float64)).max()) <= eps
     N = f.size
-    value = N * args.steps / elapsed
+
+    def timed(tag, steps):
+        for _ in range(max(1, args.warmup)):
+            step()
+        devmem.synchronize()
+        t0 = time.perf_counter()
+        for s in range(steps):
+            res, edits = step()
+            log("%s step %d: %d sweeps, %d warm edits, counts %s" % (tag, s, res.sweeps, edits, res.counts))
+        devmem.synchronize()
+        elapsed = time.perf_counter() - t0
+        assert res.converged, res.counts
+        gout = g_d.numpy()
+        assert float(np.abs(gout.astype(np.float64) - f.astype(np.float64)).max()) <= eps
+        value = N * steps / elapsed
+        return dict(value=round(value, 1), unit="voxels/s", ms_per_step=round(elapsed / steps * 1e3, 3),
+                    sweeps=res.sweeps, warm_edits=edits, edit_ratio=round(int(np.count_nonzero(gout != g0)) / N, 6),
+                    speedup_vs_plain=round(value / plain_value, 3))
+
+    out = timed("learned", args.steps)
+    out.update(plain_edit_ratio=round(main_run["edited"] / N, 6), params=os.path.basename(args.learned))
+    # the trivial warm start the network is measured against: every vertex lowered by eps
+    net.set_params(L.uniform_params(1.0))
+    out["uniform_warm_start"] = timed("uniform warm start", max(1, min(args.steps, 3)))
     net.close()
-    return dict(value=round(value, 1), unit="voxels/s", ms_per_step=round(elapsed / args.steps * 1e3, 3),
-                sweeps=res.sweeps, warm_edits=edits, edit_ratio=round(int(np.count_nonzero(gout != g0)) / N, 6),
-                plain_edit_ratio=round(main_run["edited"] / N, 6),
-                speedup_vs_plain=round(value / plain_value, 3), params=os.path.basename(args.learned))
+    return out
 
 
 def batch_fields(k, seed):

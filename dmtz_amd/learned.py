@@ -15,6 +15,18 @@ from .api import CorrectionFlags, CorrectionResult, _device_arg, correct_device,
 
 PARAMS = 4 * 6928
 CHANNELS = 16
+LAYER = 27 * CHANNELS * CHANNELS + CHANNELS  # W[27][16][16] (tap, input, output), then b[16]
+EDIT_LOGIT_BIAS = 3 * LAYER + 27 * CHANNELS * CHANNELS  # last layer, output 0 (edit logit); output 1: size
+
+
+def uniform_params(size=1.0):
+    """Parameters of the trivial warm start that the network is measured against: all weights
+    zero, so every vertex gets a positive edit logit and the edit size `size` (in units of eps):
+    g is lowered by size * eps everywhere, never past the snap of f - eps."""
+    p = np.zeros(PARAMS, np.float32)
+    p[EDIT_LOGIT_BIAS] = 10.0
+    p[EDIT_LOGIT_BIAS + 1] = size
+    return p
 
 
 def _ptr(t):

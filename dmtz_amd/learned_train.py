@@ -1,6 +1,6 @@
 """Train the warm-start network on pairs from the plain correction and evaluate its gain.
 
-    python -m dmtz_amd.learned_train --fields 200 --size 128 --epochs 12 \
+    python -m dmtz_amd.learned_train --fields 200 --size 128 --epochs 12 --pos-weight 0.35 \
         --out dmtz_amd/net_weights.npy --eval-size 384 --eval 4
 
 Training fields alternate fBm (P(k) ~ k^-3) and Gaussian mixtures (16 bumps, 0.01 noise) with eps
@@ -15,9 +15,7 @@ import time
 
 import numpy as np
 
-# index of the edit logit's bias in the parameters: 4 layers of (W[27][16][16], b[16]); the last
-# layer's output channel 0 is the edit logit
-LOGIT_BIAS = 3 * (27 * 16 * 16 + 16) + 27 * 16 * 16
+from .learned import EDIT_LOGIT_BIAS as LOGIT_BIAS
 
 
 def log(*a):
@@ -41,8 +39,9 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=12)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--lam", type=float, default=0.5)
-    ap.add_argument("--pos-weight", type=float, default=0.0,
-                    help="weight of the edited class in the loss; below 1 a false edit costs more than a missed one "
+    ap.add_argument("--pos-weight", type=float, default=0.35,
+                    help="weight of the edited class in the loss; below 1 a false edit costs more than a missed one, "
+                         "which keeps the warm edits near the vertices the plain loop edits "
                          "(0: sqrt(1 / edited fraction), capped to [1, 20])")
     ap.add_argument("--logit-shift", type=float, default=0.0,
                     help="subtract this from the edit logit's bias of the saved parameters: the warm start then "

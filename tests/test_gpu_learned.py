@@ -146,3 +146,55 @@ def test_training_lowers_loss_and_warm_start_converges(dev, net, torch_dev):
         assert warm.counts == (0,) * 8
         err = (g2.double() - fd.double()).abs().max().item()
         assert err <= eps
+
+
+def test_committed_network_gains_without_edit_blow_up(dev):
+    """The committed network on a 256^3 Gaussian mixture outside its training set (64 bumps, eps = 1e-2 range,
+    separatrix geometry, finisher): the warm-started correction needs fewer sweeps and less wall
+    time than the plain one, and edits at most 10 % more of the vertices (edit_ratio, the
+    reference's edited / N, main_3d.cu:1553). The trivial warm start (every vertex lowered by eps,
+    learned.uniform_params) is run beside it: it edits every vertex, which the network does not."""
+    import os
+
+    from dmtz_amd import devmem
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    f = fields.gaussian_mixture((256, 256, 256), 64, seed=0, noise=0.01).astype(np.float32)
+    eps, rng = fields.eps_from_rel(f, 1e-2)
+    g0 = fields.quantize(f, eps)
+    flags = dmtz_amd.CorrectionFlags(preserve_vpath=1, preserve_geometry=1, finish=1)
+    fd, g0d = devmem.DeviceArray.from_numpy(f), devmem.DeviceArray.from_numpy(g0)
+    gd = devmem.DeviceArray(f.shape, np.float32)
+    net = L.Net(seed=0, ctx=dev)
+
+    def run(params):
+        """best of two timed runs after a warm-up: (result, seconds, edit_ratio)"""
+        best = None
+        for k in range(3):
+            gd.copy_from(g0d)
+            devmem.synchronize()
+            t0 = time.perf_counter()
+            if params is None:
+                r = dmtz_amd.correct_device(fd, gd, eps, rng, flags, ctx=dev)
+            else:
+                r, _ = L.correct_learned(net, fd, gd, eps, rng, flags)
+            devmem.synchronize()
+            t = time.perf_counter() - t0
+            if k and (best is None or t < best):
+                best = t
+        g1 = gd.numpy()
+        assert r.status == 0 and r.counts == (0,) * 8
+        assert np.abs(g1.astype(np.float64) - f).max() <= eps
+        return r, best, float(np.count_nonzero(g1 != g0)) / f.size
+
+    plain, tp, ep = run(None)
+    net.set_params(np.load(os.path.join(root, "dmtz_amd", "net_weights.npy")))
+    learned, tl, el = run("net")
+    net.set_params(L.uniform_params(1.0))
+    uniform, tu, eu = run("uniform")
+    net.close()
+    print("plain %d sweeps %.1f ms edit %.4f | network %d sweeps %.1f ms edit %.4f | uniform %d sweeps %.1f ms edit %.4f"
+          % (plain.sweeps, tp * 1e3, ep, learned.sweeps, tl * 1e3, el, uniform.sweeps, tu * 1e3, eu), flush=True)
+    assert learned.sweeps < plain.sweeps and tl < tp
+    assert el <= ep + 0.10
+    assert eu > el
