@@ -398,10 +398,12 @@ def bench_batch(args):
                                      finish=args.finish)
     ctx = dmtz_amd.Context(0)
     batch = [(fd[i], gd[i], items[i][2], items[i][3]) for i in range(len(items))]
-    net = None
+    net = uniform = None
     if args.learned and args.learned != "none" and os.path.exists(args.learned):
         net = L.Net(seed=0, ctx=ctx)
         net.set_params(np.load(args.learned))
+        uniform = L.Net(seed=0, ctx=ctx)  # the trivial warm start, for comparison
+        uniform.set_params(L.uniform_params(1.0))
 
     def reset():
         for a, b in zip(gd, g0):
@@ -416,8 +418,13 @@ def bench_batch(args):
     def batched_learned():
         return L.correct_batch_learned(net, batch, flags)[0]
 
+    def batched_uniform():
+        return L.correct_batch_learned(uniform, batch, flags)[0]
+
     voxels = sum(int(f.size) for f, _, _, _, _ in items)
-    forms = [("plain", plain), ("batched", batched)] + ([("batched_learned", batched_learned)] if net else [])
+    forms = [("plain", plain), ("batched", batched)]
+    if net:
+        forms += [("batched_learned", batched_learned), ("batched_uniform_warm_start", batched_uniform)]
     out_forms = {}
     for name, fn in forms:
         for w in range(args.warmup):
@@ -443,6 +450,7 @@ def bench_batch(args):
         log("%s: %.1f ms per batch, %d sweeps" % (name, t / args.steps * 1e3, out_forms[name]["sweeps"]))
     if net:
         net.close()
+        uniform.close()
     best = "batched_learned" if net else "batched"
     plain_v = out_forms["plain"]["value"]
     workload = ("batch of %d float32 fields, sides 32..192, Gaussian mixture / fBm, eps log-uniform "
