@@ -109,15 +109,18 @@ def cpu_baseline(one_thread=True):
         r = O.correct(f, g, eps, rng, fl)
         dt = time.perf_counter() - c0
         assert r["status"] == 0 and tuple(r["counts"]) == (0,) * 8
-        runs.append((threads, 64 ** 3 / dt, r["sweeps"], dt))
-    threads, v, sw, dt = runs[0]
+        runs.append((threads, 64 ** 3 / dt, r["sweeps"], dt, r["phases"]))
+    threads, v, sw, dt, ph = runs[0]
+    # phases under the reference's timer names (main_3d.cu:7187-7188), in seconds
     res = dict(value=round(v, 1), unit="voxels/s", cores=threads, kind="port",
                sample="64^3 Gaussian mixture, 8 bumps, eps=1e-3*range, geometry, seed 0 (the CPU-runnable "
                       "workload), full correction: %d sweeps in %.2f s" % (sw, dt),
-               cpu_model=cpu_model(), nproc=os.cpu_count())
+               cpu_model=cpu_model(), nproc=os.cpu_count(),
+               phases_s={k: round(x, 4) for k, x in ph.items()})
     if one_thread:
-        _, v1, _, dt1 = runs[1]
-        res["one_thread"] = dict(value=round(v1, 1), unit="voxels/s", seconds=round(dt1, 2))
+        _, v1, _, dt1, ph1 = runs[1]
+        res["one_thread"] = dict(value=round(v1, 1), unit="voxels/s", seconds=round(dt1, 2),
+                                 phases_s={k: round(x, 4) for k, x in ph1.items()})
     return res
 
 

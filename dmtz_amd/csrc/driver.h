@@ -1892,9 +1892,12 @@ struct Driver {
     // share four hardware queues, and two streams on one queue run one after the other.
 
     // Group the n records of kind ty (1..3) by branch class; recs then points at the grouped copy.
-    // Grouping only orders the records, so lists beyond 32-bit slots are left as they are.
+    // Grouping only orders the records, so lists beyond 32-bit slots are left as they are, and
+    // so are short lists: a few blocks of records take as long as their slowest chain whatever
+    // their order, and the grouping's five launches would cost more than it saves.
+    static constexpr ull FIX_GROUP_MIN = 8192;
     int sort_class(int ty, const uint64_t*& recs, ull n) {
-        if (n < 2 || n >= (1ull << 32)) return 0;
+        if (n < FIX_GROUP_MIN || n >= (1ull << 32)) return 0;
         const int k = ty - 1;
         hipStream_t st = ctx->stream;
         int rc = grow_buf(ctx, ctx->d_fk[k], ctx->fk_cap[k], n);

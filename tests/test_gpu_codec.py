@@ -114,3 +114,33 @@ def test_rle_matches_the_oracle(dev, oracle):
         assert np.array_equal(codec.bitmap_rle(bm, ctx=dev), oracle.rle_bitmap(bm)), len(bm)
     assert codec.bitmap_rle(b"", ctx=dev).shape == (0, 2)
     assert codec.bitmap_rle(bytes([0xF0, 0x0F]), ctx=dev).tolist() == [[1, 4], [0, 8], [1, 4]]
+
+
+@pytest.mark.timeout(600)
+def test_streams_round_trip_at_512(dev):
+    """The 512^3 fBm field at eps = 1e-2 range with the counter-driven edit, geometry and finisher:
+    both edit streams decode the uncorrected field back to the corrected one bit for bit, the
+    edited count is the number of changed values, and the run-length code of the bitmap covers
+    its bits with alternating runs."""
+    f = fields.fbm((512, 512, 512), seed=0)
+    eps, rng = fields.eps_from_rel(f, 1e-2)
+    g0 = fields.quantize(f, eps)
+    fl = dmtz_amd.CorrectionFlags(preserve_vpath=1, preserve_geometry=1, finish=1, edit_type=1)
+    print("512^3 correction for the streams", flush=True)
+    fd, gd = DeviceArray.from_numpy(f), DeviceArray.from_numpy(g0)
+    cd = DeviceArray(f.shape, np.uint8)
+    r = dmtz_amd.correct_device(fd, gd, eps, rng, fl, ctx=dev, counter_out=cd)
+    assert r.status == 0 and r.counts == (0,) * 8
+    g1 = gd.numpy()
+    changed = int(np.count_nonzero(g1 != g0))
+    g0d = DeviceArray.from_numpy(g0)
+    for edit_type in (1, 0):
+        print("stream type", edit_type, flush=True)
+        s = codec.edit_stream(g0d, gd, cd if edit_type == 1 else None, eps=eps, edit_type=edit_type, ctx=dev)
+        assert s.edited == changed
+        back = DeviceArray.from_numpy(g0)
+        codec.edit_apply(back, s, eps=eps, ctx=dev)
+        assert back.numpy().tobytes() == g1.tobytes()
+        if edit_type == 1:
+            runs = codec.bitmap_rle(s.index, ctx=dev)
+            assert runs[:, 1].sum() == 8 * len(s.index) and (runs[1:, 0] != runs[:-1, 0]).all()
