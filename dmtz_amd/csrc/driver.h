@@ -2697,7 +2697,7 @@ struct Driver {
                         stalled = true;
                         return 0;
                     }
-                    ctx->radius++;
+                    ctx->radius = ctx->radius ? 2 * ctx->radius : 1;  // 1, 2, 4, ...
                     int r = finish_step(cnt);
                     if (r) return r;
                     r = kind == 0 ? eval_cp() : eval_path();

@@ -43,10 +43,10 @@ extern "C" {
 /* The reference's positional flags (main_3d.cu:6264-6271) plus a sweep cap and a finisher.
  * max_sweeps <= 0 selects the default cap (100000). mode: 0 = plain loop.
  * finish = 1 (no counterpart in the reference): on the k-th stall, snap every vertex within
- * Chebyshev distance k of the edit sets of the stalled sweeps (g = the snap of f - eps, counter 6:
- * the terminal edit of main_3d.cu:1140-1141, which the reference never undoes). When those boxes
+ * Chebyshev distance 2^(k-1) of the edit sets of the stalled sweeps (g = the snap of f - eps,
+ * counter 6: the terminal edit of main_3d.cu:1140-1141, which the reference never undoes). When those boxes
  * would cover the grid, or the edit sets are empty, restore g = f on every vertex instead
- * (counter 7, never edited again). Edits only lower g or raise counters and the radius grows
+ * (counter 7, never edited again). Edits only lower g or raise counters and the radius doubles
  * with every stall, so the loop then ends with zero mismatches and |g - f| <= eps. Runs that
  * never stall are unchanged.
  * preserve_connectors = 1 (with preserve_vpath = 1): the walls check of main_3d.cu:5064-5133

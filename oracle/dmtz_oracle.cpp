@@ -38,8 +38,8 @@
 //     that changes no value or counter and repeats an earlier control configuration), where the
 //     reference loops forever, and reports it as stalled.
 // Extension without a counterpart in the reference (flag finish, off by default): on the k-th
-// stall every vertex within Chebyshev distance k of the zone (the edit sets of the stalled sweeps
-// and the vertices of the mismatched critical cells) is snapped (the reference's terminal edit,
+// stall every vertex within Chebyshev distance 2^(k-1) of the zone (the edit sets of the stalled
+// sweeps and the vertices of the mismatched critical cells) is snapped (the reference's terminal edit,
 // main_3d.cu:1140-1141; a snap of the 14 neighbours sits commented out at main_3d.cu:1144-1151),
 // ties the float snap creates are repaired by one-ulp raises, and the box is frozen (counter 7,
 // never edited again). Once the boxes would cover the grid, or when the zone is empty, g = f is
@@ -2331,7 +2331,7 @@ static int correct_impl(const T* f, T* gfield, i64 W, i64 H, i64 D, double eps, 
         }
         stalls++;
         if (fl->finish != 1) return true;
-        radius++;
+        radius = radius ? 2 * radius : 1;  // 1, 2, 4, ...: a run that keeps stalling ends soon
         s.mismatch_vertices(c, zone);  // the zone: stalled edit sets + vertices of mismatched cells
         finished += s.finish_step(zone, radius);
         zone.clear();

@@ -324,7 +324,7 @@ def test_finisher_end_state(oracle, case):
     f, g, eps, rng = _stall_input(*case)
     r = oracle.correct(f, g, eps, rng, oracle.flags(preserve_saddles=0, max_sweeps=300, finish=1))
     assert r["status"] == 0 and r["counts"] == (0,) * 8
-    assert r["stalls"] == r["radius"] >= 1
+    assert r["stalls"] >= 1 and r["radius"] == 2 ** (r["stalls"] - 1)  # the radius doubles per stall
     frozen = r["counter"] == 7
     assert 0 < frozen.sum() <= r["finished"] < f.size  # local boxes; nothing restored to f
     assert (r["g"][~frozen] <= g[~frozen]).all()  # outside the boxes every edit lowers g
