@@ -129,23 +129,28 @@ __global__ __launch_bounds__(BLOCK) void k_features(const float* g, Dims d, i64 
 // MODE 0: Y = ReLU(conv(X, W) + b) as bf16; MODE 1: Y = conv(X, W) + b, channels 0..1 as fp32;
 // MODE 2: dX(v) = ReLU'(M(v)) * sum_tap W[tap]^T dY(v - off(tap)) as bf16 (X is dY, M the
 // layer's input, whose positive entries are those where the ReLU below passed its input).
-// Waves stride over 16-voxel runs (a block reads the weights once for several runs). Per run a
-// wave stages the run's 9 rows of X (y - 1..y + 1, z - 1..z + 1, over x0 - 1..x0 + 16,
-// channel-last as in memory) in its own part of LDS with 16-byte loads, then reads each tap's A
-// fragment (4 channels of one voxel) from there.
+// Waves stride over 16-voxel runs; each lane keeps its B fragments of all 27 taps in registers
+// for every run. Per run a wave stages the run's 9 rows of X (y - 1..y + 1, z - 1..z + 1, over
+// x0 - 1..x0 + 16, channel-last as in memory) in its own part of LDS with 16-byte loads, then
+// reads each tap's A fragment (4 channels of one voxel) from there; the taps feed three
+// independent accumulators, so consecutive MFMAs do not wait on one another.
 template <int MODE>
 __global__ __launch_bounds__(BLOCK) void k_conv(const uint16_t* __restrict__ X, const uint16_t* __restrict__ W,
                                                 const float* __restrict__ bias, const uint16_t* __restrict__ M,
                                                 int nx, int ny, int nz, uint16_t* Yb, float* Yf) {
-    __shared__ uint16_t ws[WSZ];
     __shared__ __attribute__((aligned(16))) uint16_t xt[BLOCK / 64][9][TXC][CH];
-    for (int k = threadIdx.x; k < WSZ; k += BLOCK) ws[k] = W[k];
-    __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, i = lane & 15, q = lane >> 4;
     const int nxs = (nx + 15) >> 4;
     const long S = (long)nxs * ny * nz;
     const int sgn = MODE == 2 ? -1 : 1;
     const float bi = MODE == 2 ? 0.f : bias[i];
+    // the B fragments of all 27 taps stay in registers for every run the wave takes
+    short4_t bw[TAPS];
+#pragma unroll
+    for (int tap = 0; tap < TAPS; tap++)
+        for (int r = 0; r < 4; r++)
+            bw[tap][r] = (short)(MODE == 2 ? W[tap * CH * CH + i * CH + 4 * q + r]      // W[tap][ci = i][co = 4q + r]
+                                           : W[tap * CH * CH + (4 * q + r) * CH + i]);  // W[tap][ci = 4q + r][co = i]
     // runs of one wave are ordered by wave barriers only (waves of a block take different numbers)
     for (long w = (long)blockIdx.x * (BLOCK / 64) + wv; w < S; w += (long)gridDim.x * (BLOCK / 64)) {
         int x0, y, z;
@@ -161,19 +166,19 @@ __global__ __launch_bounds__(BLOCK) void k_conv(const uint16_t* __restrict__ X, 
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
         const int x = x0 + i;
-        float4_t acc = {0.f, 0.f, 0.f, 0.f};
+        // three independent accumulator chains (taps t, t + 1, t + 2 of each group of three), so
+        // that consecutive MFMAs do not wait on each other's results; summed at the end
+        float4_t acc3[3] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
         for (int tap = 0; tap < TAPS; tap++) {
             int dx, dy, dz;
             tap_offset(tap, dx, dy, dz);
             const int row = (sgn * dz + 1) * 3 + (sgn * dy + 1);
             short4_t a = {0, 0, 0, 0};
             if (x < nx) a = *(const short4_t*)&xt[wv][row][i + sgn * dx + 1][4 * q];
-            short4_t b;
-            for (int r = 0; r < 4; r++)
-                b[r] = (short)(MODE == 2 ? ws[tap * CH * CH + i * CH + 4 * q + r]      // W[tap][ci = i][co = 4q + r]
-                                         : ws[tap * CH * CH + (4 * q + r) * CH + i]);  // W[tap][ci = 4q + r][co = i]
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+            acc3[tap % 3] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw[tap], acc3[tap % 3], 0, 0, 0);
         }
+        const float4_t acc = acc3[0] + acc3[1] + acc3[2];
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
         for (int r = 0; r < 4; r++) {
