@@ -17,7 +17,10 @@ The JSON line also carries
                   timed with HIP events on its stream;
   edit_ratio      edited voxels / N (the reference's edit_ratio, main_3d.cu:1553);
   other_eps       the same field at the other eps of the scale configuration;
-  learned         the learned warm start on the same field;
+  learned         the learned warm start on the same field, and the uniform warm start beside it;
+  batched         the batched configuration (48 fields, sides 32..192): plain (one field after
+                  another), batched, batched+learned and batched with the uniform warm start, and
+                  each one's speed-up over plain, the metric's batched+learned vs plain ratio;
   cpu_baseline    the oracle (C++/OpenMP restatement of the reference routines) on the CPU-runnable
                   workload (64^3 Gaussian mixture, 8 bumps, eps = 1e-3 * range) on all the host cores
                   it may use and on one thread.
@@ -147,6 +150,8 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB four times (the known-size "
                          "reference for scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
+    ap.add_argument("--batch-leg", type=int, default=48, metavar="K",
+                    help="fields of the batched leg reported in the main line (0: none)")
     ap.add_argument("--batch", type=int, default=0, metavar="K",
                     help="time the batched form instead: K fields with sides drawn from 32..192, half "
                          "Gaussian mixtures and half fBm, eps log-uniform in [1e-4, 1e-2]*range")
@@ -282,6 +287,13 @@ def main():
     if args.learned and args.learned != "none" and os.path.exists(args.learned) and args.dtype == "f32":
         learned = bench_learned(args, ctx, f, f_d, main_run, g0_d, g_d, flags, value)
 
+    batched = None
+    if args.batch_leg > 0 and args.dtype == "f32":
+        forms, voxels, bworkload = batch_forms(args, args.batch_leg, max(1, min(args.steps, 3)), 1)
+        plain_v = forms["plain"]["value"]
+        batched = dict(workload=bworkload, voxels=voxels, forms=forms,
+                       speedup_vs_plain={k: round(v["value"] / plain_v, 3) for k, v in forms.items()})
+
     cpu = None if args.cpu == "none" else cpu_baseline(one_thread=args.cpu == "full")
 
     out = {
@@ -308,6 +320,8 @@ def main():
     }
     if learned is not None:
         out["learned"] = learned
+    if batched is not None:
+        out["batched"] = batched
     print(json.dumps(out), flush=True)
 
 
@@ -380,19 +394,19 @@ def batch_fields(k, seed):
     return out
 
 
-def bench_batch(args):
-    """The batched forms on K device-resident fields, timed against the plain form on the same
-    fields (one field after another): plain, batched (one dmtz_correct_batch call) and
+def batch_forms(args, k, steps, warmup):
+    """The batched forms on k device-resident fields, timed against the plain form on the same
+    fields (one field after another): plain, batched (one dmtz_correct_batch call),
     batched+learned (dmtz_correct_batch_learned_f32: the warm start on every field, then the
-    batch). value is the batched+learned rate; the ratio to the plain form is the metric's."""
+    batch) and the batch with the uniform warm start. Returns (forms, voxels, workload)."""
     import numpy as np
 
     import dmtz_amd
     from dmtz_amd import devmem
     from dmtz_amd import learned as L
 
-    log("generating %d batch fields" % args.batch)
-    items = batch_fields(args.batch, args.seed)
+    log("generating %d batch fields" % k)
+    items = batch_fields(k, args.seed)
     fd = [devmem.DeviceArray.from_numpy(f) for f, _, _, _, _ in items]
     g0 = [devmem.DeviceArray.from_numpy(np.ascontiguousarray(g, dtype=np.float32)) for _, g, _, _, _ in items]
     gd = [devmem.DeviceArray(x.shape, np.float32) for x in g0]
@@ -430,11 +444,11 @@ def bench_batch(args):
         forms += [("batched_learned", batched_learned), ("batched_uniform_warm_start", batched_uniform)]
     out_forms = {}
     for name, fn in forms:
-        for w in range(args.warmup):
+        for w in range(warmup):
             reset()
             fn()
         t = 0.0
-        for s in range(args.steps):
+        for s in range(steps):
             reset()
             devmem.synchronize()
             t0 = time.perf_counter()
@@ -448,17 +462,24 @@ def bench_batch(args):
             err = float(np.abs(gi.astype(np.float64) - items[i][0].astype(np.float64)).max())
             assert err <= items[i][2], (name, i, err)
             edited += int(np.count_nonzero(gi != items[i][1]))
-        out_forms[name] = dict(value=round(voxels * args.steps / t, 1), ms_per_step=round(t / args.steps * 1e3, 3),
+        out_forms[name] = dict(value=round(voxels * steps / t, 1), ms_per_step=round(t / steps * 1e3, 3),
                                sweeps=int(sum(r.sweeps for r in rs)), edit_ratio=round(edited / voxels, 6))
-        log("%s: %.1f ms per batch, %d sweeps" % (name, t / args.steps * 1e3, out_forms[name]["sweeps"]))
+        log("%s: %.1f ms per batch, %d sweeps" % (name, t / steps * 1e3, out_forms[name]["sweeps"]))
     if net:
         net.close()
         uniform.close()
-    best = "batched_learned" if net else "batched"
-    plain_v = out_forms["plain"]["value"]
     workload = ("batch of %d float32 fields, sides 32..192, Gaussian mixture / fBm, eps log-uniform "
-                "[1e-4, 1e-2]*range, %s" % (args.batch, "critical cells + separatrix geometry"
-                                             if args.geometry else "critical cells"))
+                "[1e-4, 1e-2]*range, %s" % (k, "critical cells + separatrix geometry"
+                                         if args.geometry else "critical cells"))
+    return out_forms, voxels, workload
+
+
+def bench_batch(args):
+    """The batched forms as the line: value is the batched+learned rate; the ratio to the plain
+    form is the metric's."""
+    out_forms, voxels, workload = batch_forms(args, args.batch, args.steps, args.warmup)
+    best = "batched_learned" if "batched_learned" in out_forms else "batched"
+    plain_v = out_forms["plain"]["value"]
     out = {
         "metric": "voxels/s to zero topology mismatch within eps (%s vs plain, 1 GPU)" % best.replace("_", "+"),
         "value": out_forms[best]["value"],
