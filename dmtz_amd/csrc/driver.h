@@ -630,6 +630,15 @@ __global__ __launch_bounds__(BLOCK) void k_fix_cp(const uint64_t* recs, ull n, c
     else R.fix_maximum(id);
 }
 
+// Zero the counters whose bit is set in mask: one launch where a memset per run of counters
+// would take several.
+__global__ __launch_bounds__(64) void k_zero_ctrs(ull* ctr, uint32_t mask) {
+    const int k = (int)threadIdx.x;
+    if (k < K_NCTR && ((mask >> k) & 1u)) ctr[k] = 0;
+}
+static_assert(K_NCTR <= 32, "k_zero_ctrs takes a 32-bit mask");
+constexpr uint32_t ctr_bit(int k) { return 1u << k; }
+
 // Frozen vertices are never edited. moved counts the edits that changed a value or a counter.
 // The edit set's size is read on the device (*n_ptr, at most cap); the grid strides over the set
 // in block-uniform steps, so every lane takes part in the wave-wide appends.
@@ -1836,8 +1845,9 @@ struct Driver {
     }
 
     int detect_once() {
-        CK(hipMemsetAsync(ctx->d_ctr + K_MIN, 0, sizeof(ull) * 4, ctx->stream));
-        CK(hipMemsetAsync(ctx->d_ctr + K_DTILES, 0, sizeof(ull), ctx->stream));
+        k_zero_ctrs<<<1, 64, 0, ctx->stream>>>(ctx->d_ctr, ctr_bit(K_MIN) | ctr_bit(K_MIN + 1) | ctr_bit(K_MIN + 2) |
+                                                        ctr_bit(K_MIN + 3) | ctr_bit(K_DTILES));
+        CK(hipGetLastError());
         CK(hipEventRecord(ctx->ev0, ctx->stream));
         k_detect<<<(unsigned)((c.N + EMIT_TILE - 1) / EMIT_TILE), BLOCK, 0, ctx->stream>>>(
             ctx->d_mm, c.N, ctx->d_tile_nz, rec_lists(ctx), ctx->d_ctr + K_MIN, ctx->d_ctr + K_DTILES);
@@ -1871,7 +1881,7 @@ struct Driver {
 
     uint32_t zone_from = 0;  // first epoch of the unchanged sweeps since the last change (0: none)
 
-    int begin_edits() {
+    int begin_edits(bool for_apply = false) {
         ctx->epoch++;
         if (ctx->epoch == 0) {  // stamp wrap (after 2^32 edit sets): clear and restart; the finisher
                                 // zone then keeps only the edit sets from here on
@@ -1880,7 +1890,12 @@ struct Driver {
             ctx->epoch = 1;
             if (zone_from) zone_from = 1;
         }
-        CK(hipMemsetAsync(ctx->d_ctr + K_TGT, 0, sizeof(ull), ctx->stream));
+        // the edit set; with for_apply also the counters of its apply (changed and dirty lists,
+        // moved edits), whose last values the host has read before this epoch begins (the
+        // finisher keeps its changed list across its epochs, so it zeroes those itself)
+        k_zero_ctrs<<<1, 64, 0, ctx->stream>>>(
+            ctx->d_ctr, ctr_bit(K_TGT) | (for_apply ? ctr_bit(K_CHG) | ctr_bit(K_DIRTY) | ctr_bit(K_MOVED) : 0u));
+        CK(hipGetLastError());
         return 0;
     }
 
@@ -1926,7 +1941,7 @@ struct Driver {
     }
 
     int fix_cp(const ull counts[4]) {
-        int rc = begin_edits();
+        int rc = begin_edits(true);
         if (rc) return rc;
         ull nrec = 0;
         rc = repair_records(counts, nrec);
@@ -1985,9 +2000,7 @@ struct Driver {
     // launches, and the count of edits that changed a value or a counter (K_MOVED) is read with
     // the next detection's counters. Each vertex enters the edit set at most once per epoch, so
     // the set never exceeds the N entries of its buffer.
-    int apply(ull nrec) {
-        CK(hipMemsetAsync(ctx->d_ctr + K_CHG, 0, sizeof(ull) * 2, ctx->stream));
-        CK(hipMemsetAsync(ctx->d_ctr + K_MOVED, 0, sizeof(ull), ctx->stream));
+    int apply(ull nrec) {  // its counters start at zero with the epoch (begin_edits(true))
         if (!nrec) return 0;  // no record, no edit
         const ull est = nrec < (ull)c.N ? nrec : (ull)c.N;
         k_apply<T><<<stride_blocks(est), BLOCK, 0, ctx->stream>>>(
@@ -2562,7 +2575,7 @@ struct Driver {
     }
 
     int fix_paths(ull nds, ull nas, ull nw) {
-        int rc = begin_edits();
+        int rc = begin_edits(true);
         if (rc) return rc;
         if (fl.preserve_connectors != 1) nw = 0;
         if (nds + nas + nw) {
