@@ -27,7 +27,7 @@ def closed_counts(W, H, D):
 
 def test_counts_closed_form(oracle):
     assert oracle.counts(5, 6, 7) == (210, 1077, 1588, 720)
-    # the survey's table (sizes at the configured workloads)
+    # simplex counts at the 64^3, 256^3 and 512^3 workloads (closed forms of main_3d.cu:6401-6415)
     assert oracle.counts(64, 64, 64) == (262144, 1786239, 3024378, 1500282)
     assert oracle.counts(256, 256, 256) == (16777216, 116655615, 199366650, 99488250)
     assert oracle.counts(512, 512, 512) == (134217728, 936381439, 1602760698, 800596986)
