@@ -129,72 +129,78 @@ __global__ __launch_bounds__(BLOCK) void k_features(const float* g, Dims d, i64 
 // MODE 0: Y = ReLU(conv(X, W) + b) as bf16; MODE 1: Y = conv(X, W) + b, channels 0..1 as fp32;
 // MODE 2: dX(v) = ReLU'(M(v)) * sum_tap W[tap]^T dY(v - off(tap)) as bf16 (X is dY, M the
 // layer's input, whose positive entries are those where the ReLU below passed its input).
-// Waves stride over 16-voxel runs; each lane keeps its B fragments of all 27 taps in registers
-// for every run. Per run a wave stages the run's 9 rows of X (y - 1..y + 1, z - 1..z + 1, over
-// x0 - 1..x0 + 16, channel-last as in memory) in its own part of LDS with 16-byte loads, then
-// reads each tap's A fragment (4 channels of one voxel) from there; the taps feed three
-// independent accumulators, so consecutive MFMAs do not wait on one another.
+// Waves stride over work items of CONV_YC consecutive 16-voxel runs along y (one x0, one z); each
+// lane keeps its B fragments of all 27 taps in registers. Per item a wave stages the rows of X
+// the item's runs need (y0 - 1..y0 + CONV_YC, z - 1..z + 1, over x0 - 1..x0 + 16, channel-last as
+// in memory) in its own part of LDS with 16-byte loads, 3 (CONV_YC + 2) rows where one run at a
+// time would stage 9 CONV_YC; each run then reads its taps' A fragments (4 channels of one voxel)
+// from there, into three independent accumulators so that consecutive MFMAs do not wait on one
+// another.
+constexpr int CONV_YC = 4;
 template <int MODE>
 __global__ __launch_bounds__(BLOCK) void k_conv(const uint16_t* __restrict__ X, const uint16_t* __restrict__ W,
                                                 const float* __restrict__ bias, const uint16_t* __restrict__ M,
                                                 int nx, int ny, int nz, uint16_t* Yb, float* Yf) {
-    __shared__ __attribute__((aligned(16))) uint16_t xt[BLOCK / 64][9][TXC][CH];
+    constexpr int NR = CONV_YC + 2;  // staged rows per z plane
+    __shared__ __attribute__((aligned(16))) uint16_t xt[BLOCK / 64][3][NR][TXC][CH];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, i = lane & 15, q = lane >> 4;
-    const int nxs = (nx + 15) >> 4;
-    const long S = (long)nxs * ny * nz;
+    const int nxs = (nx + 15) >> 4, nys = (ny + CONV_YC - 1) / CONV_YC;
+    const long S = (long)nxs * nys * nz;
     const int sgn = MODE == 2 ? -1 : 1;
     const float bi = MODE == 2 ? 0.f : bias[i];
-    // the B fragments of all 27 taps stay in registers for every run the wave takes
     short4_t bw[TAPS];
 #pragma unroll
     for (int tap = 0; tap < TAPS; tap++)
         for (int r = 0; r < 4; r++)
             bw[tap][r] = (short)(MODE == 2 ? W[tap * CH * CH + i * CH + 4 * q + r]      // W[tap][ci = i][co = 4q + r]
                                            : W[tap * CH * CH + (4 * q + r) * CH + i]);  // W[tap][ci = 4q + r][co = i]
-    // runs of one wave are ordered by wave barriers only (waves of a block take different numbers)
+    // items of one wave are ordered by wave barriers only (waves of a block take different numbers)
     for (long w = (long)blockIdx.x * (BLOCK / 64) + wv; w < S; w += (long)gridDim.x * (BLOCK / 64)) {
-        int x0, y, z;
-        run_origin(w, nxs, ny, x0, y, z);
-        for (int e = lane; e < 9 * TXC * 2; e += 64) {  // 16-byte chunks (8 channels)
+        const int x0 = (int)(w % nxs) * 16;
+        const long rest = w / nxs;
+        const int y0 = (int)(rest % nys) * CONV_YC, z = (int)(rest / nys);
+        for (int e = lane; e < 3 * NR * TXC * 2; e += 64) {  // 16-byte chunks (8 channels)
             const int row = e / (TXC * 2), rem = e % (TXC * 2), xi = rem >> 1, half = rem & 1;
-            const int yy = y + row % 3 - 1, zz = z + row / 3 - 1, xx = x0 - 1 + xi;
+            const int pz = row / NR, py = row % NR;
+            const int yy = y0 - 1 + py, zz = z - 1 + pz, xx = x0 - 1 + xi;
             uint4 val = make_uint4(0u, 0u, 0u, 0u);
             if (xx >= 0 && xx < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz)
                 val = *(const uint4*)(X + (((long)zz * ny + yy) * nx + xx) * CH + 8 * half);
-            *(uint4*)&xt[wv][row][xi][8 * half] = val;
+            *(uint4*)&xt[wv][pz][py][xi][8 * half] = val;
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
         const int x = x0 + i;
-        // three independent accumulator chains (taps t, t + 1, t + 2 of each group of three), so
-        // that consecutive MFMAs do not wait on each other's results; summed at the end
-        float4_t acc3[3] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+        for (int ry = 0; ry < CONV_YC; ry++) {
+            const int y = y0 + ry;
+            if (y >= ny) break;  // uniform over the wave
+            float4_t acc3[3] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
-        for (int tap = 0; tap < TAPS; tap++) {
-            int dx, dy, dz;
-            tap_offset(tap, dx, dy, dz);
-            const int row = (sgn * dz + 1) * 3 + (sgn * dy + 1);
-            short4_t a = {0, 0, 0, 0};
-            if (x < nx) a = *(const short4_t*)&xt[wv][row][i + sgn * dx + 1][4 * q];
-            acc3[tap % 3] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw[tap], acc3[tap % 3], 0, 0, 0);
-        }
-        const float4_t acc = acc3[0] + acc3[1] + acc3[2];
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        for (int r = 0; r < 4; r++) {
-            const int vx = x0 + 4 * q + r;
-            if (vx >= nx) continue;
-            const long v = ((long)z * ny + y) * nx + vx;
-            const float val = acc[r];
-            if (MODE == 0) {
-                const float o = val + bi;
-                Yb[v * CH + i] = f2bf(o > 0.f ? o : 0.f);
-            } else if (MODE == 1) {
-                if (i < COUT_LAST) Yf[v * COUT_LAST + i] = val + bi;
-            } else {
-                Yb[v * CH + i] = bf2f(M[v * CH + i]) > 0.f ? f2bf(val) : (uint16_t)0;
+            for (int tap = 0; tap < TAPS; tap++) {
+                int dx, dy, dz;
+                tap_offset(tap, dx, dy, dz);
+                short4_t a = {0, 0, 0, 0};
+                if (x < nx) a = *(const short4_t*)&xt[wv][sgn * dz + 1][ry + sgn * dy + 1][i + sgn * dx + 1][4 * q];
+                acc3[tap % 3] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw[tap], acc3[tap % 3], 0, 0, 0);
+            }
+            const float4_t acc = acc3[0] + acc3[1] + acc3[2];
+            for (int r = 0; r < 4; r++) {
+                const int vx = x0 + 4 * q + r;
+                if (vx >= nx) continue;
+                const long v = ((long)z * ny + y) * nx + vx;
+                const float val = acc[r];
+                if (MODE == 0) {
+                    const float o = val + bi;
+                    Yb[v * CH + i] = f2bf(o > 0.f ? o : 0.f);
+                } else if (MODE == 1) {
+                    if (i < COUT_LAST) Yf[v * COUT_LAST + i] = val + bi;
+                } else {
+                    Yb[v * CH + i] = bf2f(M[v * CH + i]) > 0.f ? f2bf(val) : (uint16_t)0;
+                }
             }
         }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -432,10 +438,17 @@ unsigned conv_blocks(int64_t nx, int64_t ny, int64_t nz) {
     return (unsigned)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
 }
 
+// k_conv's grid: its work items are CONV_YC runs each, two items per wave.
+unsigned conv_item_blocks(int64_t nx, int64_t ny, int64_t nz) {
+    const long items = ((nx + 15) / 16) * ((ny + CONV_YC - 1) / CONV_YC) * nz;
+    const long b = (items + 7) / 8;
+    return (unsigned)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
+}
+
 // Forward pass from feat; the output layer's result goes to out (2 floats per vertex).
 int forward(dmtz_ctx* ctx, dmtz_net* net, const uint16_t* feat, int64_t nx, int64_t ny, int64_t nz, float* out) {
     hipStream_t s = dmtz_internal::stream(ctx);
-    const unsigned gb = conv_blocks(nx, ny, nz);
+    const unsigned gb = conv_item_blocks(nx, ny, nz);
     const uint16_t* in = feat;
     for (int l = 0; l < NL - 1; l++) {
         k_conv<0><<<gb, BLOCK, 0, s>>>(in, net->wb + l * WSZ, net->params + l * LP + WSZ, nullptr, (int)nx, (int)ny,
@@ -463,7 +476,7 @@ int backprop(dmtz_ctx* ctx, dmtz_net* net, const uint16_t* feat, const float* la
     NK(hipMemsetAsync(net->sums, 0, sizeof(float) * 2, s));
     k_loss<<<blocks_for(n), BLOCK, 0, s>>>(net->z, labels, n, pos_weight, lambda, net->grad[0], net->sums);
     NK(hipGetLastError());
-    const unsigned gb = conv_blocks(nx, ny, nz);
+    const unsigned gb = conv_item_blocks(nx, ny, nz);
     const long runs = ((nx + 15) / 16) * ny * nz;
     const unsigned wg = (unsigned)std::min<long>(512, (runs + 3) / 4);
     int cur = 0;  // grad[cur] holds the gradient at the current layer's output (before its ReLU)
