@@ -437,6 +437,52 @@ struct TargetSink {
     }
 };
 
+// The targets a lane's rule emits, held in registers so that the rule's chain of reads never
+// waits on the edit set: the stamp exchanges of all of them are issued together at the end, and
+// the wave claims list slots for the new ones with one atomic. No repair rule emits more than
+// four; a fifth target would go through the direct sink at once.
+struct HoldSink {
+    TargetSink direct;
+    uint32_t t0, t1, t2, t3;
+    int n;
+    __device__ __forceinline__ void operator()(i64 v) {
+        if (n < 4) {
+            t3 = t2;
+            t2 = t1;
+            t1 = t0;
+            t0 = (uint32_t)v;
+            n++;
+        } else {
+            direct(v);
+        }
+    }
+    // Every lane of the wave must call this. Adds the held targets not yet in this epoch's set.
+    __device__ __forceinline__ void flush() const {
+        const uint32_t e = direct.epoch;
+        uint32_t o0 = e, o1 = e, o2 = e, o3 = e;
+        if (n > 0) o0 = atomicExch(direct.stamp + t0, e);
+        if (n > 1) o1 = atomicExch(direct.stamp + t1, e);
+        if (n > 2) o2 = atomicExch(direct.stamp + t2, e);
+        if (n > 3) o3 = atomicExch(direct.stamp + t3, e);
+        const bool n0 = o0 != e, n1 = o1 != e, n2 = o2 != e, n3 = o3 != e;
+        ull at = wave_alloc(direct.ctr, (int)n0 + (int)n1 + (int)n2 + (int)n3);
+        const ull cap = direct.cap;
+        if (n0) {
+            if (at < cap) direct.list[at] = t0;
+            at++;
+        }
+        if (n1) {
+            if (at < cap) direct.list[at] = t1;
+            at++;
+        }
+        if (n2) {
+            if (at < cap) direct.list[at] = t2;
+            at++;
+        }
+        if (n3 && at < cap) direct.list[at] = t3;
+    }
+};
+
 // Branch classes of the repair rules: the first split of each rule, i.e. the cell's partner in f
 // (an edge with its vertex or a triangle; a triangle with an edge or a tet) or, when f leaves it
 // critical, its partner in g, and for the 2-saddle and maximum rules the next split (how the
@@ -618,16 +664,18 @@ __global__ __launch_bounds__(BLOCK) void k_fix_cp(const uint64_t* recs, ull n, c
                                                   const uint8_t* counter, double eps, double range, int edit_type,
                                                   TargetSink sink) {
     const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const uint64_t r = recs[i];
-    const i64 v = (i64)(r >> 7);
-    const int lc = (int)(r & 127u);
-    const i64 id = local_cell_id(d, c, v, lc);
-    Repair<T, TargetSink> R(f, cf, g, cg, d, c, counter, eps, range, edit_type, sink);
-    if (TYPE == 0) R.fix_minimum(id);
-    else if (TYPE == 1) R.fix_saddle(id);
-    else if (TYPE == 2) R.fix_2saddle(id);
-    else R.fix_maximum(id);
+    Repair<T, HoldSink> R(f, cf, g, cg, d, c, counter, eps, range, edit_type, HoldSink{sink, 0u, 0u, 0u, 0u, 0});
+    if (i < n) {
+        const uint64_t r = recs[i];
+        const i64 v = (i64)(r >> 7);
+        const int lc = (int)(r & 127u);
+        const i64 id = local_cell_id(d, c, v, lc);
+        if (TYPE == 0) R.fix_minimum(id);
+        else if (TYPE == 1) R.fix_saddle(id);
+        else if (TYPE == 2) R.fix_2saddle(id);
+        else R.fix_maximum(id);
+    }
+    R.sink.flush();
 }
 
 // Zero the counters whose bit is set in mask: one launch where a memset per run of counters
@@ -1487,10 +1535,11 @@ __global__ __launch_bounds__(BLOCK) void k_fix_paths(const int64_t* as_rec, ull 
                                                      Counts c, const uint8_t* counter, double eps, double range,
                                                      int edit_type, TargetSink sink) {
     const ull i = (ull)blockIdx.x * BLOCK + threadIdx.x;
-    Repair<T, TargetSink> R(f, cf, g, cg, d, c, counter, eps, range, edit_type, sink);
+    Repair<T, HoldSink> R(f, cf, g, cg, d, c, counter, eps, range, edit_type, HoldSink{sink, 0u, 0u, 0u, 0u, 0});
     if (i < n_as) R.fix_as_vpath(as_rec[i]);
     else if (i < n_as + n_ds) R.fix_ds_vpath(ds_rec[i - n_as]);
     else if (i < n_as + n_ds + n_wall) R.fix_ds_wall(wall_rec[i - n_as - n_ds]);
+    R.sink.flush();
 }
 
 // Dense pairing arrays for tests (dmtz_gradient_*): one thread per cell of each dimension.
