@@ -1221,7 +1221,7 @@ __global__ __launch_bounds__(BLOCK) void k_as_count_ws(const uint64_t* fcf, ull 
 }
 
 // Steps of thread t's branch at off[t]: copied from the counting walk's first keep steps, or
-// walked again. Thread t takes branch order[t] (branches by path length, so that a wave's walks
+// walked again. Thread t takes branch order[t] (the check order of k_len_scatter: a wave's walks
 // are alike; off then holds the offsets in that order) or branch t when order is null.
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_as_fill(const uint64_t* fcf, ull nb, const uint32_t* order, const T* f,
@@ -1293,11 +1293,27 @@ __global__ __launch_bounds__(BLOCK) void k_as_gather_cnt(ull nb, const uint32_t*
     else if (t == nb) out[nb] = 0;
 }
 
-// Branch order of the checks: branches grouped by step count (capped at 255), so that a wave's
-// walks have similar lengths and the branches without a step come first (the checks skip them).
+// Branch order of the checks: the branches without a step first (the checks skip them), then
+// the others in chunks of LEN_CHUNK_BLOCKS * CS_PER_BLOCK consecutive branches, within a chunk
+// grouped by step count (capped at 255) so that a wave's walks have similar lengths. The
+// f-critical faces are listed in about vertex order, so a chunk covers a few planes of the grid
+// and the checks read one region at a time: neighbouring branches share the values and star
+// codes they read while these are cached. (Grouped by length over the whole grid, most branches
+// being a few steps long, every region was read again for each length.)
 // A counting sort like k_class_hist / k_class_scatter, with the key read from the counts.
 constexpr int LEN_BINS = 256;
+constexpr ull LEN_CHUNK_BLOCKS = 64;
 __device__ __forceinline__ int len_bin(const ull* cnt, ull j) { return (int)(cnt[j] < 255 ? cnt[j] : 255); }
+
+// Position of (block blk, length bin) in the count table of nblk blocks: bin 0 of every block,
+// then the chunks in order, each bin-major (the last chunk may be shorter). Bin 1 of block 0
+// is at nblk, so the scan there is the number of branches without a step.
+__device__ __forceinline__ ull len_slot(ull blk, int bin, ull nblk) {
+    if (bin == 0) return blk;
+    const ull b0 = blk / LEN_CHUNK_BLOCKS * LEN_CHUNK_BLOCKS;
+    const ull w = nblk - b0 < LEN_CHUNK_BLOCKS ? nblk - b0 : LEN_CHUNK_BLOCKS;
+    return nblk + b0 * (LEN_BINS - 1) + (ull)(bin - 1) * w + (blk - b0);
+}
 
 __global__ __launch_bounds__(BLOCK) void k_len_hist(ull nb, const ull* cnt, ull nblk, uint32_t* hist) {
     __shared__ uint32_t h[LEN_BINS];
@@ -1309,13 +1325,13 @@ __global__ __launch_bounds__(BLOCK) void k_len_hist(ull nb, const ull* cnt, ull 
         (void)wave_bin_claim(j < nb ? len_bin(cnt, j) : -1, h);
     }
     __syncthreads();
-    hist[(ull)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+    hist[len_slot(blockIdx.x, threadIdx.x, nblk)] = h[threadIdx.x];
 }
 
 __global__ __launch_bounds__(BLOCK) void k_len_scatter(ull nb, const ull* cnt, ull nblk, const uint32_t* first,
                                                        uint32_t* order) {
     __shared__ uint32_t h[LEN_BINS];
-    h[threadIdx.x] = first[(ull)threadIdx.x * nblk + blockIdx.x];
+    h[threadIdx.x] = first[len_slot(blockIdx.x, threadIdx.x, nblk)];
     __syncthreads();
     const ull base = (ull)blockIdx.x * CS_PER_BLOCK;
     for (int k = 0; k < CS_V; k++) {
@@ -1613,7 +1629,7 @@ struct dmtz_ctx {
     size_t as_first_cap = 0;
     uint32_t* d_as_off32 = nullptr;  // 32-bit branch offsets and per-step cubes for the dirty test
     uint32_t* d_as_cube = nullptr;
-    uint32_t* d_as_order = nullptr;  // branches by step count (first nb), sort input (second nb)
+    uint32_t* d_as_order = nullptr;  // branches in check order (first nb), sort input (second nb)
     uint64_t* d_as_sd = nullptr;     // per-step dirty bits
     uint32_t* d_as_fails = nullptr;  // per-branch bit: the last check found a mismatch
     size_t as_off32_cap = 0, as_cube_cap = 0, as_order_cap = 0, as_sd_cap = 0, as_fails_cap = 0;
@@ -2402,6 +2418,10 @@ struct Driver {
         as_full = true;
         as_n0 = as_compact ? (ctx->h_as_total[1] & 0xffffffffull) : 0;
         if (as_compact && nfcf) CK(hipMemsetAsync(ctx->d_walk, 0xFF, sizeof(int64_t) * 4 * nfcf, ctx->stream));
+        const char* lg = std::getenv("DMTZ_LOG");
+        if (lg && lg[0] == '1')
+            std::fprintf(stderr, "dmtz ascending paths: %llu branches, %llu without a step, %llu steps\n",
+                         (unsigned long long)(2 * nfcf), (unsigned long long)as_n0, (unsigned long long)as_total);
         return 0;
     }
 
@@ -2508,7 +2528,7 @@ struct Driver {
         int rc = grow_buf(ctx, ctx->d_as_steps, ctx->as_steps_cap, total);
         if (rc) return rc;
         as_compact = total < ((ull)1 << 32) && nb < ((ull)1 << 31);
-        if (as_compact) {  // branch order by path length (for the fill and the checks)
+        if (as_compact) {  // check order: by region, then path length (for the fill and the checks)
             rc = grow_buf(ctx, ctx->d_as_order, ctx->as_order_cap, nb);
             if (rc) return rc;
             const ull nblk = (nb + CS_PER_BLOCK - 1) / CS_PER_BLOCK, m = nblk * LEN_BINS;
