@@ -1441,6 +1441,39 @@ __global__ __launch_bounds__(BLOCK) void k_as_combine(const int64_t* res, ull n,
     out.push_block(has, a, b);
 }
 
+// k_as_combine over the failure bits (bit j set when branch j failed; the bits mirror
+// res[2 * j] >= 0): one thread per 32 faces, the 64 branch bits of one word pair, so that a face
+// without a failing branch costs one bit instead of a thread. Faces whose bits are set are read
+// twice: once to count what the thread appends, once to append it.
+__global__ __launch_bounds__(BLOCK) void k_as_combine_bits(const int64_t* res, ull n, PathList out,
+                                                           const uint32_t* fails) {
+    const ull t = (ull)blockIdx.x * BLOCK + threadIdx.x;
+    const ull f0 = t * 32;
+    uint64_t fm = 0;  // bit 2i: face f0 + i has a failing branch
+    if (f0 < n) {
+        uint64_t bits = (uint64_t)fails[2 * t] | ((uint64_t)fails[2 * t + 1] << 32);
+        const ull nf = n - f0 < 32 ? n - f0 : 32;
+        if (nf < 32) bits &= (1ull << (2 * nf)) - 1ull;
+        fm = (bits | (bits >> 1)) & 0x5555555555555555ull;
+    }
+    uint64_t hm = 0;  // faces that append
+    for (uint64_t m = fm; m; m &= m - 1) {
+        const int b = __builtin_ctzll(m);
+        const int64_t* p = res + 4 * (f0 + (ull)(b >> 1));
+        if (p[0] >= 0 || p[2] >= 0) hm |= 1ull << b;
+    }
+    ull at = block_alloc(out.ctr, __popcll(hm));
+    for (uint64_t m = hm; m; m &= m - 1) {
+        const int b = __builtin_ctzll(m);
+        const int64_t* p = res + 4 * (f0 + (ull)(b >> 1));
+        if (at < out.cap) {
+            out.id[at] = p[0] >= 0 ? p[0] : p[2];
+            out.rec[at] = p[0] >= 0 ? p[1] : p[3];
+        }
+        at++;
+    }
+}
+
 // Endpoint labels of f (get_orginal_saddle_min_pairs / _max_pairs, main_3d.cu:4613-4759), one
 // thread per (f-critical edge, endpoint) and per (f-critical face, cofacet); ascending labels
 // are indexed by face (resolution of the shadowed index at main_3d.cu:4750).
@@ -2306,9 +2339,12 @@ struct Driver {
             CK(hipGetLastError());
         }
         const ull nstarts = conn && as_gfaces ? n_gfaces : nfcf;
-        if (nstarts) {
-            k_as_combine<<<nblocks(nstarts), BLOCK, 0, ctx->stream>>>(ctx->d_walk, nstarts, path_list(1),
-                                                                      pre && as_compact ? ctx->d_as_fails : nullptr);
+        if (nstarts && pre && as_compact) {
+            k_as_combine_bits<<<nblocks((nstarts + 31) / 32), BLOCK, 0, ctx->stream>>>(ctx->d_walk, nstarts,
+                                                                                      path_list(1), ctx->d_as_fails);
+            CK(hipGetLastError());
+        } else if (nstarts) {
+            k_as_combine<<<nblocks(nstarts), BLOCK, 0, ctx->stream>>>(ctx->d_walk, nstarts, path_list(1), nullptr);
             CK(hipGetLastError());
         }
         if (nfce && pre) {
