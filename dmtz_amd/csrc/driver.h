@@ -375,26 +375,47 @@ __global__ __launch_bounds__(BLOCK) void k_critf_spread(const StarCode* cf, Dims
 // per-vertex mismatch masks kept with the g star codes (16 bytes per vertex). Waves without a
 // mismatch skip the list appends.
 // A tile whose count of nonzero masks is zero is skipped without reading its masks; tiles_read
-// counts the tiles whose masks were read.
+// counts the tiles whose masks were read. A grid of at most DETECT_BLOCKS blocks strides over
+// the tiles: block b takes tiles b, b + grid, ..., reads the counts of 64 of them at once and
+// visits only the nonzero ones, so a late sweep with a few mismatches does not dispatch a block
+// per tile of the grid.
+constexpr unsigned DETECT_BLOCKS = 2048;
+
 __global__ __launch_bounds__(BLOCK) void k_detect(const CritMask* mm, i64 n, const uint32_t* tile_nz, Lists4 out,
                                                   ull* ctr, ull* tiles_read) {
-    const i64 base = (i64)blockIdx.x * EMIT_TILE;
-    const bool any = tile_nz[blockIdx.x] != 0;
-    if (any && threadIdx.x == 0) atomicAdd(tiles_read, 1ull);
-    uint64_t lo[EMIT_V];
-    uint32_t hi[EMIT_V];
+    __shared__ uint64_t nz_mask;
+    const i64 ntiles = (n + EMIT_TILE - 1) / EMIT_TILE, stride = (i64)gridDim.x;
+    for (i64 first = blockIdx.x; first < ntiles; first += 64 * stride) {
+        if (threadIdx.x < 64) {
+            const i64 tile = first + (i64)threadIdx.x * stride;
+            const uint64_t m = __ballot(tile < ntiles && tile_nz[tile] != 0);
+            if (threadIdx.x == 0) {
+                nz_mask = m;
+                if (m) atomicAdd(tiles_read, (ull)__popcll(m));
+            }
+        }
+        __syncthreads();
+        uint64_t todo = nz_mask;
+        __syncthreads();  // nz_mask is rewritten by the next group
+        while (todo) {
+            const i64 base = (first + (i64)__builtin_ctzll(todo) * stride) * EMIT_TILE;
+            todo &= todo - 1;
+            uint64_t lo[EMIT_V];
+            uint32_t hi[EMIT_V];
 #pragma unroll
-    for (int k = 0; k < EMIT_V; k++) {
-        const i64 v = base + (i64)k * BLOCK + threadIdx.x;
-        lo[k] = 0;
-        hi[k] = 0;
-        if (any && v < n) {
-            const CritMask m = mm[v];
-            lo[k] = m.lo;
-            hi[k] = m.hi;
+            for (int k = 0; k < EMIT_V; k++) {
+                const i64 v = base + (i64)k * BLOCK + threadIdx.x;
+                lo[k] = 0;
+                hi[k] = 0;
+                if (v < n) {
+                    const CritMask m = mm[v];
+                    lo[k] = m.lo;
+                    hi[k] = m.hi;
+                }
+            }
+            block_emit(base, lo, hi, out, ctr);
         }
     }
-    block_emit(base, lo, hi, out, ctr);
 }
 
 // f-critical cells owned by each vertex in f; type_mask selects which types are listed.
@@ -1947,7 +1968,8 @@ struct Driver {
                                                         ctr_bit(K_MIN + 3) | ctr_bit(K_DTILES));
         CK(hipGetLastError());
         CK(hipEventRecord(ctx->ev0, ctx->stream));
-        k_detect<<<(unsigned)((c.N + EMIT_TILE - 1) / EMIT_TILE), BLOCK, 0, ctx->stream>>>(
+        const ull ntiles = (ull)((c.N + EMIT_TILE - 1) / EMIT_TILE);
+        k_detect<<<(unsigned)(ntiles < DETECT_BLOCKS ? ntiles : DETECT_BLOCKS), BLOCK, 0, ctx->stream>>>(
             ctx->d_mm, c.N, ctx->d_tile_nz, rec_lists(ctx), ctx->d_ctr + K_MIN, ctx->d_ctr + K_DTILES);
         CK(hipGetLastError());
         CK(hipEventRecord(ctx->ev1, ctx->stream));
@@ -1958,10 +1980,9 @@ struct Driver {
         // bytes the kernel must move: the tile counts, the masks of the tiles with a mismatch,
         // and one 8-byte record written per mismatch emitted
         ull m = ctx->h_ctr[0] + ctx->h_ctr[1] + ctx->h_ctr[2] + ctx->h_ctr[3];
-        const double ntiles = (double)((c.N + EMIT_TILE - 1) / EMIT_TILE);
         ctx->det_launches += 1;
         ctx->det_ms += ms;
-        ctx->det_bytes += 4.0 * ntiles + (double)ctx->h_ctr[K_DTILES] * (double)EMIT_TILE * sizeof(CritMask) +
+        ctx->det_bytes += 4.0 * (double)ntiles + (double)ctx->h_ctr[K_DTILES] * (double)EMIT_TILE * sizeof(CritMask) +
                           8.0 * (double)m;
         ctx->det_voxels += (double)c.N;
         return 0;
