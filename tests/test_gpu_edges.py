@@ -6,7 +6,10 @@
     every flag mode of the parity corpus;
   - integer fields with heavy ties in both f and g;
   - a ragged 84 M-vertex grid (40 x 700 x 3001), too large for the oracle: zero mismatches, every
-    value within eps, and f's critical cells in the corrected field.
+    value within eps, and f's critical cells in the corrected field;
+  - a grid of more than 2048 x 64 detection tiles of 2048 vertices (268.7 M vertices), whose one
+    planted mismatch lies in a tile that the detection grid reaches in its second group of 64
+    tiles per block.
 """
 import numpy as np
 import pytest
@@ -88,3 +91,26 @@ def test_ragged_84m_grid(dev):
     assert cf == cg
     for a, b in zip(_critical_cells(lof, hif, shape), _critical_cells(log_, hig, shape)):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.timeout(600)
+def test_detection_past_the_first_group_of_tiles(dev):
+    shape = (65600, 64, 64)  # nz, ny, nx: 268,697,600 vertices, 131,200 tiles > 2048 * 64
+    n = int(np.prod(shape))
+    assert n > 2048 * 64 * 2048
+    # f increases with the vertex index (float32 ties are broken by index): no critical cell but
+    # the minimum. g lowers one vertex of the last tiles far below its neighbours.
+    f = np.arange(n, dtype=np.int64).astype(np.float32).reshape(shape)
+    g = f.copy()
+    z, y, x = shape[0] - 2, 30, 30
+    v0 = (z * shape[1] + y) * shape[2] + x
+    assert v0 // 2048 >= 2048 * 64
+    eps = 1e5
+    g[z, y, x] = f[z, y, x] - np.float32(5e4)
+    fl = dmtz_amd.CorrectionFlags(preserve_vpath=0, preserve_geometry=0, finish=1)
+    print("detection past the first group", shape, flush=True)
+    r = dmtz_amd.correct(f, g, eps, float(n - 1), fl, ctx=dev)
+    assert r.status == 0 and r.counts == (0,) * 8
+    assert r.sweeps >= 2 and r.trajectory[0].any()  # the planted mismatch was found
+    assert np.abs(r.g.astype(np.float64) - f).max() <= eps
+    assert dmtz_amd.classify(r.g, ctx=dev) == dmtz_amd.classify(f, ctx=dev)
