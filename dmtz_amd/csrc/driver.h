@@ -325,7 +325,10 @@ __global__ __launch_bounds__(BLOCK, CODE_BLOCKS_PER_CU) void k_code_list_g(const
 }
 
 // f-critical status of the local cells of every vertex (once per run), in two passes: each
-// vertex writes the critical cells it owns, then spreads them to the cells' other vertices.
+// vertex writes the critical cells it owns (k_critf_own), then gathers the ones its 14
+// neighbours own that contain it (k_critf_gather: 15 coalesced 16-byte reads and one store per
+// vertex; the reverse, each owner spreading its cells with atomics, cost 13 ms at 512^3 for the
+// atomics' traffic to memory).
 __global__ __launch_bounds__(BLOCK) void k_critf_own(const StarCode* cf, i64 n, CritMask* out) {
     const i64 v = (i64)blockIdx.x * BLOCK + threadIdx.x;
     if (v >= n) return;
@@ -362,13 +365,24 @@ __global__ __launch_bounds__(BLOCK) void k_crit_count(const StarCode* code, i64 
     }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_critf_spread(const StarCode* cf, Dims d, i64 n, CritMask* out) {
+__global__ __launch_bounds__(BLOCK) void k_critf_gather(const CritMask* own, Dims d, i64 n, CritMask* out) {
     const i64 v = (i64)blockIdx.x * BLOCK + threadIdx.x;
     if (v >= n) return;
-    crit_scatter_spread(d, cf[v], v, [&](i64 w, int lc) {
-        if (lc < 51) atomicOr((ull*)&out[w].lo, 1ull << lc);
-        else atomicOr(&out[w].hi, 1u << (lc - 51));
-    });
+    uint64_t lo;
+    uint32_t hi;
+    crit_gather(
+        d, v,
+        [&](i64 u, uint64_t& a, uint32_t& b) {
+            const CritMask m = own[u];
+            a = m.lo;
+            b = m.hi;
+        },
+        lo, hi);
+    CritMask m;
+    m.lo = lo;
+    m.hi = hi;
+    m.pad = 0;
+    out[v] = m;
 }
 
 // Dense mismatch detection: every vertex lists the mismatched cells it owns in g, from the
@@ -2246,9 +2260,10 @@ struct Driver {
     int f_codes() {
         int rc = codes(f, ctx->d_code_f);
         if (rc) return rc;
-        k_critf_own<<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(ctx->d_code_f, c.N, ctx->d_critf);
+        // the owned cells go to the mismatch masks first, which the g codes overwrite next
+        k_critf_own<<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(ctx->d_code_f, c.N, ctx->d_mm);
         CK(hipGetLastError());
-        k_critf_spread<<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(ctx->d_code_f, d, c.N, ctx->d_critf);
+        k_critf_gather<<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(ctx->d_mm, d, c.N, ctx->d_critf);
         CK(hipGetLastError());
         return 0;
     }

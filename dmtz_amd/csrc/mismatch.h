@@ -136,7 +136,7 @@ DMTZ_HD void cell_at_other_vertices(const Dims& d, i64 u, int lc, Fn fn) {
 // The critical status in f of every local cell, written the other way round from local_crit: a
 // vertex's mask first holds the critical cells it owns (code_masks of its code); then every
 // vertex spreads its owned critical cells to their other vertices: spread(w, lc) adds cell lc
-// at w.
+// at w. (The device gathers instead, crit_gather below; the host check compares all three.)
 template <class Spread>
 DMTZ_HD void crit_scatter_spread(const Dims& d, const StarCode& code, i64 v, Spread spread) {
     uint64_t c_lo, i_lo;
@@ -153,6 +153,91 @@ DMTZ_HD void crit_scatter_spread(const Dims& d, const StarCode& code, i64 v, Spr
         const int lc = LC_TET + __builtin_ctz(h);
         h &= h - 1;
         cell_at_other_vertices(d, v, lc, spread);
+    }
+}
+
+// The gather form of cell_at_other_vertices. For a vertex u and its neighbour w in slot s
+// (w = u + DIR[s]): the local cells of u that contain w (cont_lo[s]: bits below LC_TET,
+// cont_hi[s]: tet bits) and the local index of each of them in w's star (at_w[s][lc], -1 for a
+// cell without w).
+struct SpreadTables {
+    uint64_t cont_lo[14];
+    uint32_t cont_hi[14];
+    int8_t at_w[14][LC_COUNT];
+};
+
+namespace detail {
+constexpr int c_slot_from(int from, int to) {
+    const int dx = DIR[to][0] - DIR[from][0], dy = DIR[to][1] - DIR[from][1], dz = DIR[to][2] - DIR[from][2];
+    return (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1)
+               ? -1
+               : ST.slot_of[(dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)];
+}
+constexpr int c_tet_of(int a, int b, int c) {
+    const uint32_t key = (1u << a) | (1u << b) | (1u << c);
+    int r = -1;
+    for (int k = 0; k < 24; k++)
+        if (((1u << ST.tet_s[k][0]) | (1u << ST.tet_s[k][1]) | (1u << ST.tet_s[k][2])) == key) r = k;
+    return r;
+}
+constexpr void c_put(SpreadTables& t, int s, int lc, int at) {
+    if (lc < LC_TET) t.cont_lo[s] |= 1ull << lc;
+    else t.cont_hi[s] |= 1u << (lc - LC_TET);
+    t.at_w[s][lc] = (int8_t)at;
+}
+constexpr SpreadTables make_spread_tables() {
+    SpreadTables t{};
+    for (int s = 0; s < 14; s++) {
+        t.cont_lo[s] = 0;
+        t.cont_hi[s] = 0;
+        for (int lc = 0; lc < LC_COUNT; lc++) t.at_w[s][lc] = -1;
+    }
+    for (int e = 0; e < 14; e++) c_put(t, e, LC_EDGE + e, LC_EDGE + (e ^ 1));
+    for (int tr = 0; tr < 36; tr++) {
+        const int i = ST.tri_i[tr], j = ST.tri_j[tr];
+        c_put(t, i, LC_TRI + tr, LC_TRI + ST.tri_of[i ^ 1][c_slot_from(i, j)]);
+        c_put(t, j, LC_TRI + tr, LC_TRI + ST.tri_of[j ^ 1][c_slot_from(j, i)]);
+    }
+    for (int q = 0; q < 24; q++) {
+        const int s0 = ST.tet_s[q][0], s1 = ST.tet_s[q][1], s2 = ST.tet_s[q][2];
+        c_put(t, s0, LC_TET + q, LC_TET + c_tet_of(s0 ^ 1, c_slot_from(s0, s1), c_slot_from(s0, s2)));
+        c_put(t, s1, LC_TET + q, LC_TET + c_tet_of(s1 ^ 1, c_slot_from(s1, s0), c_slot_from(s1, s2)));
+        c_put(t, s2, LC_TET + q, LC_TET + c_tet_of(s2 ^ 1, c_slot_from(s2, s0), c_slot_from(s2, s1)));
+    }
+    return t;
+}
+}  // namespace detail
+
+constexpr SpreadTables SP = detail::make_spread_tables();
+
+// The critical status in f of every local cell of v, gathered: own(u, lo, hi) gives the critical
+// cells vertex u owns (code_masks of its code, restricted to its members). v's own bits, plus, for
+// each neighbour u = v + DIR[s] in the grid, u's owned critical cells that contain v, moved to
+// their local indices at v (v is in slot s ^ 1 of u's star). Every cell of v's star is owned by
+// v or by one of its 14 neighbours, so this equals local_crit and crit_scatter_spread, with
+// plain reads of the neighbours' masks instead of lookups or atomics.
+template <class Own>
+DMTZ_HD void crit_gather(const Dims& d, i64 v, Own own, uint64_t& lo, uint32_t& hi) {
+    own(v, lo, hi);
+    i64 x, y, z;
+    vxyz(d, v, x, y, z);
+    for (int s = 0; s < 14; s++) {
+        const i64 nx = x + DIR[s][0], ny = y + DIR[s][1], nz = z + DIR[s][2];
+        if (!inb(d, nx, ny, nz)) continue;
+        uint64_t ulo;
+        uint32_t uhi;
+        own(vidx(d, nx, ny, nz), ulo, uhi);
+        const int r = s ^ 1;
+        uint64_t m = ulo & SP.cont_lo[r];
+        uint32_t h = uhi & SP.cont_hi[r];
+        while (m | h) {
+            const int c = m ? __builtin_ctzll(m) : LC_TET + __builtin_ctz(h);
+            if (m) m &= m - 1;
+            else h &= h - 1;
+            const int w = SP.at_w[r][c];
+            if (w < LC_TET) lo |= 1ull << w;
+            else hi |= 1u << (w - LC_TET);
+        }
     }
 }
 

@@ -288,9 +288,10 @@ extern "C" void hc_decode(i64 W, i64 H, i64 D, const i64* tris, i64 nt, const i6
     for (i64 k = 0; k < nq; k++) tet_verts(d, tets[k], o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
 }
 
-// f-critical masks of every vertex two ways: gathered (local_crit, lookups at each cell's owner)
-// and scattered (each owner spreads its critical cells to the cells' other vertices). Returns the
-// number of vertices whose masks differ.
+// f-critical masks of every vertex three ways: by lookups at each cell's owner (local_crit),
+// scattered (each owner spreads its critical cells to the cells' other vertices) and gathered
+// from the neighbours' owned critical cells (crit_gather). Returns the number of vertices whose
+// scattered or gathered masks differ from local_crit's.
 template <class T>
 static i64 critf_two_ways_impl(const T* f, i64 W, i64 H, i64 D) {
     Dims d{W, H, D};
@@ -306,6 +307,8 @@ static i64 critf_two_ways_impl(const T* f, i64 W, i64 H, i64 D) {
         slo[v] = lo & ilo;
         shi[v] = hi & ihi;
     }
+    const std::vector<uint64_t> olo = slo;
+    const std::vector<uint32_t> ohi = shi;
     for (i64 v = 0; v < c.N; v++)
         crit_scatter_spread(d, cf[v], v, [&](i64 w, int lc) {
             if (lc < 51) slo[w] |= 1ull << lc;
@@ -313,10 +316,11 @@ static i64 critf_two_ways_impl(const T* f, i64 W, i64 H, i64 D) {
         });
     i64 bad = 0;
     for (i64 v = 0; v < c.N; v++) {
-        uint64_t lo;
-        uint32_t hi;
+        uint64_t lo, glo;
+        uint32_t hi, ghi;
         local_crit(F, v, lo, hi);
-        bad += (lo != slo[v]) || (hi != shi[v]);
+        crit_gather(d, v, [&](i64 u, uint64_t& a, uint32_t& b) { a = olo[u]; b = ohi[u]; }, glo, ghi);
+        bad += (lo != slo[v]) || (hi != shi[v]) || (lo != glo) || (hi != ghi);
     }
     return bad;
 }

@@ -170,8 +170,9 @@ def test_decoders_match_oracle_on_every_id(hc, oracle):
 
 
 def test_critical_masks_scattered_equal_gathered(hc):
-    """The f-critical masks of all local cells, spread from each cell's owner (the device's
-    k_critf) equal the masks gathered by pairing lookups at the owners (local_crit)."""
+    """The f-critical masks of all local cells, spread from each cell's owner and gathered from
+    the neighbours' owned critical cells (the device's k_critf_gather), equal the masks found by
+    pairing lookups at the owners (local_crit)."""
     for name in ("hc_critf_two_ways_f32", "hc_critf_two_ways_f64"):
         getattr(hc, name).argtypes = [P, I, I, I]
         getattr(hc, name).restype = I
