@@ -327,8 +327,8 @@ __global__ __launch_bounds__(BLOCK, CODE_BLOCKS_PER_CU) void k_code_list_g(const
 // f-critical status of the local cells of every vertex (once per run), in two passes: each
 // vertex writes the critical cells it owns (k_critf_own), then gathers the ones its 14
 // neighbours own that contain it (k_critf_gather: 15 coalesced 16-byte reads and one store per
-// vertex; the reverse, each owner spreading its cells with atomics, cost 13 ms at 512^3 for the
-// atomics' traffic to memory).
+// vertex; the reverse, each owner spreading its cells with atomics, wrote five times the masks'
+// size to memory and took twice as long).
 __global__ __launch_bounds__(BLOCK) void k_critf_own(const StarCode* cf, i64 n, CritMask* out) {
     const i64 v = (i64)blockIdx.x * BLOCK + threadIdx.x;
     if (v >= n) return;
