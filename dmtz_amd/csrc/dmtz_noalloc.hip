@@ -88,7 +88,7 @@ int dmtz_classify_noalloc_f32(dmtz_ctx* ctx, const float* d_field, int64_t nx, i
     hipStream_t s = ctx->stream;
     k_clear_counters<<<1, 64, 0, s>>>(ctr);
     CK(hipGetLastError());
-    k_code<float><<<nblocks(N), BLOCK, 0, s>>>(d_field, d, c, code);
+    k_code<float><<<nblocks(N), BLOCK, 0, s>>>(d_field, d, c, code, nullptr);
     CK(hipGetLastError());
     k_crit_count<<<nblocks(N), BLOCK, 0, s>>>(code, N, ctr);
     CK(hipGetLastError());

@@ -237,13 +237,22 @@ struct __align__(16) CritMask {
 // the code kernels at 128 VGPRs: four waves per SIMD to hide the loads of the star values.
 constexpr int CODE_BLOCKS_PER_CU = 4;
 
+// own (nullable) receives the critical cells each vertex owns, from the lower star itself (the
+// first pass of the f-critical masks, without reading the codes back).
 template <class T>
-__global__ __launch_bounds__(BLOCK, CODE_BLOCKS_PER_CU) void k_code(const T* fld, Dims d, Counts c, StarCode* code) {
+__global__ __launch_bounds__(BLOCK, CODE_BLOCKS_PER_CU) void k_code(const T* fld, Dims d, Counts c, StarCode* code,
+                                                                    CritMask* own) {
     const i64 v = (i64)blockIdx.x * BLOCK + threadIdx.x;
     if (v >= c.N) return;
     LowerStar ls;
     star_of(fld, d, v, ls);
     encode_star(ls, code[v]);
+    if (own) {
+        CritMask m;
+        crit_mask(ls, m.lo, m.hi);
+        m.pad = 0;
+        own[v] = m;
+    }
 }
 
 // g star code of vertex v and its mismatch mask: the cells v owns in g whose critical status in
@@ -325,10 +334,10 @@ __global__ __launch_bounds__(BLOCK, CODE_BLOCKS_PER_CU) void k_code_list_g(const
 }
 
 // f-critical status of the local cells of every vertex (once per run), in two passes: each
-// vertex writes the critical cells it owns (k_critf_own), then gathers the ones its 14
-// neighbours own that contain it (k_critf_gather: 15 coalesced 16-byte reads and one store per
-// vertex; the reverse, each owner spreading its cells with atomics, wrote five times the masks'
-// size to memory and took twice as long).
+// vertex writes the critical cells it owns (k_code's own output, or k_critf_own from the codes),
+// then gathers the ones its 14 neighbours own that contain it (k_critf_gather: 15 coalesced
+// 16-byte reads and one store per vertex; the reverse, each owner spreading its cells with
+// atomics, wrote five times the masks' size to memory and took twice as long).
 __global__ __launch_bounds__(BLOCK) void k_critf_own(const StarCode* cf, i64 n, CritMask* out) {
     const i64 v = (i64)blockIdx.x * BLOCK + threadIdx.x;
     if (v >= n) return;
@@ -2258,11 +2267,9 @@ struct Driver {
     // Star codes of f and the f-critical status of every vertex's local cells (the gradient of
     // f, main_3d.cu:6539-6564, done once per run).
     int f_codes() {
-        int rc = codes(f, ctx->d_code_f);
+        // the owned critical cells go to the mismatch masks first, which the g codes overwrite next
+        int rc = codes(f, ctx->d_code_f, ctx->d_mm);
         if (rc) return rc;
-        // the owned cells go to the mismatch masks first, which the g codes overwrite next
-        k_critf_own<<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(ctx->d_code_f, c.N, ctx->d_mm);
-        CK(hipGetLastError());
         k_critf_gather<<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(ctx->d_mm, d, c.N, ctx->d_critf);
         CK(hipGetLastError());
         return 0;
@@ -2280,8 +2287,8 @@ struct Driver {
     }
 
     // Star codes of every vertex of a field.
-    int codes(const T* fld, StarCode* out) {
-        k_code<T><<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(fld, d, c, out);
+    int codes(const T* fld, StarCode* out, CritMask* own = nullptr) {
+        k_code<T><<<nblocks(c.N), BLOCK, 0, ctx->stream>>>(fld, d, c, out, own);
         CK(hipGetLastError());
         return 0;
     }
@@ -3017,7 +3024,7 @@ int classify_host(dmtz_ctx* ctx, const T* fld, i64 nx, i64 ny, i64 nz, int64_t* 
     Dims d{nx, ny, nz};
     Counts c = make_counts(d);
     CK(hipMemcpyAsync(ctx->d_f, fld, sizeof(T) * (size_t)N, hipMemcpyHostToDevice, ctx->stream));
-    k_code<T><<<nblocks(N), BLOCK, 0, ctx->stream>>>((const T*)ctx->d_f, d, c, ctx->d_code_f);
+    k_code<T><<<nblocks(N), BLOCK, 0, ctx->stream>>>((const T*)ctx->d_f, d, c, ctx->d_code_f, nullptr);
     CK(hipGetLastError());
     CK(hipMemsetAsync(ctx->d_ctr, 0, sizeof(ull) * 4, ctx->stream));
     k_crit_count<<<nblocks(N), BLOCK, 0, ctx->stream>>>(ctx->d_code_f, N, ctx->d_ctr);
@@ -3137,7 +3144,7 @@ int gradient_host(dmtz_ctx* ctx, const T* fld, i64 nx, i64 ny, i64 nz, int64_t* 
     rc = ensure(ctx, c.N, sizeof(T), true);
     if (rc) return rc;
     CK(hipMemcpyAsync(ctx->d_f, fld, sizeof(T) * (size_t)c.N, hipMemcpyHostToDevice, ctx->stream));
-    k_code<T><<<nblocks(c.N), BLOCK, 0, ctx->stream>>>((const T*)ctx->d_f, d, c, ctx->d_code_f);
+    k_code<T><<<nblocks(c.N), BLOCK, 0, ctx->stream>>>((const T*)ctx->d_f, d, c, ctx->d_code_f, nullptr);
     CK(hipGetLastError());
     int64_t* outs[4] = {pv, pe, pf, pt};
     i64 sizes[4] = {c.N, c.E, c.F, c.T};
