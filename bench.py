@@ -263,13 +263,13 @@ def main():
     bytes_per_launch = ks["bytes"] / launches
     det_gbs = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     det_prof = committed_profile("k_detect_traffic.json", workload, args.seed)
-    kernel_roofline = dict(bound="hbm", kernel="k_detect (dense mismatch detection, once per sweep)",
+    kernel_roofline = dict(bound="hbm", kernel="k_detect (mismatch detection, once per sweep)",
                            achieved=round(det_gbs, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                            frac=round(det_gbs / HBM_PEAK_GBS, 5),
                            traffic=int(det_prof["traffic_bytes_per_launch"]) if det_prof else None,
                            bytes_per_launch=int(bytes_per_launch), avg_launch_ms=round(avg_ms, 4),
                            launches=int(ks["launches"]),
-                           basis="16-byte mismatch mask per voxel read + 8 B per record written")
+                           basis="4 B count per 2048-voxel tile + the 16-byte mismatch masks of the tiles with a mismatch read + 8 B per record written")
 
     other = {}
     for r in [x for x in args.other_rel.split(",") if x.strip()]:

@@ -229,9 +229,10 @@ int dmtz_phase_times(const dmtz_ctx* ctx, double* out6);
 
 /* Live statistics of the dense mismatch-detection kernel (the per-sweep classification of
  * main_3d.cu:6864-6874 + 6903-6909) accumulated over calls on ctx, measured with HIP events on
- * the context stream: out4 = {launches, total milliseconds, bytes the kernel must move (the
- * 16-byte mismatch mask of every voxel read + 8 bytes written per emitted mismatch record),
- * voxels classified}. reset != 0 zeroes them. */
+ * the context stream: out4 = {launches, total milliseconds, bytes the kernel must move (a 4-byte
+ * count per 2048-voxel tile, the 16-byte mismatch masks of the tiles whose count is nonzero,
+ * and 8 bytes written per emitted mismatch record), voxels classified}. reset != 0 zeroes
+ * them. */
 int dmtz_kernel_stats(dmtz_ctx* ctx, double* out4, int reset);
 
 /* Outcome of the last correction on ctx beyond the counts: out4 = {stalls, vertices changed by
